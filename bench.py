@@ -1,0 +1,240 @@
+"""Benchmark of the RHS + RK hot path on one MI355X.
+
+Workload (default): the 3D spherical blast with AMR (ideal gas, gamma 1.4, unit
+cube, base 32^3 refined two levels by a density-gradient criterion with 2:1
+balance, re-adapted every 10 steps).  The blast is evolved with host AMR for
+--evolve steps (untimed) so the mesh has the size and hanging-face mix of the
+running case; then K SSP-RK3 steps are timed on that mesh with every input
+resident in HBM.  A step = RK(0) + 3 x (faceinterp, RHS, RK(k)); the metric is
+cell-updates/s = leaves x RK stages / second.
+
+The plain FP64 form gives `value`; the mixed FP32-flux / FP64-accumulate form is
+timed on the same state and reported beside it with its relative L1 error
+against the plain form after --acc-steps steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["recon", "grad", "face", "gather_rk"]
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="blast", choices=["blast", "sod", "cavitation", "tgv", "vortex"])
+    ap.add_argument("--evolve", type=int, default=120, help="untimed AMR steps before timing (blast)")
+    ap.add_argument("--acc-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg, quiet=False):
+    if not quiet:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes(kernel, m, neq, rs, ns_visc, first_stage):
+    """Algorithmic HBM bytes of one launch: each input read once, each output
+    written once (per-unit figures in the design notes)."""
+    N, F, Q, Fb = m["N"], m["F"], m["Q"], m["Fb"]
+    if kernel == "recon":
+        return 8 * neq * N + 4 * N + 24 * N + 16 * Q + 6 * neq * rs * N
+    if kernel == "grad":
+        return 8 * 3 * N + 4 * N + 24 * N + 16 * Q + 9 * rs * N
+    if kernel == "face":
+        b = 12 * F + neq * rs * (2 * F - Fb) + neq * rs * F
+        if ns_visc:
+            b += 9 * rs * (2 * F - Fb)
+        return b
+    if kernel == "gather_rk":
+        b = 4 * N + 24 * N + neq * rs * F + 8 * neq * N * (1 if first_stage else 2) + 8 * neq * N
+        if first_stage:
+            b += 8 * neq * N
+        return b
+    raise KeyError(kernel)
+
+
+def build_workload(name, evolve, quiet):
+    import fv_140823_amd as fv
+    from fv_140823_amd import configs, driver
+    if name == "blast":
+        F, p, meta = configs.blast(5, 2)
+        g = fv.Solver.from_forest(F, p)
+        t0 = time.perf_counter()
+        flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, base_level=6)
+        st = driver.evolve(F, g, evolve, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"],
+                           log=lambda s: log(s, quiet))
+        # final adaptation so the timed mesh matches the state
+        driver.remesh(F, g, flags, meta["max_level"])
+        meta = dict(meta, evolved_steps=evolve, evolved_time=st["time"],
+                    evolve_wall_s=time.perf_counter() - t0, remesh_count=len(st["remesh"]) + 1)
+        F.set_state(g.get_state())
+        g.close()
+        return F, p, meta
+    builder = getattr(configs, name)
+    F, p, meta = builder()
+    return F, p, meta
+
+
+def mesh_facts(F):
+    import fv_140823_amd as fv
+    fl = fv.FlatMesh(F.run)
+    A = fl.arrays()
+    Fb = int(((A["face_meta"] >> 6) & 3 != 0).sum())
+    _, lev = F.leaf_info()
+    return dict(N=fl.ncell, F=fl.nface, Q=int(fl.mesh.nquad), Fb=Fb,
+                levels={int(l): int(c) for l, c in zip(*np.unique(lev, return_counts=True))})
+
+
+def time_form(F, p, steps, warmup, dt):
+    import torch
+
+    import fv_140823_amd as fv
+    g = fv.Solver.from_forest(F, p)
+    if warmup:
+        g.steps(warmup, dt, 3)
+    g.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.steps(steps, dt, 3)
+    g.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-kernel durations from HIP events recorded on the context's stream
+    ev_total, kms = g.time_steps(max(3, steps // 3), dt, 3)
+    g.close()
+    return wall, kms
+
+
+def l1_rel_density(F, p_plain, p_mixed, steps, dt):
+    import fv_140823_amd as fv
+    out = {}
+    for tag, p in (("plain", p_plain), ("mixed", p_mixed)):
+        g = fv.Solver.from_forest(F, p)
+        g.steps(steps, dt, 3)
+        out[tag] = g.get_state()
+        g.close()
+    _, lev = F.leaf_info()
+    V = 8.0 ** (-(lev - 1).astype(float))
+    a, b = out["mixed"], out["plain"]
+    rho = float((np.abs(a[0] - b[0]) * V).sum() / (np.abs(b[0]) * V).sum())
+    from fv_140823_amd.configs import cons_to_prim
+    pa, pb = cons_to_prim(a)[4], cons_to_prim(b)[4]
+    pres = float((np.abs(pa - pb) * V).sum() / (np.abs(pb) * V).sum())
+    return dict(density=rho, pressure=pres, steps=steps)
+
+
+def cpu_baseline(F, p, dt, budget_s, quiet):
+    import oracle
+    N = F.nleaves
+    S0 = F.get_state()
+    t0 = time.perf_counter()
+    oracle.faceinterp(F, p)
+    oracle.rhs(F, p)
+    one = time.perf_counter() - t0
+    stages = max(1, min(9, int(budget_s / max(one, 1e-3))))
+    oracle.rk(F, 0, dt)
+    t0 = time.perf_counter()
+    irk = [1, 2, 3]
+    for k in range(stages):
+        if p.ischeme == 2:
+            oracle.faceinterp(F, p)
+        if p.ns:
+            oracle.gradcalc(F, p)
+        oracle.rhs(F, p)
+        oracle.rk(F, irk[k % 3], dt)
+    wall = time.perf_counter() - t0
+    F.set_state(S0)
+    log(f"cpu oracle: {stages} stages on {N} leaves in {wall:.2f}s", quiet)
+    return dict(value=N * stages / wall, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"{stages} RK stages (faceinterp+RHS+RK) of the oracle on the same {N}-leaf mesh, 1 thread")
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the path has no "
+              "multi-GPU halo exchange; see the sharding notes in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import fv_140823_amd as fv
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    log(f"building workload {a.workload}", a.quiet)
+    F, p, meta = build_workload(a.workload, a.evolve, a.quiet)
+    mf = mesh_facts(F)
+    log(f"mesh: {mf}", a.quiet)
+    neq = fv.neq_of(p.model)
+    g = fv.Solver.from_forest(F, p)
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    pm = type(p).from_buffer_copy(p)
+    pm.precision = fv.FVG_MIXED
+    log(f"timing plain FP64: {a.steps} steps after {a.warmup} warm-up", a.quiet)
+    wall_p, k_p = time_form(F, p, a.steps, a.warmup, dt)
+    log(f"timing mixed: {a.steps} steps", a.quiet)
+    wall_m, k_m = time_form(F, pm, a.steps, a.warmup, dt)
+    stages = 3
+    cu = mf["N"] * stages * a.steps
+    val_p, val_m = cu / wall_p, cu / wall_m
+    log(f"plain {val_p:.4g} cell-upd/s, mixed {val_m:.4g} cell-upd/s", a.quiet)
+    # dominant kernel of the plain form (largest time per step)
+    per_step = {KERNELS[i]: k_p[i] * (3 if i < 4 else 1) for i in range(4) if k_p[i] > 0}
+    dom = max(per_step, key=per_step.get)
+    di = KERNELS.index(dom)
+    nb = alg_bytes(dom, mf, neq, 8, p.ns, False)
+    achieved = nb / (k_p[di] * 1e-3) / 1e9
+    roof = dict(bound="hbm", kernel=f"k_{dom}", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                alg_bytes_per_launch=int(nb), avg_launch_ms=round(k_p[di], 5))
+    kern = {KERNELS[i]: dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5),
+                             plain_gbs=round(alg_bytes(KERNELS[i], mf, neq, 8, p.ns, False) / (k_p[i] * 1e-3) / 1e9, 1) if k_p[i] else None,
+                             mixed_gbs=round(alg_bytes(KERNELS[i], mf, neq, 4, p.ns, False) / (k_m[i] * 1e-3) / 1e9, 1) if k_m[i] else None)
+            for i in range(4) if k_p[i] > 0}
+    acc = l1_rel_density(F, p, pm, a.acc_steps, dt)
+    log(f"mixed vs plain after {a.acc_steps} steps: {acc}", a.quiet)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(F, p, dt, a.cpu_budget, a.quiet)
+    res = {
+        "metric": "cell-updates/s (RHS+RK, per RK stage), FP64 plain; FP32-flux mixed beside it",
+        "value": round(val_p, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall_p / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"],
+                   "hanging_quads": mf["Q"], "levels": mf["levels"], "model": "ideal gas gamma 1.4",
+                   "scheme": "MUSCL-minmod + HLLC(Davis), SSP-RK3", "dt": dt,
+                   **{k: v for k, v in meta.items() if k != "name"}},
+        "mixed": {"value": round(val_m, 1), "ms_per_step": round(wall_m / a.steps * 1e3, 4),
+                  "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,304 @@
+"""Python face of the C ABI: the host forest, the flattened mesh and the device
+solver context.  Arrays cross the boundary as contiguous float64 numpy arrays in
+the var-major layouts of include/fvgpu.h ([neq][ncell], [6][neq][ncell], ...).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import (FIELD_FACEFLUX, FIELD_RHS, FIELD_S, FIELD_SF, FIELD_SG, FIELD_SN, FVG_MINMOD,
+                    FVG_MIXED, FVG_PLAIN, FVG_VANALBADA, FvgParams, check, lib)
+
+# reference bc order of the six sides (inputset.c:99-104)
+SIDES = ("-x", "+x", "-y", "+y", "-z", "+z")
+BC_INLET, BC_OUTFLOW = 1, 2
+
+
+def neq_of(model: int) -> int:
+    n = lib().fvg_neq(model)
+    if n < 0:
+        raise ValueError(f"model {model} is not supported")
+    return n
+
+
+def params(model: int = 1, **kw) -> FvgParams:
+    """fvg_params with the reference's defaults, overridden by keyword."""
+    p = FvgParams()
+    lib().fvg_params_default(C.byref(p), model)
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = {"plain": FVG_PLAIN, "mixed": FVG_MIXED}[v]
+        if k == "limiter" and isinstance(v, str):
+            v = {"minmod": FVG_MINMOD, "vanalbada": FVG_VANALBADA}[v]
+        if k == "inlet":
+            arr = (C.c_double * 5)(*([float(x) for x in v] + [0.0] * (5 - len(v))))
+            p.inlet = arr
+            continue
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Forest:
+    """Host forest of octrees (reference BRANCH/CELL/LEAF structs)."""
+
+    def __init__(self, nroot: Sequence[int] = (1, 1, 1), x0: Sequence[float] = (0, 0, 0),
+                 x1: Sequence[float] = (1, 1, 1), level: int = 3,
+                 bc: Sequence[int] = (2, 2, 2, 2, 2, 2), periodic: Sequence[int] = (0, 0, 0),
+                 neq: int = 5):
+        L = lib()
+        self.neq = int(neq)
+        h = C.c_void_p()
+        ni = (C.c_int * 3)(*nroot)
+        a0 = (C.c_double * 3)(*map(float, x0))
+        a1 = (C.c_double * 3)(*map(float, x1))
+        b6 = (C.c_int * 6)(*bc)
+        pe = (C.c_int * 3)(*periodic)
+        check(L.fvh_forest_create(ni, a0, a1, int(level), b6, pe, self.neq, C.byref(h)),
+              "fvh_forest_create")
+        self._h = h
+        self.box = (np.array(x0, float), np.array(x1, float))
+        self.periodic = tuple(int(p) for p in periodic)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fvh_forest_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def run(self) -> C.c_void_p:
+        """The RUN* of this forest (for the reference-signature entry points)."""
+        return C.c_void_p(lib().fvh_forest_run(self._h))
+
+    @property
+    def nleaves(self) -> int:
+        return lib().fvh_forest_nleaves(self._h)
+
+    @property
+    def max_level(self) -> int:
+        return lib().fvh_forest_max_level(self._h)
+
+    def leaf_info(self):
+        n = self.nleaves
+        xc = np.empty((n, 3), np.float64)
+        lev = np.empty(n, np.int32)
+        check(lib().fvh_forest_leaf_info(self._h, _dptr(xc), lev.ctypes.data_as(C.POINTER(C.c_int32))),
+              "fvh_forest_leaf_info")
+        return xc, lev
+
+    def _shape(self, which: int):
+        n, q = self.nleaves, self.neq
+        return {FIELD_SF: (6, q, n), FIELD_SG: (q + 1, 3, n)}.get(which, (q, n))
+
+    def set_field(self, which: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, np.float64).reshape(self._shape(which))
+        check(lib().fvh_forest_set_field(self._h, which, _dptr(a)), "fvh_forest_set_field")
+
+    def get_field(self, which: int) -> np.ndarray:
+        a = np.empty(self._shape(which), np.float64)
+        check(lib().fvh_forest_get_field(self._h, which, _dptr(a)), "fvh_forest_get_field")
+        return a
+
+    def set_state(self, S: np.ndarray) -> None:
+        self.set_field(FIELD_S, S)
+
+    def get_state(self) -> np.ndarray:
+        return self.get_field(FIELD_S)
+
+    def adapt(self, flags: np.ndarray, max_level: int) -> int:
+        f = np.ascontiguousarray(flags, np.int8)
+        if f.shape != (self.nleaves,):
+            raise ValueError("one flag per leaf")
+        n = lib().fvh_forest_adapt(self._h, f.ctypes.data_as(C.POINTER(C.c_int8)), int(max_level))
+        if n < 0:
+            check(n, "fvh_forest_adapt")
+        return n
+
+
+class FlatMesh:
+    """The flattened struct-of-arrays mesh of a RUN (owned native arrays)."""
+
+    def __init__(self, run: C.c_void_p):
+        h = C.c_void_p()
+        check(lib().fvh_flatten_run(run, C.byref(h)), "fvh_flatten_run")
+        self._h = h
+        self.mesh = lib().fvh_flat_mesh(h).contents
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvh_flat_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ncell(self) -> int:
+        return self.mesh.ncell
+
+    @property
+    def nface(self) -> int:
+        return self.mesh.nface
+
+    def arrays(self) -> dict:
+        m = self.mesh
+        N, F, Q, nl = m.ncell, m.nface, m.nquad, m.nlevel
+
+        def arr(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+        return dict(
+            cflag=arr(m.cflag, N, np.uint32), nbr=arr(m.nbr, 6 * N, np.int32).reshape(6, N),
+            quad=arr(m.quad_nbr, 4 * Q, np.int32).reshape(Q, 4) if Q else np.zeros((0, 4), np.int32),
+            face_L=arr(m.face_L, F, np.int32), face_R=arr(m.face_R, F, np.int32),
+            face_meta=arr(m.face_meta, F, np.uint32), gslot=arr(m.gslot, 6 * N, np.int32).reshape(6, N),
+            h=arr(m.h, 3 * nl, np.float64).reshape(nl, 3),
+            dist=arr(m.dist, 18 * nl, np.float64).reshape(nl, 3, 3, 2),
+            area=arr(m.area, 3 * nl, np.float64).reshape(nl, 3),
+            vol=arr(m.vol, nl, np.float64), nface_axis=tuple(m.nface_axis))
+
+
+class Solver:
+    """A device context (fvg_ctx) on one mesh."""
+
+    def __init__(self, p: FvgParams, device: int = 0):
+        h = C.c_void_p()
+        self.params = p
+        check(lib().fvg_create(int(device), C.byref(p), C.byref(h)), "fvg_create")
+        self._h = h
+        self.neq = neq_of(p.model)
+        self.ncell = 0
+        self.nface = 0
+
+    @classmethod
+    def from_forest(cls, forest: Forest, p: FvgParams, device: int = 0, upload: bool = True):
+        s = cls(p, device)
+        s.set_mesh(FlatMesh(forest.run))
+        if upload:
+            s.set_state(forest.get_state())
+        return s
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fvg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _ck(self, rc: int, where: str) -> None:
+        check(rc, where, self._h)
+
+    def set_params(self, p: FvgParams) -> None:
+        self._ck(lib().fvg_set_params(self._h, C.byref(p)), "fvg_set_params")
+        self.params = p
+
+    def set_mesh(self, flat: FlatMesh) -> None:
+        self._ck(lib().fvg_set_mesh(self._h, C.byref(flat.mesh)), "fvg_set_mesh")
+        self.ncell = flat.ncell
+        self.nface = flat.nface
+
+    def _shape(self, which: int):
+        n, q = self.ncell, self.neq
+        return {FIELD_SF: (6, q, n), FIELD_SG: (q + 1, 3, n), FIELD_FACEFLUX: (q, self.nface)}.get(which, (q, n))
+
+    def set_field(self, which: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, np.float64).reshape(self._shape(which))
+        self._ck(lib().fvg_set_field(self._h, which, _dptr(a)), "fvg_set_field")
+
+    def get_field(self, which: int) -> np.ndarray:
+        a = np.empty(self._shape(which), np.float64)
+        self._ck(lib().fvg_get_field(self._h, which, _dptr(a)), "fvg_get_field")
+        return a
+
+    def set_state(self, S: np.ndarray) -> None:
+        self.set_field(FIELD_S, S)
+
+    def get_state(self) -> np.ndarray:
+        return self.get_field(FIELD_S)
+
+    def device_ptr(self, which: int) -> int:
+        p = C.c_void_p()
+        self._ck(lib().fvg_device_ptr(self._h, which, C.byref(p)), "fvg_device_ptr")
+        return p.value
+
+    def faceinterp(self) -> None:
+        self._ck(lib().fvg_faceinterp(self._h), "fvg_faceinterp")
+
+    def gradcalc(self) -> None:
+        self._ck(lib().fvg_gradcalc(self._h), "fvg_gradcalc")
+
+    def rhs(self) -> None:
+        self._ck(lib().fvg_rhs(self._h), "fvg_rhs")
+
+    def rk(self, irk: int, dt: float) -> None:
+        self._ck(lib().fvg_rk(self._h, int(irk), float(dt)), "fvg_rk")
+
+    def step(self, dt: float, method: int = 3) -> None:
+        self._ck(lib().fvg_step(self._h, int(method), float(dt)), "fvg_step")
+
+    def steps(self, n: int, dt: float, method: int = 3) -> None:
+        self._ck(lib().fvg_steps(self._h, int(n), int(method), float(dt)), "fvg_steps")
+
+    def cfl_dt(self, cfl: float) -> float:
+        d = C.c_double()
+        self._ck(lib().fvg_cfl_dt(self._h, float(cfl), C.byref(d)), "fvg_cfl_dt")
+        return d.value
+
+    def totals(self) -> np.ndarray:
+        out = np.empty(self.neq + 1, np.float64)
+        self._ck(lib().fvg_totals(self._h, _dptr(out)), "fvg_totals")
+        return out
+
+    def check_state(self) -> int:
+        b = C.c_int32()
+        self._ck(lib().fvg_check_state(self._h, C.byref(b)), "fvg_check_state")
+        return b.value
+
+    def sync(self) -> None:
+        self._ck(lib().fvg_sync(self._h), "fvg_sync")
+
+    def time_steps(self, n: int, dt: float, method: int = 3):
+        tot = C.c_double()
+        k = (C.c_double * 8)()
+        self._ck(lib().fvg_time_steps(self._h, int(n), int(method), float(dt), C.byref(tot), k),
+                 "fvg_time_steps")
+        return tot.value, list(k)
+
+    def full_rhs(self) -> np.ndarray:
+        """faceinterp (MUSCL) + gradcalc (viscous) + RHS of the current S."""
+        if self.params.ischeme == 2:
+            self.faceinterp()
+        if self.params.ns:
+            self.gradcalc()
+        self.rhs()
+        return self.get_field(FIELD_RHS)

@@ -1,0 +1,1201 @@
+/*
+ * fvgpu.hip — gfx950 kernels and the fvg_* C ABI of the RHS + RK hot path.
+ *
+ * Data layout in HBM (one context = one mesh):
+ *   S, SN, RHS          f64 [neq][ns]          conserved state, RK base, residual
+ *   SF                  R   [6][neq][ns]       MUSCL face states (R = f64 plain, f32 mixed)
+ *   SG                  f64 [neq+1][3][ns]     full Green-Gauss gradients (gradcalc)
+ *   SGm                 R   [9][ns]            momentum gradients for the viscous flux
+ *   G                   R   [neq][nfp]         owner-side face flux, -Area*(F.n + HLLC)
+ *   cflag/nbr/gslot     per cell               classes, neighbours, gather slots
+ *   fL/fR/fmeta         per face, grouped by axis
+ * ns = ncell rounded up to 64, nfp = nface rounded up to 64.
+ *
+ * Kernels per RK stage (each a coalesced stream over cells or faces):
+ *   k_recon      faceinterp.c:21-61     S -> SF
+ *   k_gradmom    gradcalc.c:23-42       S -> SGm (only when ns == 1)
+ *   k_face       residual_faces_AMR.c   SF (+SGm) -> G, each face once
+ *   k_gather_rk  residual_faces_AMR.c:173-175 + RK.c:24-47
+ *                                       G -> RHS (optional) -> S
+ * The gather reads each cell's face fluxes in the reference's order
+ * f = 0..5, k = 0..3 (no atomics, bitwise deterministic).
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "fvgpu.h"
+#include "fv_physics.h"
+
+/* ------------------------------------------------------------------ constants */
+
+__constant__ int c_face_axis[6] = {1, 0, 1, 0, 2, 2};
+__constant__ int c_face_sign[6] = {-1, 1, 1, -1, -1, 1};
+
+
+#define FV_BLOCK 256
+#define FV_NKT 8
+
+enum { CLS_SAME = 0, CLS_FINER = 1, CLS_COARSER = 2, CLS_BOUNDARY = 3 };
+
+static inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+/* ------------------------------------------------------------------ mesh views */
+
+struct DevMesh {
+  int N, ns, F, nfp, Q, nlevel;
+  const uint32_t *cflag;
+  const int32_t *nbr;      /* [6][ns] */
+  const int32_t *quad;     /* [Q][4] */
+  const int32_t *fL, *fR;
+  const uint32_t *fmeta;
+  const int32_t *gslot;    /* [6][ns] */
+  const double *h, *dist, *area, *vol;
+};
+
+struct DevState {
+  double *S, *SN, *RHS, *SG;
+  void *SF, *SGm, *G;
+  int *err;        /* [0] first bad cell (min), [1] flags */
+  double *red;     /* reduction scratch */
+};
+
+/* ------------------------------------------------------------------ helpers */
+
+template <int NEQ>
+__device__ __forceinline__ void load_S(const double *__restrict__ S, int ns, int i, double *U) {
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) U[v] = S[(size_t)v * ns + i];
+}
+
+/* neighbour state across face f for MUSCL deltas (facevalues_MUSCL.c:18-49 and
+ * the bc ghost of deltavalues.c:12-13) */
+template <int NEQ>
+__device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
+                                         const double *__restrict__ S, int i, uint32_t cf, int f,
+                                         const double *Uown, double *Un) {
+  int cls = (cf >> (2 * f)) & 3;
+  int nb = M.nbr[(size_t)f * M.ns + i];
+  if (cls == CLS_SAME || cls == CLS_COARSER) {
+    load_S<NEQ>(S, M.ns, nb, Un);
+  } else if (cls == CLS_FINER) {
+    int q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = M.quad[4 * nb + k];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc += S[(size_t)v * M.ns + q[k]] / ((double)4);
+      Un[v] = acc;
+    }
+  } else {
+    int bc = -1 - nb;
+    if (bc == 1) {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Un[v] = P.inlet[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Un[v] = Uown[v];
+    }
+  }
+}
+
+__device__ __forceinline__ void face_dist(const DevMesh &M, uint32_t cf, int f, double &dcf,
+                                          double &dcc) {
+  int cls = (cf >> (2 * f)) & 3;
+  if (cls == CLS_BOUNDARY) { dcf = 0.0; dcc = 0.0; return; }
+  int L = (cf >> 16) & 0xff;
+  int a = c_face_axis[f];
+  int c = cls == CLS_SAME ? 0 : (cls == CLS_FINER ? 1 : 2);
+  const double *d = M.dist + ((L * 3 + a) * 3 + c) * 2;
+  dcf = d[0];
+  dcc = d[1];
+}
+
+/* reconctruct_V3.c:11-45 for one variable; lim 1 = smooth van Albada (extension) */
+__device__ __forceinline__ double limited(int lim, double eps, bool bnd, double d0, double d1,
+                                          double dp1cc, double dm1cc) {
+  double grad = 0.0;
+  if (bnd) return grad;
+  if (lim == FVG_MINMOD) {
+    if (fabs(d0) > 1e-10) {
+      double a = -d1 / dm1cc, b = d0 / dp1cc;
+      double prod = a * b;
+      if (prod > 0.0) grad = fabs(a) > fabs(b) ? b : a;
+    }
+  } else {
+    double a = dm1cc != 0.0 ? -d1 / dm1cc : 0.0;
+    double b = dp1cc != 0.0 ? d0 / dp1cc : 0.0;
+    grad = (a * (b * b + eps) + b * (a * a + eps)) / (a * a + b * b + 2.0 * eps);
+  }
+  return grad;
+}
+
+/* XCD-aware block order: consecutive logical blocks (contiguous Morton ranges
+ * of cells or faces) land on the same XCD and so share its L2. */
+__device__ __forceinline__ int logical_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = nb >> 3;
+  if (per == 0 || (nb & 7)) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
+/* ------------------------------------------------------------------ kernels */
+
+template <int MODEL, typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
+                                                    const double *__restrict__ S,
+                                                    R *__restrict__ SF) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  double U[NEQ];
+  load_S<NEQ>(S, M.ns, i, U);
+  /* axis pairs (f+, f-): x (1,3), y (2,0), z (5,4) */
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
+    const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
+    double Up[NEQ], Um[NEQ];
+    nb_state<NEQ>(M, P, S, i, cf, fp, U, Up);
+    nb_state<NEQ>(M, P, S, i, cf, fm, U, Um);
+    double pcf, pcc, mcf, mcc;
+    face_dist(M, cf, fp, pcf, pcc);
+    face_dist(M, cf, fm, mcf, mcc);
+    const bool bp = ((cf >> (2 * fp)) & 3) == CLS_BOUNDARY;
+    const bool bm = ((cf >> (2 * fm)) & 3) == CLS_BOUNDARY;
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      double dP = Up[v] - U[v];
+      double dM = Um[v] - U[v];
+      double gp = limited(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc);
+      double gm = limited(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc);
+      SF[((size_t)fp * NEQ + v) * M.ns + i] = (R)(U[v] + pcf * gp);
+      SF[((size_t)fm * NEQ + v) * M.ns + i] = (R)(U[v] + mcf * gm);
+    }
+  }
+}
+
+/* Green-Gauss gradients (gradcalc.c:23-42).  FULL: all neq+1 rows (incl. p) to
+ * SG f64 [neq+1][3][ns]; otherwise the momentum rows 1..3 to SGm [9][ns]. */
+template <int MODEL, bool FULL, typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
+                                                   const double *__restrict__ S,
+                                                   double *__restrict__ SG, R *__restrict__ SGm) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  const int L = (cf >> 16) & 0xff;
+  const double V = M.vol[L];
+  double U[NEQ];
+  load_S<NEQ>(S, M.ns, i, U);
+  double pown = 0.0;
+  if constexpr (FULL) {
+    double r, u[3], e, p, c, y;
+    eos64<MODEL>(P, U, false, r, u, e, p, c, y);
+    pown = p;
+  }
+  constexpr int V0 = FULL ? 0 : 1;
+  constexpr int V1 = FULL ? NEQ : 4;
+  /* per axis d, faces of that axis in increasing face id */
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int fa = d == 0 ? 1 : (d == 1 ? 0 : 4);
+    const int fb = d == 0 ? 3 : (d == 1 ? 2 : 5);
+    double acc[NEQ + 1];
+#pragma unroll
+    for (int v = 0; v <= NEQ; ++v) acc[v] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int f = t == 0 ? fa : fb;
+      const double nd = (double)c_face_sign[f];
+      const int cls = (cf >> (2 * f)) & 3;
+      const int nb = M.nbr[(size_t)f * M.ns + i];
+      const int nk = cls == CLS_FINER ? 4 : 1;
+      const double A = M.area[3 * (cls == CLS_FINER ? L + 1 : L) + d];
+      for (int k = 0; k < nk; ++k) {
+        double Un[NEQ];
+        if (cls == CLS_FINER) load_S<NEQ>(S, M.ns, M.quad[4 * nb + k], Un);
+        else if (cls == CLS_BOUNDARY) {
+          if (-1 - nb == 1) {
+#pragma unroll
+            for (int v = 0; v < NEQ; ++v) Un[v] = P.inlet[v];
+          } else {
+#pragma unroll
+            for (int v = 0; v < NEQ; ++v) Un[v] = U[v];
+          }
+        } else load_S<NEQ>(S, M.ns, nb, Un);
+#pragma unroll
+        for (int v = V0; v < V1; ++v) acc[v] += 0.5 * (Un[v] + U[v]) * nd * A / V;
+        if constexpr (FULL) {
+          double r, u[3], e, p, c, y;
+          eos64<MODEL>(P, Un, false, r, u, e, p, c, y);
+          acc[NEQ] += 0.5 * (p + pown) * nd * A / V;
+        }
+      }
+    }
+    if constexpr (FULL) {
+#pragma unroll
+      for (int v = 0; v <= NEQ; ++v) SG[((size_t)v * 3 + d) * M.ns + i] = acc[v];
+    } else {
+#pragma unroll
+      for (int v = 1; v < 4; ++v) SGm[((size_t)(v - 1) * 3 + d) * M.ns + i] = (R)acc[v];
+    }
+  }
+}
+
+/* one face, both sides (residual_faces_AMR.c:95-150 for the owner side), with
+ * the axis a of the face as a compile-time constant */
+template <int MODEL, bool NS, int a, typename T, typename R>
+__device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, int j,
+                                          const double *__restrict__ S, const R *__restrict__ SF,
+                                          const R *__restrict__ SGm, R *__restrict__ G,
+                                          int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int iL = M.fL[j], iR = M.fR[j];
+  const uint32_t meta = M.fmeta[j];
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const T s = (T)c_face_sign[fl];
+  double UL[NEQ], UR[NEQ];
+  const bool first = P.ischeme == 1;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v)
+    UL[v] = first ? S[(size_t)v * M.ns + iL] : (double)SF[((size_t)fl * NEQ + v) * M.ns + iL];
+  if (bc == 0) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v)
+      UR[v] = first ? S[(size_t)v * M.ns + iR] : (double)SF[((size_t)fr * NEQ + v) * M.ns + iR];
+  } else if (bc == 1) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) UR[v] = UL[v];
+  }
+  FState<T> L, Rs;
+  const bool clip = !first;
+  bool ok = make_state<MODEL, T>(P, UL, clip, L);
+  ok = make_state<MODEL, T>(P, UR, clip && bc != 1, Rs) && ok;
+  if constexpr (NS) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
+      Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
+    }
+  }
+  T out[NEQ];
+  ok = hllc_face<MODEL, NS, a, T>(P, L, Rs, s, out) && ok;
+  const T A = (T)M.area[3 * lev + a];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    T g = -(A) * out[v];
+    G[(size_t)v * M.nfp + j] = (R)g;
+  }
+  if (!ok) atomicMin(err, iL);
+}
+
+/* Faces are grouped by axis; blocks [0, b0) take x faces, [b0, b0+b1) y faces,
+ * the rest z faces, so the axis is uniform within a block. */
+template <int MODEL, bool NS, typename T, typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b0, int b1,
+                                                   int n0, int n1, int n2,
+                                                   const double *__restrict__ S,
+                                                   const R *__restrict__ SF,
+                                                   const R *__restrict__ SGm,
+                                                   R *__restrict__ G, int *__restrict__ err) {
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  if (lb < b0) {
+    const int j = lb * FV_BLOCK + t;
+    if (j < n0) face_body<MODEL, NS, 0, T, R>(M, P, j, S, SF, SGm, G, err);
+  } else if (lb < b0 + b1) {
+    const int q = (lb - b0) * FV_BLOCK + t;
+    if (q < n1) face_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SF, SGm, G, err);
+  } else {
+    const int q = (lb - b0 - b1) * FV_BLOCK + t;
+    if (q < n2) face_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SF, SGm, G, err);
+  }
+}
+
+/* RHS gather (residual_faces_AMR.c:173-175, order f = 0..5, k = 0..3) fused with
+ * the RK stage (RK.c:24-47).  irk 0 < stage: 1..4; FIRST: also SN = S (RK(0)). */
+template <int NEQ, typename R, bool WRITE_RHS, bool DO_RK, bool FIRST>
+__global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__restrict__ G,
+                                                        double *__restrict__ S,
+                                                        double *__restrict__ SN,
+                                                        double *__restrict__ RHS, int irk,
+                                                        double dt, int *__restrict__ err) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  double rhs[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) rhs[v] = 0.0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int g = M.gslot[(size_t)f * M.ns + i];
+    const int cls = (cf >> (2 * f)) & 3;
+    double ff[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) ff[v] = 0.0;
+    if (g >= 0) {
+      const int jf = g >> 1;
+      if (cls == CLS_FINER) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)v * M.nfp + jf + k];
+      } else if (g & 1) {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) ff[v] += -(double)G[(size_t)v * M.nfp + jf];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)v * M.nfp + jf];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) rhs[v] += ff[v];
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) bad |= (rhs[v] != rhs[v]);
+  if (bad) atomicMin(err, i);
+  if constexpr (WRITE_RHS) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) RHS[(size_t)v * M.ns + i] = rhs[v];
+  }
+  if constexpr (DO_RK) {
+    const double Vol = M.vol[(cf >> 16) & 0xff];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const size_t o = (size_t)v * M.ns + i;
+      double s = S[o];
+      double sn;
+      if constexpr (FIRST) { sn = s; SN[o] = s; } else { sn = SN[o]; }
+      double r = rhs[v];
+      double out;
+      switch (irk) {
+        case 1: out = sn + dt * (1.0 / Vol) * r; break;
+        case 2: out = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
+        case 3: out = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
+        case 4: out = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
+        default: out = s; break;
+      }
+      S[o] = out;
+    }
+  }
+}
+
+/* standalone RK(irk) from the RHS array (RK.c:16-51) */
+template <int NEQ>
+__global__ __launch_bounds__(FV_BLOCK) void k_rk(DevMesh M, double *__restrict__ S,
+                                                 double *__restrict__ SN,
+                                                 const double *__restrict__ RHS, int irk,
+                                                 double dt) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const double Vol = M.vol[(M.cflag[i] >> 16) & 0xff];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const size_t o = (size_t)v * M.ns + i;
+    const double s = S[o];
+    switch (irk) {
+      case 0: SN[o] = s; break;
+      case 1: S[o] = SN[o] + dt * (1.0 / Vol) * RHS[o]; break;
+      case 2: S[o] = (3.0 / 4.0) * SN[o] + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * RHS[o] / Vol; break;
+      case 3: S[o] = (1.0 / 3.0) * SN[o] + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * RHS[o] / Vol; break;
+      case 4: S[o] = (1.0 / 2.0) * SN[o] + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * RHS[o] / Vol; break;
+      default: break;
+    }
+  }
+}
+
+/* per-cell sum_a (|u_a| + c)/h_a, wavefront max by lane shuffles, block max in
+ * LDS, one 64-bit integer atomicMax per block on the (non-negative) double bits */
+template <int MODEL>
+__global__ __launch_bounds__(FV_BLOCK) void k_cfl(DevMesh M, DevParams P,
+                                                  const double *__restrict__ S,
+                                                  unsigned long long *__restrict__ out,
+                                                  int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  __shared__ double red[FV_BLOCK / 64];
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  double m = 0.0;
+  if (i < M.N) {
+    double U[NEQ];
+    load_S<NEQ>(S, M.ns, i, U);
+    double r, u[3], e, p, c, y;
+    eos64<MODEL>(P, U, false, r, u, e, p, c, y);
+    const int L = (M.cflag[i] >> 16) & 0xff;
+    const double *h = M.h + 3 * L;
+    m = (fabs(u[0]) + c) / h[0] + (fabs(u[1]) + c) / h[1] + (fabs(u[2]) + c) / h[2];
+    if (!(m == m) || !(r > 0.0)) { atomicMin(err, i); m = 0.0; }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < FV_BLOCK / 64; ++w) b = fmax(b, red[w]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+/* fixed-order totals: block b sums cells [b*chunk, (b+1)*chunk) thread-strided,
+ * tree-reduces in LDS, writes part[b][neq+1]; a second pass sums the parts */
+template <int NEQ>
+__global__ __launch_bounds__(FV_BLOCK) void k_totals_part(DevMesh M, const double *__restrict__ S,
+                                                          double *__restrict__ part, int chunk) {
+  __shared__ double sh[NEQ + 1][FV_BLOCK];
+  const int b = blockIdx.x, t = threadIdx.x;
+  double acc[NEQ + 1];
+#pragma unroll
+  for (int v = 0; v <= NEQ; ++v) acc[v] = 0.0;
+  const int i0 = b * chunk, i1 = min(M.N, i0 + chunk);
+  for (int i = i0 + t; i < i1; i += FV_BLOCK) {
+    const double V = M.vol[(M.cflag[i] >> 16) & 0xff];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) acc[v] += S[(size_t)v * M.ns + i] * V;
+    acc[NEQ] += V;
+  }
+#pragma unroll
+  for (int v = 0; v <= NEQ; ++v) sh[v][t] = acc[v];
+  __syncthreads();
+  for (int w = FV_BLOCK / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int v = 0; v <= NEQ; ++v) sh[v][t] += sh[v][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int v = 0; v <= NEQ; ++v) part[(size_t)b * (NEQ + 1) + v] = sh[v][0];
+  }
+}
+
+template <int NEQ>
+__global__ void k_totals_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int v = 0; v <= NEQ; ++v) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * (NEQ + 1) + v];
+    out[v] = s;
+  }
+}
+
+template <int NEQ>
+__global__ __launch_bounds__(FV_BLOCK) void k_check(DevMesh M, const double *__restrict__ S,
+                                                    const double *__restrict__ RHS,
+                                                    int *__restrict__ err) {
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  bool bad = !(S[i] > 0.0);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    double s = S[(size_t)v * M.ns + i], r = RHS[(size_t)v * M.ns + i];
+    bad |= !(s - s == 0.0) || !(r - r == 0.0);
+  }
+  if (bad) atomicMin(err, i);
+}
+
+__global__ void k_fill_int(int *p, int n, int val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long val) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = val;
+}
+
+/* ------------------------------------------------------------------ context */
+
+struct fvg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fvg_params prm;
+  DevParams dp;
+  int neq = 5;
+  bool has_mesh = false;
+  int N = 0, ns = 0, F = 0, nfp = 0, Q = 0, nlevel = 0;
+  int nfa[3] = {0, 0, 0};
+  /* device buffers */
+  uint32_t *cflag = nullptr;
+  int32_t *nbr = nullptr, *quad = nullptr, *fL = nullptr, *fR = nullptr, *gslot = nullptr;
+  uint32_t *fmeta = nullptr;
+  double *tab = nullptr;
+  double *S = nullptr, *SN = nullptr, *RHS = nullptr, *SG = nullptr;
+  void *SF = nullptr, *SGm = nullptr, *G = nullptr;
+  int *err = nullptr;
+  double *red = nullptr;
+  unsigned long long *redu = nullptr;
+  size_t red_cap = 0;
+  size_t rsize = 8;  /* bytes of R (8 plain, 4 mixed) */
+  std::string last;
+  hipEvent_t ev[2 * 64 + 2];
+  bool ev_ok = false;
+};
+
+static int set_err(fvg_ctx *c, int code, const char *msg) {
+  if (c) c->last = msg ? msg : "";
+  return code;
+}
+
+#define HIPCHK(c, call)                                                           \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      char b_[256];                                                               \
+      snprintf(b_, sizeof b_, "%s: %s", #call, hipGetErrorString(e_));            \
+      return set_err((c), FVG_ERR_HIP, b_);                                       \
+    }                                                                             \
+  } while (0)
+
+static DevMesh dev_mesh(const fvg_ctx *c) {
+  DevMesh m;
+  m.N = c->N;
+  m.ns = c->ns;
+  m.F = c->F;
+  m.nfp = c->nfp;
+  m.Q = c->Q;
+  m.nlevel = c->nlevel;
+  m.cflag = c->cflag;
+  m.nbr = c->nbr;
+  m.quad = c->quad;
+  m.fL = c->fL;
+  m.fR = c->fR;
+  m.fmeta = c->fmeta;
+  m.gslot = c->gslot;
+  const int nl = c->nlevel;
+  m.h = c->tab;
+  m.dist = c->tab + 3 * nl;
+  m.area = c->tab + 3 * nl + 18 * nl;
+  m.vol = c->tab + 3 * nl + 18 * nl + 3 * nl;
+  return m;
+}
+
+static void to_dev_params(const fvg_params *p, DevParams *d) {
+  d->model = p->model;
+  d->ns = p->ns;
+  d->ischeme = p->ischeme;
+  d->limiter = p->limiter;
+  d->visc_work = p->visc_work;
+  d->neq = fvg_neq(p->model);
+  d->gamma = p->gamma;
+  d->pinf = p->pinf;
+  d->mu = p->mu;
+  d->ymin = p->ymin;
+  d->baro_nexp = p->baro_nexp;
+  d->baro_beta = p->baro_beta;
+  d->baro_cbar = p->baro_cbar;
+  d->baro_psat = p->baro_psat;
+  d->baro_rsat = p->baro_rsat;
+  d->mix_RG = p->mix_RG;
+  d->mix_Tref = p->mix_Tref;
+  d->mix_rsat = p->mix_rsat;
+  d->mix_psat = p->mix_psat;
+  d->mix_cliq = p->mix_cliq;
+  d->mix_cmix = p->mix_cmix;
+  for (int v = 0; v < 5; ++v) d->inlet[v] = p->inlet[v];
+  d->va_eps = p->vanalbada_eps;
+}
+
+static int check_params(const fvg_params *p) {
+  if (!p) return FVG_ERR_ARG;
+  if (fvg_neq(p->model) < 0) return FVG_ERR_UNSUPPORTED;
+  if (p->ischeme != 1 && p->ischeme != 2) return FVG_ERR_UNSUPPORTED;
+  if (p->limiter != FVG_MINMOD && p->limiter != FVG_VANALBADA) return FVG_ERR_UNSUPPORTED;
+  if (p->precision != FVG_PLAIN && p->precision != FVG_MIXED) return FVG_ERR_UNSUPPORTED;
+  if (p->ns && p->model == 2) return FVG_ERR_UNSUPPORTED; /* viscous path needs 5 rows of SG */
+  return FVG_OK;
+}
+
+static void free_mesh(fvg_ctx *c) {
+  void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->gslot, c->fmeta, c->tab,
+                  c->S, c->SN, c->RHS, c->SG, c->SF, c->SGm, c->G};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->gslot = nullptr;
+  c->fmeta = nullptr; c->tab = nullptr;
+  c->S = c->SN = c->RHS = c->SG = nullptr;
+  c->SF = c->SGm = c->G = nullptr;
+  c->has_mesh = false;
+}
+
+extern "C" {
+
+int fvg_create(int device, const fvg_params *p, fvg_ctx **out) {
+  if (!out) return FVG_ERR_ARG;
+  int rc = check_params(p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FVG_ERR_HIP;
+  if (device < 0 || device >= ndev) return FVG_ERR_ARG;
+  fvg_ctx *c = new fvg_ctx();
+  c->device = device;
+  c->prm = *p;
+  c->neq = fvg_neq(p->model);
+  c->rsize = p->precision == FVG_MIXED ? 4 : 8;
+  to_dev_params(p, &c->dp);
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->err, 4 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->redu, 64) != hipSuccess) {
+    delete c;
+    return FVG_ERR_HIP;
+  }
+  *out = c;
+  return FVG_OK;
+}
+
+int fvg_destroy(fvg_ctx *c) {
+  if (!c) return FVG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  if (c->err) (void)hipFree(c->err);
+  if (c->red) (void)hipFree(c->red);
+  if (c->redu) (void)hipFree(c->redu);
+  if (c->ev_ok)
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FVG_OK;
+}
+
+int fvg_set_params(fvg_ctx *c, const fvg_params *p) {
+  if (!c) return FVG_ERR_ARG;
+  int rc = check_params(p);
+  if (rc) return set_err(c, rc, "unsupported parameters");
+  if (fvg_neq(p->model) != c->neq) return set_err(c, FVG_ERR_ARG, "model changes neq");
+  size_t rs = p->precision == FVG_MIXED ? 4 : 8;
+  if (c->has_mesh && rs != c->rsize) return set_err(c, FVG_ERR_STATE, "precision fixed by set_mesh");
+  c->prm = *p;
+  c->rsize = rs;
+  to_dev_params(p, &c->dp);
+  return FVG_OK;
+}
+
+int fvg_ncell(const fvg_ctx *c) { return c ? c->N : FVG_ERR_ARG; }
+void *fvg_stream(fvg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+const char *fvg_last_error(const fvg_ctx *c) { return c ? c->last.c_str() : "null context"; }
+
+int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
+  if (!c || !m) return FVG_ERR_ARG;
+  if (m->neq != c->neq) return set_err(c, FVG_ERR_ARG, "mesh neq differs from the model's");
+  if (m->ncell <= 0 || m->nface <= 0 || m->nlevel <= 0 || m->nlevel > 32 || m->nquad < 0)
+    return set_err(c, FVG_ERR_ARG, "bad mesh sizes");
+  if ((int64_t)m->nface_axis[0] + m->nface_axis[1] + m->nface_axis[2] != m->nface)
+    return set_err(c, FVG_ERR_ARG, "face axis counts do not sum to nface");
+  const int N = m->ncell, F = m->nface, Q = m->nquad, nl = m->nlevel;
+  /* host-side validation of every index the kernels dereference */
+  for (int i = 0; i < N; ++i) {
+    uint32_t cf = m->cflag[i];
+    int L = (cf >> 16) & 0xff;
+    if (L < 1 || L + 1 >= nl + 1) return set_err(c, FVG_ERR_ARG, "cell level outside tables");
+    for (int f = 0; f < 6; ++f) {
+      int cls = (cf >> (2 * f)) & 3;
+      int nb = m->nbr[(size_t)f * N + i];
+      if (cls == CLS_SAME || cls == CLS_COARSER) {
+        if (nb < 0 || nb >= N) return set_err(c, FVG_ERR_ARG, "neighbour index out of range");
+      } else if (cls == CLS_FINER) {
+        if (nb < 0 || nb >= Q) return set_err(c, FVG_ERR_ARG, "quad index out of range");
+        if (L + 1 >= nl) return set_err(c, FVG_ERR_ARG, "finer level outside tables");
+      } else {
+        if (nb != -2 && nb != -3) return set_err(c, FVG_ERR_ARG, "boundary type not 1 or 2");
+      }
+      int g = m->gslot[(size_t)f * N + i];
+      if (g < -1 || (g >= 0 && (g >> 1) + (cls == CLS_FINER ? 3 : 0) >= F))
+        return set_err(c, FVG_ERR_ARG, "gather slot out of range");
+    }
+  }
+  for (int64_t q = 0; q < 4LL * Q; ++q)
+    if (m->quad_nbr[q] < 0 || m->quad_nbr[q] >= N) return set_err(c, FVG_ERR_ARG, "bad quad neighbour");
+  for (int j = 0; j < F; ++j) {
+    if (m->face_L[j] < 0 || m->face_L[j] >= N || m->face_R[j] < 0 || m->face_R[j] >= N)
+      return set_err(c, FVG_ERR_ARG, "face cell out of range");
+    uint32_t mt = m->face_meta[j];
+    int lev = (mt >> 8) & 0xff;
+    if ((mt & 7) > 5 || ((mt >> 3) & 7) > 5 || lev < 1 || lev >= nl || ((mt >> 6) & 3) == 3)
+      return set_err(c, FVG_ERR_ARG, "bad face meta");
+  }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  c->N = N;
+  c->ns = (int)round_up(N, 64);
+  c->F = F;
+  c->nfp = (int)round_up(F, 64);
+  c->Q = Q;
+  c->nlevel = nl;
+  for (int a = 0; a < 3; ++a) c->nfa[a] = m->nface_axis[a];
+  const int ns = c->ns, neq = c->neq;
+  const size_t rs = c->rsize;
+  HIPCHK(c, hipMalloc(&c->cflag, (size_t)ns * 4));
+  HIPCHK(c, hipMalloc(&c->nbr, (size_t)6 * ns * 4));
+  HIPCHK(c, hipMalloc(&c->gslot, (size_t)6 * ns * 4));
+  HIPCHK(c, hipMalloc(&c->quad, (size_t)4 * (Q ? Q : 1) * 4));
+  HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->tab, (size_t)25 * nl * 8));
+  HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SN, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->RHS, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SG, (size_t)(neq + 1) * 3 * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SF, (size_t)6 * neq * ns * rs));
+  HIPCHK(c, hipMalloc(&c->SGm, (size_t)9 * ns * rs));
+  HIPCHK(c, hipMalloc(&c->G, (size_t)neq * c->nfp * rs));
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->cflag, m->cflag, (size_t)N * 4, hipMemcpyHostToDevice, st));
+  for (int f = 0; f < 6; ++f) {
+    HIPCHK(c, hipMemcpyAsync(c->nbr + (size_t)f * ns, m->nbr + (size_t)f * N, (size_t)N * 4,
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->gslot + (size_t)f * ns, m->gslot + (size_t)f * N, (size_t)N * 4,
+                             hipMemcpyHostToDevice, st));
+  }
+  if (Q) HIPCHK(c, hipMemcpyAsync(c->quad, m->quad_nbr, (size_t)4 * Q * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->fL, m->face_L, (size_t)F * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->fR, m->face_R, (size_t)F * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->fmeta, m->face_meta, (size_t)F * 4, hipMemcpyHostToDevice, st));
+  std::vector<double> tab((size_t)25 * nl, 0.0);
+  memcpy(tab.data(), m->h, (size_t)3 * nl * 8);
+  memcpy(tab.data() + 3 * nl, m->dist, (size_t)18 * nl * 8);
+  memcpy(tab.data() + 21 * nl, m->area, (size_t)3 * nl * 8);
+  memcpy(tab.data() + 24 * nl, m->vol, (size_t)nl * 8);
+  HIPCHK(c, hipMemcpyAsync(c->tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st));
+  /* zero the padding and every state array so no kernel reads uninitialised memory */
+  HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SN, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)9 * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  c->has_mesh = true;
+  return FVG_OK;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------ field copies */
+
+static size_t field_rows(const fvg_ctx *c, int which, int *len, void **dptr, bool *isR) {
+  *isR = false;
+  switch (which) {
+    case FVG_FIELD_S: *dptr = c->S; *len = c->ns; return c->neq;
+    case FVG_FIELD_SN: *dptr = c->SN; *len = c->ns; return c->neq;
+    case FVG_FIELD_RHS: *dptr = c->RHS; *len = c->ns; return c->neq;
+    case FVG_FIELD_SF: *dptr = c->SF; *len = c->ns; *isR = true; return 6 * c->neq;
+    case FVG_FIELD_SG: *dptr = c->SG; *len = c->ns; return 3 * (c->neq + 1);
+    case FVG_FIELD_FACEFLUX: *dptr = c->G; *len = c->nfp; *isR = true; return c->neq;
+    default: return 0;
+  }
+}
+
+static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
+  if (!c || !host) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  int len;
+  void *d;
+  bool isR;
+  size_t rows = field_rows(c, which, &len, &d, &isR);
+  if (!rows) return set_err(c, FVG_ERR_ARG, "unknown field");
+  const int n = which == FVG_FIELD_FACEFLUX ? c->F : c->N;
+  (void)hipSetDevice(c->device);
+  const size_t es = isR ? c->rsize : 8;
+  std::vector<char> tmp((size_t)n * es);
+  for (size_t r = 0; r < rows; ++r) {
+    char *dp = (char *)d + r * (size_t)len * es;
+    double *hp = host + r * (size_t)n;
+    if (to_dev) {
+      if (es == 8) memcpy(tmp.data(), hp, (size_t)n * 8);
+      else for (int i = 0; i < n; ++i) ((float *)tmp.data())[i] = (float)hp[i];
+      HIPCHK(c, hipMemcpyAsync(dp, tmp.data(), (size_t)n * es, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+      HIPCHK(c, hipMemcpyAsync(tmp.data(), dp, (size_t)n * es, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (es == 8) memcpy(hp, tmp.data(), (size_t)n * 8);
+      else for (int i = 0; i < n; ++i) hp[i] = (double)((float *)tmp.data())[i];
+    }
+  }
+  return FVG_OK;
+}
+
+/* ------------------------------------------------------------------ launches */
+
+static inline int nblocks(int n) { return (n + FV_BLOCK - 1) / FV_BLOCK; }
+
+template <int MODEL>
+static void launch_recon(fvg_ctx *c) {
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  if (c->rsize == 8)
+    hipLaunchKernelGGL((k_recon<MODEL, double>), g, b, 0, c->stream, M, c->dp, c->S, (double *)c->SF);
+  else
+    hipLaunchKernelGGL((k_recon<MODEL, float>), g, b, 0, c->stream, M, c->dp, c->S, (float *)c->SF);
+}
+
+template <int MODEL>
+static void launch_grad(fvg_ctx *c, bool full) {
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  if (full)
+    hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+                       (double *)nullptr);
+  else if (c->rsize == 8)
+    hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+                       (double *)c->SGm);
+  else
+    hipLaunchKernelGGL((k_grad<MODEL, false, float>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+                       (float *)c->SGm);
+}
+
+template <int MODEL, bool NS>
+static void launch_face_m(fvg_ctx *c) {
+  DevMesh M = dev_mesh(c);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
+  dim3 g(b0 + b1 + b2), b(FV_BLOCK);
+  if (c->rsize == 8)
+    hipLaunchKernelGGL((k_face<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
+                       n1, n2, c->S, (const double *)c->SF, (const double *)c->SGm, (double *)c->G,
+                       c->err);
+  else
+    hipLaunchKernelGGL((k_face<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
+                       n1, n2, c->S, (const float *)c->SF, (const float *)c->SGm, (float *)c->G,
+                       c->err);
+}
+
+template <int MODEL>
+static void launch_face(fvg_ctx *c) {
+  if (c->prm.ns) launch_face_m<MODEL, true>(c);
+  else launch_face_m<MODEL, false>(c);
+}
+
+template <int NEQ, typename R>
+static void launch_gather_t(fvg_ctx *c, bool write_rhs, bool do_rk, bool first, int irk, double dt) {
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  const R *G = (const R *)c->G;
+  if (write_rhs && !do_rk)
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, c->S,
+                       c->SN, c->RHS, irk, dt, c->err);
+  else if (write_rhs && first)
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, c->S,
+                       c->SN, c->RHS, irk, dt, c->err);
+  else if (write_rhs)
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, c->S,
+                       c->SN, c->RHS, irk, dt, c->err);
+  else if (first)
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, c->S,
+                       c->SN, c->RHS, irk, dt, c->err);
+  else
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, c->S,
+                       c->SN, c->RHS, irk, dt, c->err);
+}
+
+static void launch_gather(fvg_ctx *c, bool write_rhs, bool do_rk, bool first, int irk, double dt) {
+  if (c->neq == 5) {
+    if (c->rsize == 8) launch_gather_t<5, double>(c, write_rhs, do_rk, first, irk, dt);
+    else launch_gather_t<5, float>(c, write_rhs, do_rk, first, irk, dt);
+  } else {
+    if (c->rsize == 8) launch_gather_t<4, double>(c, write_rhs, do_rk, first, irk, dt);
+    else launch_gather_t<4, float>(c, write_rhs, do_rk, first, irk, dt);
+  }
+}
+
+static void dispatch_recon(fvg_ctx *c) {
+  switch (c->prm.model) {
+    case 1: launch_recon<1>(c); break;
+    case 2: launch_recon<2>(c); break;
+    default: launch_recon<3>(c); break;
+  }
+}
+static void dispatch_grad(fvg_ctx *c, bool full) {
+  switch (c->prm.model) {
+    case 1: launch_grad<1>(c, full); break;
+    case 2: launch_grad<2>(c, full); break;
+    default: launch_grad<3>(c, full); break;
+  }
+}
+static void dispatch_face(fvg_ctx *c) {
+  switch (c->prm.model) {
+    case 1: launch_face<1>(c); break;
+    case 2: launch_face<2>(c); break;
+    default: launch_face<3>(c); break;
+  }
+}
+
+static int reset_err(fvg_ctx *c) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->err, 4, 0x7fffffff);
+  return FVG_OK;
+}
+
+static int read_err(fvg_ctx *c, int32_t *bad) {
+  int h[4];
+  HIPCHK(c, hipMemcpyAsync(h, c->err, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *bad = h[0] == 0x7fffffff ? -1 : h[0];
+  return FVG_OK;
+}
+
+static int finish(fvg_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char b[256];
+    snprintf(b, sizeof b, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return set_err(c, FVG_ERR_HIP, b);
+  }
+  return FVG_OK;
+}
+
+/* one stage's kernels; kt (optional) receives per-kernel event ids */
+static void stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs, hipEvent_t *ev,
+                  int *nev, int *kid) {
+  auto mark = [&](int k) {
+    if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
+  };
+  if (c->prm.ischeme == 2) { mark(0); dispatch_recon(c); }
+  if (c->prm.ns) { mark(1); dispatch_grad(c, false); }
+  mark(2);
+  dispatch_face(c);
+  mark(3);
+  launch_gather(c, write_rhs, true, first, irk, dt);
+  mark(-1);
+}
+
+static const int *stage_list(int method, int *n) {
+  static const int st3[3] = {1, 2, 3}, st2[2] = {1, 4}, st1[1] = {1};
+  if (method == 3) { *n = 3; return st3; }
+  if (method == 2) { *n = 2; return st2; }
+  *n = 1;
+  return st1;
+}
+
+extern "C" {
+
+int fvg_set_state(fvg_ctx *c, const double *S) { return copy_field(c, FVG_FIELD_S, (double *)S, true); }
+int fvg_get_state(fvg_ctx *c, double *S) { return copy_field(c, FVG_FIELD_S, S, false); }
+int fvg_set_field(fvg_ctx *c, int which, const double *src) {
+  return copy_field(c, which, (double *)src, true);
+}
+int fvg_get_field(fvg_ctx *c, int which, double *dst) { return copy_field(c, which, dst, false); }
+
+int fvg_device_ptr(fvg_ctx *c, int which, void **ptr) {
+  if (!c || !ptr) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  int len;
+  bool isR;
+  if (!field_rows(c, which, &len, ptr, &isR)) return set_err(c, FVG_ERR_ARG, "unknown field");
+  return FVG_OK;
+}
+
+int fvg_faceinterp(fvg_ctx *c) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  dispatch_recon(c);
+  return finish(c, "faceinterp");
+}
+
+int fvg_gradcalc(fvg_ctx *c) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  dispatch_grad(c, true);
+  if (c->prm.ns) dispatch_grad(c, false);
+  return finish(c, "gradcalc");
+}
+
+int fvg_rhs(fvg_ctx *c) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  reset_err(c);
+  dispatch_face(c);
+  launch_gather(c, true, false, false, 0, 0.0);
+  int rc = finish(c, "rhs");
+  if (rc) return rc;
+  int32_t bad;
+  rc = read_err(c, &bad);
+  if (rc) return rc;
+  if (bad >= 0) {
+    char b[96];
+    snprintf(b, sizeof b, "non-finite residual or failed Riemann/EOS at cell %d", bad);
+    return set_err(c, FVG_ERR_NONFINITE, b);
+  }
+  return FVG_OK;
+}
+
+int fvg_rk(fvg_ctx *c, int irk, double dt) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  if (irk < 0 || irk > 4) return set_err(c, FVG_ERR_ARG, "irk must be 0..4");
+  (void)hipSetDevice(c->device);
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  if (c->neq == 5)
+    hipLaunchKernelGGL(k_rk<5>, g, b, 0, c->stream, M, c->S, c->SN, c->RHS, irk, dt);
+  else
+    hipLaunchKernelGGL(k_rk<4>, g, b, 0, c->stream, M, c->S, c->SN, c->RHS, irk, dt);
+  return finish(c, "rk");
+}
+
+int fvg_steps(fvg_ctx *c, int nsteps, int method, double dt) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  if (method < 1 || method > 3 || nsteps < 0) return set_err(c, FVG_ERR_ARG, "bad method/nsteps");
+  (void)hipSetDevice(c->device);
+  int n;
+  const int *st = stage_list(method, &n);
+  reset_err(c);
+  /* RHS is written on the last stage only: afterwards it holds that stage's residual */
+  for (int s = 0; s < nsteps; ++s)
+    for (int k = 0; k < n; ++k)
+      stage(c, st[k], dt, k == 0, s == nsteps - 1 && k == n - 1, nullptr, nullptr, nullptr);
+  int rc = finish(c, "steps");
+  if (rc) return rc;
+  int32_t bad;
+  rc = read_err(c, &bad);
+  if (rc) return rc;
+  if (bad >= 0) {
+    char b[96];
+    snprintf(b, sizeof b, "non-finite residual or failed Riemann/EOS at cell %d", bad);
+    return set_err(c, FVG_ERR_NONFINITE, b);
+  }
+  return FVG_OK;
+}
+
+int fvg_step(fvg_ctx *c, int method, double dt) { return fvg_steps(c, 1, method, dt); }
+
+int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_total,
+                   double *ms_kernel) {
+  if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
+  if (method < 1 || method > 3 || nsteps < 1) return set_err(c, FVG_ERR_ARG, "bad method/nsteps");
+  (void)hipSetDevice(c->device);
+  if (!c->ev_ok) {
+    for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    c->ev_ok = true;
+  }
+  int n;
+  const int *st = stage_list(method, &n);
+  double sum[FV_NKT] = {0};
+  int cnt[FV_NKT] = {0};
+  reset_err(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipEvent_t &t0 = c->ev[2 * 64], &t1 = c->ev[2 * 64 + 1];
+  HIPCHK(c, hipEventRecord(t0, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    int nev = 0, kid[64];
+    for (int k = 0; k < n; ++k) stage(c, st[k], dt, k == 0, false, c->ev, &nev, kid);
+    HIPCHK(c, hipEventSynchronize(c->ev[nev - 1]));
+    for (int q = 0; q + 1 < nev; ++q) {
+      if (kid[q] < 0) continue;
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
+      sum[kid[q]] += ms;
+      cnt[kid[q]] += 1;
+    }
+  }
+  HIPCHK(c, hipEventRecord(t1, c->stream));
+  HIPCHK(c, hipEventSynchronize(t1));
+  float tot = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&tot, t0, t1));
+  if (ms_total) *ms_total = tot;
+  if (ms_kernel)
+    for (int k = 0; k < FV_NKT; ++k) ms_kernel[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  int32_t bad;
+  int rc = read_err(c, &bad);
+  if (rc) return rc;
+  if (bad >= 0) return set_err(c, FVG_ERR_NONFINITE, "non-finite state during timed steps");
+  return FVG_OK;
+}
+
+int fvg_cfl_dt(fvg_ctx *c, double cfl, double *dt) {
+  if (!c || !dt) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  (void)hipSetDevice(c->device);
+  reset_err(c);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->redu, 0ull);
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  switch (c->prm.model) {
+    case 1: hipLaunchKernelGGL(k_cfl<1>, g, b, 0, c->stream, M, c->dp, c->S, c->redu, c->err); break;
+    case 2: hipLaunchKernelGGL(k_cfl<2>, g, b, 0, c->stream, M, c->dp, c->S, c->redu, c->err); break;
+    default: hipLaunchKernelGGL(k_cfl<3>, g, b, 0, c->stream, M, c->dp, c->S, c->redu, c->err); break;
+  }
+  int rc = finish(c, "cfl");
+  if (rc) return rc;
+  unsigned long long bits;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->redu, 8, hipMemcpyDeviceToHost, c->stream));
+  int32_t bad;
+  rc = read_err(c, &bad);
+  if (rc) return rc;
+  if (bad >= 0) return set_err(c, FVG_ERR_NONFINITE, "non-physical state in CFL scan");
+  double m;
+  memcpy(&m, &bits, 8);
+  if (!(m > 0.0)) return set_err(c, FVG_ERR_NONFINITE, "zero wave speed");
+  *dt = cfl / m;
+  return FVG_OK;
+}
+
+int fvg_totals(fvg_ctx *c, double *out) {
+  if (!c || !out) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  (void)hipSetDevice(c->device);
+  const int chunk = 16 * FV_BLOCK;
+  const int nb = (c->N + chunk - 1) / chunk;
+  const size_t need = (size_t)(nb + 1) * (c->neq + 1);
+  if (c->red_cap < need) {
+    if (c->red) (void)hipFree(c->red);
+    c->red = nullptr;
+    HIPCHK(c, hipMalloc(&c->red, need * 8));
+    c->red_cap = need;
+  }
+  DevMesh M = dev_mesh(c);
+  double *fin = c->red + (size_t)nb * (c->neq + 1);
+  if (c->neq == 5) {
+    hipLaunchKernelGGL(k_totals_part<5>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, M, c->S, c->red, chunk);
+    hipLaunchKernelGGL(k_totals_final<5>, dim3(1), dim3(64), 0, c->stream, c->red, nb, fin);
+  } else {
+    hipLaunchKernelGGL(k_totals_part<4>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, M, c->S, c->red, chunk);
+    hipLaunchKernelGGL(k_totals_final<4>, dim3(1), dim3(64), 0, c->stream, c->red, nb, fin);
+  }
+  int rc = finish(c, "totals");
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, fin, (size_t)(c->neq + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
+}
+
+int fvg_check_state(fvg_ctx *c, int32_t *bad) {
+  if (!c || !bad) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  (void)hipSetDevice(c->device);
+  reset_err(c);
+  DevMesh M = dev_mesh(c);
+  if (c->neq == 5)
+    hipLaunchKernelGGL(k_check<5>, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream, M, c->S, c->RHS, c->err);
+  else
+    hipLaunchKernelGGL(k_check<4>, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream, M, c->S, c->RHS, c->err);
+  int rc = finish(c, "check");
+  if (rc) return rc;
+  return read_err(c, bad);
+}
+
+int fvg_sync(fvg_ctx *c) {
+  if (!c) return FVG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
+}
+
+}  // extern "C"

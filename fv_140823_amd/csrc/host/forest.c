@@ -1,0 +1,663 @@
+/*
+ * Host forest of octrees built directly as the reference's BRANCH/CELL/LEAF
+ * structures (strdata.h:76-168), with
+ *   - neighbour pointers with the semantics of elconn/elconnfin
+ *     (elconn.c:25-180, elconnfin.c:19-123): lfc_neigs 1 for a same-level or
+ *     coarser neighbour, 4 for finer ones in sub-face order, glob on the domain
+ *     boundary (lfc_neigs 1 at root level, 0 below);
+ *   - geometry by the formulas of normalvector.c:54-59, tagvector.c:43-53 and
+ *     meshproperties.c:15-111 (areas of hanging sub-faces from the finer
+ *     neighbour's face, volume as six tetrahedra, centre = node mean);
+ *   - the leaf list in depth-first kid order 0..7 (rebuildlist.c:34-91), i.e.
+ *     Morton order within each root;
+ *   - split by injection of the parent state (split.c:492-502) and merge by the
+ *     arithmetic mean of the kids (merge.c:92-107), with 2:1 face balance.
+ * Periodic pairing of opposite sides is an extension (the reference has none).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fvgpu.h"
+#include "fv_tables.h"
+
+struct fvh_forest {
+  RUN run;
+  CON con;
+  PAR par;
+  FOREST topo;
+  BRANCH *glob;
+  BRANCH **roots;
+  int nroots;
+  int nroot[3];
+  int periodic[3];
+  int bc[6];
+  int neq;
+  /* hash (level,i,j,k) -> branch */
+  BRANCH **ht;
+  uint64_t *hk;
+  size_t hcap, hcount;
+  /* leaves in list order */
+  BRANCH **leaves;
+  int nleaves;
+};
+
+/* ------------------------------------------------------------------ allocation */
+
+static size_t align8(size_t x) { return (x + 7) & ~(size_t)7; }
+
+/* One block per branch: BRANCH + neighbour tables + CELL + LEAF arrays. */
+static BRANCH *branch_alloc(int neq) {
+  size_t sz = 0;
+  size_t o_br = sz; sz = align8(sz + sizeof(BRANCH));
+  size_t o_ntr = sz; sz = align8(sz + 6 * sizeof(BRANCH **));
+  size_t o_ntr_rows = sz; sz = align8(sz + 6 * 4 * sizeof(BRANCH *));
+  size_t o_ints = sz; sz = align8(sz + 4 * 6 * sizeof(int));
+  size_t o_kids = sz; sz = align8(sz + 8 * sizeof(BRANCH *));
+  size_t o_cl = sz; sz = align8(sz + sizeof(CELL));
+  size_t o_norm = sz; sz = align8(sz + 9 * 6 * sizeof(double));
+  size_t o_area = sz; sz = align8(sz + 6 * sizeof(double *));
+  size_t o_area_rows = sz; sz = align8(sz + 6 * 4 * sizeof(double));
+  size_t o_nd = sz; sz = align8(sz + 8 * sizeof(ND));
+  size_t o_fc = sz; sz = align8(sz + 6 * sizeof(FC));
+  size_t o_el = sz; sz = align8(sz + sizeof(LEAF));
+  size_t o_s = sz; sz = align8(sz + 3 * (size_t)neq * sizeof(double));
+  size_t o_sf = sz; sz = align8(sz + 6 * sizeof(double *));
+  size_t o_sf_rows = sz; sz = align8(sz + 6 * (size_t)neq * sizeof(double));
+  size_t o_sg = sz; sz = align8(sz + (size_t)(neq + 1) * sizeof(double *));
+  size_t o_sg_rows = sz; sz = align8(sz + (size_t)(neq + 1) * 3 * sizeof(double));
+  size_t o_ff = sz; sz = align8(sz + 6 * sizeof(int));
+  char *blk = (char *)calloc(1, sz);
+  if (!blk) return NULL;
+  BRANCH *b = (BRANCH *)(blk + o_br);
+  b->neigtr = (BRANCH ***)(blk + o_ntr);
+  for (int f = 0; f < 6; ++f) b->neigtr[f] = (BRANCH **)(blk + o_ntr_rows) + 4 * f;
+  int *ints = (int *)(blk + o_ints);
+  b->neigfc = ints;
+  b->lfc_neigs = ints + 6;
+  b->nsfc = ints + 12;
+  b->neigag = ints + 18;
+  b->kids = (BRANCH **)(blk + o_kids);
+  CELL *cl = (CELL *)(blk + o_cl);
+  double *nv = (double *)(blk + o_norm);
+  cl->nx = nv; cl->ny = nv + 6; cl->nz = nv + 12;
+  cl->nxt1 = nv + 18; cl->nyt1 = nv + 24; cl->nzt1 = nv + 30;
+  cl->nxt2 = nv + 36; cl->nyt2 = nv + 42; cl->nzt2 = nv + 48;
+  cl->Area = (double **)(blk + o_area);
+  for (int f = 0; f < 6; ++f) cl->Area[f] = (double *)(blk + o_area_rows) + 4 * f;
+  cl->nd = (ND *)(blk + o_nd);
+  cl->fc = (FC *)(blk + o_fc);
+  b->cl = cl;
+  LEAF *el = (LEAF *)(blk + o_el);
+  double *s = (double *)(blk + o_s);
+  el->S = s; el->SN = s + neq; el->RHS = s + 2 * neq;
+  el->SF = (double **)(blk + o_sf);
+  for (int f = 0; f < 6; ++f) el->SF[f] = (double *)(blk + o_sf_rows) + (size_t)neq * f;
+  el->SG = (double **)(blk + o_sg);
+  for (int v = 0; v <= neq; ++v) el->SG[v] = (double *)(blk + o_sg_rows) + 3 * v;
+  el->flux_flag = (int *)(blk + o_ff);
+  el->flux_face = NULL;
+  b->el = el;
+  b->type = 0;
+  b->nlnd = 8;
+  b->nlfc = 6;
+  b->nsplit = 0;
+  b->fv_index = -1;
+  return b;
+}
+
+/* ------------------------------------------------------------------ hash */
+
+static uint64_t hkey(int level, const int ijk[3]) {
+  return ((uint64_t)(level + 1) << 57) | ((uint64_t)(uint32_t)ijk[0] << 38) |
+         ((uint64_t)(uint32_t)ijk[1] << 19) | (uint64_t)(uint32_t)ijk[2];
+}
+static uint64_t hmix(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return k;
+}
+static int ht_reserve(struct fvh_forest *fr, size_t need) {
+  if (fr->hcap >= 2 * need + 16) return 0;
+  size_t cap = 1024;
+  while (cap < 2 * need + 16) cap <<= 1;
+  BRANCH **ht = (BRANCH **)calloc(cap, sizeof(BRANCH *));
+  uint64_t *hk = (uint64_t *)calloc(cap, sizeof(uint64_t));
+  if (!ht || !hk) { free(ht); free(hk); return FVG_ERR_NOMEM; }
+  for (size_t i = 0; i < fr->hcap; ++i) {
+    if (!fr->ht[i]) continue;
+    size_t p = hmix(fr->hk[i]) & (cap - 1);
+    while (ht[p]) p = (p + 1) & (cap - 1);
+    ht[p] = fr->ht[i]; hk[p] = fr->hk[i];
+  }
+  free(fr->ht); free(fr->hk);
+  fr->ht = ht; fr->hk = hk; fr->hcap = cap;
+  return 0;
+}
+static void ht_clear(struct fvh_forest *fr) {
+  if (fr->ht) memset(fr->ht, 0, fr->hcap * sizeof(BRANCH *));
+  fr->hcount = 0;
+}
+static int ht_insert(struct fvh_forest *fr, BRANCH *b) {
+  int rc = ht_reserve(fr, fr->hcount + 1);
+  if (rc) return rc;
+  uint64_t k = hkey(b->level, b->fv_ijk);
+  size_t p = hmix(k) & (fr->hcap - 1);
+  while (fr->ht[p]) {
+    if (fr->hk[p] == k) { fr->ht[p] = b; return 0; }
+    p = (p + 1) & (fr->hcap - 1);
+  }
+  fr->ht[p] = b; fr->hk[p] = k; fr->hcount++;
+  return 0;
+}
+static BRANCH *ht_find(const struct fvh_forest *fr, int level, const int ijk[3]) {
+  if (!fr->hcap) return NULL;
+  uint64_t k = hkey(level, ijk);
+  size_t p = hmix(k) & (fr->hcap - 1);
+  while (fr->ht[p]) {
+    if (fr->hk[p] == k) return fr->ht[p];
+    p = (p + 1) & (fr->hcap - 1);
+  }
+  return NULL;
+}
+
+/* ------------------------------------------------------------------ geometry */
+
+static int ncell_axis(const struct fvh_forest *fr, int level, int a) {
+  return fr->nroot[a] << (level - 1);
+}
+
+static void set_nodes(struct fvh_forest *fr, BRANCH *b) {
+  double L[3];
+  int N[3];
+  for (int a = 0; a < 3; ++a) {
+    L[a] = fr->topo.fv_x1[a] - fr->topo.fv_x0[a];
+    N[a] = ncell_axis(fr, b->level, a);
+  }
+  for (int n = 0; n < 8; ++n) {
+    int bit[3] = {n & 1, (n >> 1) & 1, (n >> 2) & 1};
+    double x[3];
+    for (int a = 0; a < 3; ++a)
+      x[a] = fr->topo.fv_x0[a] + ((double)(b->fv_ijk[a] + bit[a]) * L[a]) / (double)N[a];
+    b->cl->nd[n].num = n;
+    b->cl->nd[n].x = x[0];
+    b->cl->nd[n].y = x[1];
+    b->cl->nd[n].z = x[2];
+  }
+}
+
+/* normalvector.c:54-59 and tagvector.c:43-53 */
+static void set_normals(BRANCH *b) {
+  CELL *cl = b->cl;
+  for (int f = 0; f < 6; ++f) {
+    const ND *p0 = &cl->nd[FV_FACE_NODES[f][0]];
+    const ND *p1 = &cl->nd[FV_FACE_NODES[f][1]];
+    const ND *p2 = &cl->nd[FV_FACE_NODES[f][2]];
+    double cx = (p1->y - p0->y) * (p2->z - p0->z) - (p2->y - p0->y) * (p1->z - p0->z);
+    double cy = (p1->x - p0->x) * (p2->z - p0->z) - (p2->x - p0->x) * (p1->z - p0->z);
+    double cz = (p1->x - p0->x) * (p2->y - p0->y) - (p2->x - p0->x) * (p1->y - p0->y);
+    double norm = sqrt(cx * cx + cy * cy + cz * cz);
+    cl->nx[f] = cx / norm;
+    cl->ny[f] = -cy / norm;
+    cl->nz[f] = cz / norm;
+    double ex = p1->x - p0->x, ey = p1->y - p0->y, ez = p1->z - p0->z;
+    double n1 = sqrt(ey * ey + ex * ex + ez * ez);
+    cl->nxt1[f] = ex / n1;
+    cl->nyt1[f] = ey / n1;
+    cl->nzt1[f] = ez / n1;
+    double qx = cl->ny[f] * cl->nzt1[f] - cl->nyt1[f] * cl->nz[f];
+    double qy = cl->nx[f] * cl->nzt1[f] - cl->nxt1[f] * cl->nz[f];
+    double qz = cl->nx[f] * cl->nyt1[f] - cl->nxt1[f] * cl->ny[f];
+    double n2 = sqrt(qx * qx + qy * qy + qz * qz);
+    cl->nxt2[f] = qx / n2;
+    cl->nyt2[f] = -qy / n2;
+    cl->nzt2[f] = qz / n2;
+  }
+}
+
+/* area of face fopp of `o` as two triangles (meshproperties.c:54-62) */
+static double face_area(const BRANCH *o, int fopp) {
+  double A = 0.0;
+  const ND *nd = o->cl->nd;
+  for (int it = 0; it < 2; ++it) {
+    const ND *q0 = &nd[FV_FACE_NODES[fopp][0]];
+    const ND *q1 = &nd[FV_FACE_NODES[fopp][1 + it]];
+    const ND *q2 = &nd[FV_FACE_NODES[fopp][2 + it]];
+    double dx1 = q1->x - q0->x, dx2 = q2->x - q0->x;
+    double dy1 = q1->y - q0->y, dy2 = q2->y - q0->y;
+    double dz1 = q1->z - q0->z, dz2 = q2->z - q0->z;
+    double a = dy1 * dz2 - dz1 * dy2, bb = dz1 * dx2 - dx1 * dz2, c = dx1 * dy2 - dy1 * dx2;
+    A += fabs(0.5 * sqrt(a * a + bb * bb + c * c));
+  }
+  return A;
+}
+
+/* meshproperties.c:15-111 */
+static void set_props(BRANCH *b) {
+  CELL *cl = b->cl;
+  for (int f = 0; f < 6; ++f) {
+    for (int k = 0; k < 4; ++k) cl->Area[f][k] = 0.0;
+    for (int k = 0; k < b->nsfc[f]; ++k) {
+      if (b->nsfc[f] == 1) cl->Area[f][k] = face_area(b, f);
+      else cl->Area[f][k] = face_area(b->neigtr[f][k], b->neigfc[f]);
+    }
+  }
+  double V = 0.0;
+  for (int it = 0; it < 6; ++it) {
+    double dx[3], dy[3], dz[3];
+    for (int in = 0; in < 3; ++in) {
+      const ND *q = &cl->nd[FV_VOL_TETS[it][in]];
+      dx[in] = q->x - cl->nd[0].x;
+      dy[in] = q->y - cl->nd[0].y;
+      dz[in] = q->z - cl->nd[0].z;
+    }
+    double c0 = dy[0] * dz[1] - dz[0] * dy[1];
+    double c1 = dz[0] * dx[1] - dx[0] * dz[1];
+    double c2 = dx[0] * dy[1] - dy[0] * dx[1];
+    V += (c0 * dx[2] + c1 * dy[2] + c2 * dz[2]) / 6.0;
+  }
+  cl->Vol = V;
+  cl->xc = cl->yc = cl->zc = 0.0;
+  for (int n = 0; n < 8; ++n) {
+    cl->xc += cl->nd[n].x / 8;
+    cl->yc += cl->nd[n].y / 8;
+    cl->zc += cl->nd[n].z / 8;
+  }
+}
+
+/* ------------------------------------------------------------------ topology */
+
+static BRANCH *new_branch(struct fvh_forest *fr, BRANCH *prnt, int level, const int ijk[3],
+                          int root) {
+  BRANCH *b = branch_alloc(fr->neq);
+  if (!b) return NULL;
+  b->level = level;
+  b->prnt = prnt;
+  b->root = root;
+  for (int a = 0; a < 3; ++a) b->fv_ijk[a] = ijk[a];
+  for (int f = 0; f < 6; ++f) {
+    b->cl->fc[f].num = f;
+    b->cl->fc[f].type = 0;
+    b->cl->fc[f].nnds = 4;
+    b->cl->fc[f].bc = 0;
+  }
+  set_nodes(fr, b);
+  return b;
+}
+
+static int split_branch(struct fvh_forest *fr, BRANCH *b) {
+  if (b->nkids) return 0;
+  for (int c = 0; c < 8; ++c) {
+    int ijk[3] = {2 * b->fv_ijk[0] + (c & 1), 2 * b->fv_ijk[1] + ((c >> 1) & 1),
+                  2 * b->fv_ijk[2] + ((c >> 2) & 1)};
+    BRANCH *k = new_branch(fr, b, b->level + 1, ijk, b->root);
+    if (!k) return FVG_ERR_NOMEM;
+    for (int v = 0; v < fr->neq; ++v) {
+      k->el->S[v] = b->el->S[v];
+      k->el->SG[v][0] = b->el->SG[v][0];
+      k->el->SG[v][1] = b->el->SG[v][1];
+      k->el->SG[v][2] = b->el->SG[v][2];
+    }
+    b->kids[c] = k;
+    int rc = ht_insert(fr, k);
+    if (rc) return rc;
+  }
+  b->nkids = 8;
+  b->nsplit = 8;
+  return 0;
+}
+
+static void free_subtree(BRANCH *b) {
+  for (int c = 0; c < b->nkids; ++c) free_subtree(b->kids[c]);
+  free(b);
+}
+
+static int merge_branch(BRANCH *b, int neq) {
+  for (int v = 0; v < neq; ++v) {
+    b->el->S[v] = 0.0;
+    b->el->SG[v][0] = b->el->SG[v][1] = b->el->SG[v][2] = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const BRANCH *k = b->kids[c];
+      b->el->S[v] += k->el->S[v] / ((double)8);
+      for (int d = 0; d < 3; ++d) b->el->SG[v][d] += k->el->SG[v][d] * k->cl->Vol / b->cl->Vol;
+    }
+  }
+  for (int c = 0; c < 8; ++c) { free_subtree(b->kids[c]); b->kids[c] = NULL; }
+  b->nkids = 0;
+  b->nsplit = 0;
+  return 0;
+}
+
+static void collect_leaves(BRANCH *b, BRANCH ***out, int *n, int *cap) {
+  if (b->nkids == 0) {
+    if (*n == *cap) {
+      *cap = *cap ? 2 * *cap : 1024;
+      *out = (BRANCH **)realloc(*out, (size_t)*cap * sizeof(BRANCH *));
+    }
+    (*out)[(*n)++] = b;
+    return;
+  }
+  for (int c = 0; c < 8; ++c) collect_leaves(b->kids[c], out, n, cap);
+}
+
+static void hash_subtree(struct fvh_forest *fr, BRANCH *b, int *rc) {
+  if (*rc) return;
+  *rc = ht_insert(fr, b);
+  for (int c = 0; c < b->nkids; ++c) hash_subtree(fr, b->kids[c], rc);
+}
+
+/* neighbour across face f of leaf b: fills neigtr/lfc_neigs/neigfc/nsfc and fc[f].bc */
+static int resolve_face(struct fvh_forest *fr, BRANCH *b, int f) {
+  int a = FV_FACE_AXIS[f], s = FV_FACE_SIGN[f];
+  int n[3] = {b->fv_ijk[0], b->fv_ijk[1], b->fv_ijk[2]};
+  int Na = ncell_axis(fr, b->level, a);
+  n[a] += s;
+  for (int k = 0; k < 4; ++k) b->neigtr[f][k] = NULL;
+  b->neigag[f] = 0;
+  if (n[a] < 0 || n[a] >= Na) {
+    if (!fr->periodic[a]) {
+      b->neigtr[f][0] = fr->glob;
+      b->lfc_neigs[f] = (b->level == 1) ? 1 : 0;
+      b->neigfc[f] = FV_FACE_OPP[f];
+      b->nsfc[f] = 1;
+      b->cl->fc[f].bc = fr->bc[FV_FACE_SIDE[f]];
+      return 0;
+    }
+    n[a] = (n[a] + Na) % Na;
+  }
+  b->cl->fc[f].bc = 0;
+  b->neigfc[f] = FV_FACE_OPP[f];
+  BRANCH *nb = ht_find(fr, b->level, n);
+  if (nb) {
+    if (nb->nkids == 0) {
+      b->neigtr[f][0] = nb;
+      b->lfc_neigs[f] = 1;
+    } else {
+      b->lfc_neigs[f] = 4;
+      for (int k = 0; k < 4; ++k) {
+        int kid = FV_FACE_NODES[f][k] ^ (1 << a);
+        BRANCH *kb = nb->kids[kid];
+        if (kb->nkids) return FVG_ERR_STATE; /* 2:1 violated */
+        b->neigtr[f][k] = kb;
+      }
+    }
+  } else {
+    if (b->level <= 1) return FVG_ERR_STATE;
+    int m[3] = {n[0] >> 1, n[1] >> 1, n[2] >> 1};
+    BRANCH *cb = ht_find(fr, b->level - 1, m);
+    if (!cb || cb->nkids) return FVG_ERR_STATE; /* 2:1 violated */
+    b->neigtr[f][0] = cb;
+    b->lfc_neigs[f] = 1;
+  }
+  b->nsfc[f] = b->lfc_neigs[f] > 1 ? b->lfc_neigs[f] : 1;
+  return 0;
+}
+
+/* Leaf list, hash, neighbours and geometry after any topology change. */
+static int rebuild(struct fvh_forest *fr) {
+  int rc = 0;
+  ht_clear(fr);
+  for (int r = 0; r < fr->nroots; ++r) hash_subtree(fr, fr->roots[r], &rc);
+  if (rc) return rc;
+  int n = 0, cap = fr->nleaves > 0 ? fr->nleaves : 0;
+  BRANCH **lv = fr->leaves;
+  for (int r = 0; r < fr->nroots; ++r) collect_leaves(fr->roots[r], &lv, &n, &cap);
+  fr->leaves = lv;
+  fr->nleaves = n;
+  for (int i = 0; i < n; ++i) {
+    BRANCH *b = lv[i];
+    b->fv_index = i;
+    b->lprv = i ? lv[i - 1] : fr->glob;
+    b->lnxt = (i + 1 < n) ? lv[i + 1] : NULL;
+    for (int f = 0; f < 6; ++f) {
+      rc = resolve_face(fr, b, f);
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    set_normals(lv[i]);
+    set_props(lv[i]);
+  }
+  fr->topo.locl = n ? lv[0] : NULL;
+  fr->topo.nleaves = n;
+  fr->topo.tleaves = n;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ public */
+
+int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3], int level,
+                      const int bc[6], const int periodic[3], int neq, fvh_forest **out) {
+  if (!out || !nroot || !x0 || !x1 || !bc || level < 0 || level > 16 || neq < 1 || neq > 8)
+    return FVG_ERR_ARG;
+  for (int a = 0; a < 3; ++a)
+    if (nroot[a] < 1 || !(x1[a] > x0[a])) return FVG_ERR_ARG;
+  struct fvh_forest *fr = (struct fvh_forest *)calloc(1, sizeof(*fr));
+  if (!fr) return FVG_ERR_NOMEM;
+  fr->neq = neq;
+  for (int a = 0; a < 3; ++a) {
+    fr->nroot[a] = nroot[a];
+    fr->periodic[a] = periodic ? (periodic[a] != 0) : 0;
+    fr->topo.fv_nroot[a] = nroot[a];
+    fr->topo.fv_x0[a] = x0[a];
+    fr->topo.fv_x1[a] = x1[a];
+    fr->topo.fv_period[a] = fr->periodic[a] ? (x1[a] - x0[a]) : 0.0;
+  }
+  for (int s = 0; s < 6; ++s) fr->bc[s] = bc[s];
+  fr->run.con = &fr->con;
+  fr->run.par = &fr->par;
+  fr->run.topo = &fr->topo;
+  fr->con.neq = neq;
+  fr->con.neq_temp = neq;
+  fr->con.nprimitiv = neq;
+  fr->con.level = level;
+  fr->glob = branch_alloc(neq);
+  if (!fr->glob) { free(fr); return FVG_ERR_NOMEM; }
+  fr->glob->level = 0;
+  fr->glob->root = -1;
+  fr->glob->prnt = fr->glob;
+  fr->glob->type = -1;
+  fr->topo.glob = fr->glob;
+  fr->nroots = nroot[0] * nroot[1] * nroot[2];
+  fr->topo.ntrees = fr->nroots;
+  fr->roots = (BRANCH **)calloc((size_t)fr->nroots, sizeof(BRANCH *));
+  int r = 0;
+  for (int k = 0; k < nroot[2]; ++k)
+    for (int j = 0; j < nroot[1]; ++j)
+      for (int i = 0; i < nroot[0]; ++i) {
+        int ijk[3] = {i, j, k};
+        BRANCH *b = new_branch(fr, fr->glob, 1, ijk, r);
+        if (!b) { fvh_forest_destroy(fr); return FVG_ERR_NOMEM; }
+        fr->roots[r++] = b;
+        int rc = ht_insert(fr, b);
+        if (rc) { fvh_forest_destroy(fr); return rc; }
+      }
+  /* uniform refinement, breadth first */
+  for (int l = 0; l < level; ++l) {
+    int n = 0, cap = 0;
+    BRANCH **lv = NULL;
+    for (int q = 0; q < fr->nroots; ++q) collect_leaves(fr->roots[q], &lv, &n, &cap);
+    int rc = ht_reserve(fr, fr->hcount + (size_t)8 * n);
+    for (int i = 0; i < n && !rc; ++i) rc = split_branch(fr, lv[i]);
+    free(lv);
+    if (rc) { fvh_forest_destroy(fr); return rc; }
+  }
+  int rc = rebuild(fr);
+  if (rc) { fvh_forest_destroy(fr); return rc; }
+  *out = fr;
+  return FVG_OK;
+}
+
+void fvh_forest_destroy(fvh_forest *fr) {
+  if (!fr) return;
+  for (int r = 0; r < fr->nroots; ++r)
+    if (fr->roots[r]) free_subtree(fr->roots[r]);
+  free(fr->roots);
+  free(fr->glob);
+  free(fr->ht);
+  free(fr->hk);
+  free(fr->leaves);
+  free(fr);
+}
+
+RUN *fvh_forest_run(fvh_forest *fr) { return fr ? &fr->run : NULL; }
+int fvh_forest_nleaves(fvh_forest *fr) { return fr ? fr->nleaves : FVG_ERR_ARG; }
+
+int fvh_forest_max_level(fvh_forest *fr) {
+  if (!fr) return FVG_ERR_ARG;
+  int m = 0;
+  for (int i = 0; i < fr->nleaves; ++i)
+    if (fr->leaves[i]->level > m) m = fr->leaves[i]->level;
+  return m;
+}
+
+int fvh_forest_leaf_info(fvh_forest *fr, double *xc, int32_t *level) {
+  if (!fr) return FVG_ERR_ARG;
+  for (int i = 0; i < fr->nleaves; ++i) {
+    const BRANCH *b = fr->leaves[i];
+    if (xc) {
+      xc[3 * i] = b->cl->xc;
+      xc[3 * i + 1] = b->cl->yc;
+      xc[3 * i + 2] = b->cl->zc;
+    }
+    if (level) level[i] = b->level;
+  }
+  return FVG_OK;
+}
+
+/* var-major layouts as FVG_FIELD_* */
+static int field_io(fvh_forest *fr, int which, double *buf, int to_leaves) {
+  if (!fr || !buf) return FVG_ERR_ARG;
+  const int N = fr->nleaves, neq = fr->neq;
+  for (int i = 0; i < N; ++i) {
+    LEAF *el = fr->leaves[i]->el;
+    switch (which) {
+      case FVG_FIELD_S:
+      case FVG_FIELD_SN:
+      case FVG_FIELD_RHS: {
+        double *p = which == FVG_FIELD_S ? el->S : (which == FVG_FIELD_SN ? el->SN : el->RHS);
+        for (int v = 0; v < neq; ++v) {
+          if (to_leaves) p[v] = buf[(size_t)v * N + i];
+          else buf[(size_t)v * N + i] = p[v];
+        }
+        break;
+      }
+      case FVG_FIELD_SF:
+        for (int f = 0; f < 6; ++f)
+          for (int v = 0; v < neq; ++v) {
+            size_t o = ((size_t)f * neq + v) * N + i;
+            if (to_leaves) el->SF[f][v] = buf[o];
+            else buf[o] = el->SF[f][v];
+          }
+        break;
+      case FVG_FIELD_SG:
+        for (int v = 0; v <= neq; ++v)
+          for (int d = 0; d < 3; ++d) {
+            size_t o = ((size_t)v * 3 + d) * N + i;
+            if (to_leaves) el->SG[v][d] = buf[o];
+            else buf[o] = el->SG[v][d];
+          }
+        break;
+      default:
+        return FVG_ERR_ARG;
+    }
+  }
+  return FVG_OK;
+}
+
+int fvh_forest_set_field(fvh_forest *fr, int which, const double *src) {
+  return field_io(fr, which, (double *)src, 1);
+}
+int fvh_forest_get_field(fvh_forest *fr, int which, double *dst) {
+  return field_io(fr, which, dst, 0);
+}
+
+/* true if some leaf adjacent to leaf b across a face is two or more levels finer */
+static int needs_balance(const struct fvh_forest *fr, const BRANCH *b) {
+  for (int f = 0; f < 6; ++f) {
+    int a = FV_FACE_AXIS[f];
+    int n[3] = {b->fv_ijk[0], b->fv_ijk[1], b->fv_ijk[2]};
+    int Na = ncell_axis(fr, b->level, a);
+    n[a] += FV_FACE_SIGN[f];
+    if (n[a] < 0 || n[a] >= Na) {
+      if (!fr->periodic[a]) continue;
+      n[a] = (n[a] + Na) % Na;
+    }
+    const BRANCH *nb = ht_find(fr, b->level, n);
+    if (!nb || !nb->nkids) continue;
+    for (int k = 0; k < 4; ++k)
+      if (nb->kids[FV_FACE_NODES[f][k] ^ (1 << a)]->nkids) return 1;
+  }
+  return 0;
+}
+
+/* a merge of parent p keeps 2:1 only if no kid has a finer face neighbour */
+static int merge_allowed(const struct fvh_forest *fr, const BRANCH *p) {
+  for (int c = 0; c < 8; ++c) {
+    const BRANCH *k = p->kids[c];
+    for (int f = 0; f < 6; ++f) {
+      int a = FV_FACE_AXIS[f];
+      int n[3] = {k->fv_ijk[0], k->fv_ijk[1], k->fv_ijk[2]};
+      int Na = ncell_axis(fr, k->level, a);
+      n[a] += FV_FACE_SIGN[f];
+      if (n[a] < 0 || n[a] >= Na) {
+        if (!fr->periodic[a]) continue;
+        n[a] = (n[a] + Na) % Na;
+      }
+      const BRANCH *nb = ht_find(fr, k->level, n);
+      if (nb && nb->nkids) return 0;
+    }
+  }
+  return 1;
+}
+
+int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
+  if (!fr || !flag) return FVG_ERR_ARG;
+  const int N = fr->nleaves;
+  BRANCH **lv = fr->leaves;
+  for (int i = 0; i < N; ++i) {
+    lv[i]->split = (flag[i] > 0 && lv[i]->level < max_level);
+    lv[i]->merge = (flag[i] < 0);
+  }
+  /* merge candidates on the old tree: parents whose 8 kids are leaves flagged -1 */
+  int ncand = 0;
+  BRANCH **cand = (BRANCH **)malloc((size_t)(N / 8 + 1) * sizeof(BRANCH *));
+  if (!cand) return FVG_ERR_NOMEM;
+  for (int i = 0; i < N; ++i) {
+    BRANCH *b = lv[i];
+    if (b->level <= 1 || b->prnt->kids[0] != b) continue;
+    BRANCH *p = b->prnt;
+    int ok = 1;
+    for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0 && p->kids[c]->merge);
+    if (ok) cand[ncand++] = p;
+  }
+  int rc = ht_reserve(fr, fr->hcount + (size_t)8 * N);
+  for (int i = 0; i < N && !rc; ++i)
+    if (lv[i]->split) rc = split_branch(fr, lv[i]);
+  /* 2:1 face balance by further splits */
+  while (!rc) {
+    int n = 0, cap = 0, changed = 0;
+    BRANCH **cur = NULL;
+    for (int r = 0; r < fr->nroots; ++r) collect_leaves(fr->roots[r], &cur, &n, &cap);
+    for (int i = 0; i < n && !rc; ++i)
+      if (needs_balance(fr, cur[i])) { rc = split_branch(fr, cur[i]); ++changed; }
+    free(cur);
+    if (!changed) break;
+  }
+  if (rc) { free(cand); return rc; }
+  /* decide every merge on the same (post-split) tree, then apply */
+  int nm = 0;
+  for (int q = 0; q < ncand; ++q) {
+    BRANCH *p = cand[q];
+    int ok = 1;
+    for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0);
+    if (ok && merge_allowed(fr, p)) cand[nm++] = p;
+  }
+  for (int q = 0; q < nm; ++q) merge_branch(cand[q], fr->neq);
+  free(cand);
+  /* fr->leaves still lists freed kids of merged parents: rebuild before any use */
+  rc = rebuild(fr);
+  if (rc) return rc;
+  for (int i = 0; i < fr->nleaves; ++i) fr->leaves[i]->split = fr->leaves[i]->merge = 0;
+  return fr->nleaves;
+}

@@ -1,0 +1,53 @@
+"""Host time loop around the device path: the stage sequence of the reference's
+(undefined) advancetimeexplicit -- RK(0), then per stage faceinterp, gradcalc,
+RHS, RK(k) (loop.c:35-74, RK.c:16-51) -- with a CFL time step and a host mesh
+adaptation every `adapt_every` steps (meshadaptation.c:23-57): the device
+state is copied into the leaves, the forest is adapted, re-flattened and
+uploaded again.
+"""
+from __future__ import annotations
+
+import time
+from typing import Callable, Optional
+
+import numpy as np
+
+from ._capi import FIELD_SG
+from .core import FlatMesh, Forest, Solver
+
+
+def remesh(forest: Forest, solver: Solver, flags_fn: Callable, max_level: int) -> dict:
+    """Adapt the forest on the device state and re-upload.  Returns timings."""
+    t0 = time.perf_counter()
+    S = solver.get_state()
+    forest.set_state(S)
+    solver.gradcalc()
+    SG = solver.get_field(FIELD_SG)
+    t1 = time.perf_counter()
+    flags = flags_fn(forest, SG, S, max_level)
+    forest.adapt(flags, max_level)
+    t2 = time.perf_counter()
+    flat = FlatMesh(forest.run)
+    solver.set_mesh(flat)
+    solver.set_state(forest.get_state())
+    t3 = time.perf_counter()
+    return dict(download_s=t1 - t0, adapt_s=t2 - t1, flatten_upload_s=t3 - t2, nleaves=forest.nleaves)
+
+
+def evolve(forest: Forest, solver: Solver, steps: int, cfl: float, method: int = 3,
+           adapt_every: int = 0, flags_fn: Optional[Callable] = None, max_level: int = 0,
+           dt_fixed: Optional[float] = None, log: Optional[Callable] = None) -> dict:
+    t = 0.0
+    stats = dict(steps=0, cell_updates=0, remesh=[])
+    for n in range(steps):
+        if adapt_every and flags_fn is not None and n > 0 and n % adapt_every == 0:
+            stats["remesh"].append(remesh(forest, solver, flags_fn, max_level))
+            if log:
+                log(f"step {n}: remeshed to {forest.nleaves} leaves")
+        dt = dt_fixed if dt_fixed is not None else solver.cfl_dt(cfl)
+        solver.step(dt, method)
+        t += dt
+        stats["steps"] += 1
+        stats["cell_updates"] += solver.ncell * method
+    stats["time"] = t
+    return stats

@@ -1,0 +1,193 @@
+/*
+ * fvgpu.h — C ABI of the MI355X (gfx950) right-hand-side + Runge-Kutta path.
+ *
+ * Three groups of entry points live in libfv140823.so:
+ *
+ *  1. fvh_*  host octree + flattener.  The forest is a tree of the reference's
+ *     BRANCH/LEAF/CELL structs (fv_strdata.h); fvh_flatten_run() turns the
+ *     leaf list of any RUN into the struct-of-arrays mesh the kernels read.
+ *  2. fvg_*  device context: state upload/download and the kernels that
+ *     replace faceinterp() (faceinterp.c:21), gradcalc() (gradcalc.c:9),
+ *     RHS() (RHS.c:3 -> residual_faces_AMR.c:10) and RK() (RK.c:3).
+ *  3. RHS/RK/faceinterp/gradcalc with the reference signatures (declared in
+ *     fv_strdata.h), bound to a device context by fv_gpu_attach().
+ *
+ * All functions return an int status (FVG_OK or a negative FVG_ERR_*), except
+ * where stated.  A context owns one HIP stream created at default priority;
+ * calls on one context must not overlap from several host threads.
+ */
+#ifndef FVGPU_H
+#define FVGPU_H
+
+#include <stdint.h>
+#include "fv_strdata.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVG_API_VERSION 1
+
+enum {
+  FVG_OK = 0,
+  FVG_ERR_ARG = -1,         /* bad argument or size */
+  FVG_ERR_HIP = -2,         /* a HIP runtime call failed */
+  FVG_ERR_STATE = -3,       /* call out of order (e.g. no mesh yet) */
+  FVG_ERR_NONFINITE = -4,   /* NaN/Inf in RHS or state; first bad cell via fvg_first_bad_cell */
+  FVG_ERR_UNSUPPORTED = -5, /* option with no defined behaviour (bc 3/4, model 4, ...) */
+  FVG_ERR_NOMEM = -6
+};
+
+/* precision forms */
+enum { FVG_PLAIN = 0, FVG_MIXED = 1 };
+/* limiters: minmod is the reference's hard-wired one (reconctruct_V3.c:19-30) */
+enum { FVG_MINMOD = 0, FVG_VANALBADA = 1 };
+/* fields for fvg_get_field / fvg_device_ptr */
+enum {
+  FVG_FIELD_S = 0,    /* [neq][ncell] f64 */
+  FVG_FIELD_SN = 1,   /* [neq][ncell] f64 */
+  FVG_FIELD_RHS = 2,  /* [neq][ncell] f64, already multiplied by -Area, not divided by Vol */
+  FVG_FIELD_SF = 3,   /* [6][neq][ncell] (f64 plain / f32 mixed on device; f64 on copy-out) */
+  FVG_FIELD_SG = 4,   /* [neq+1][3][ncell] f64 (gradcalc output) */
+  FVG_FIELD_FACEFLUX = 5 /* [neq][nface] face flux of the owner side (f64 / f32) */
+};
+
+typedef struct fvg_params {
+  int model;      /* 1 ideal/stiffened gas (neq 5); 2 barotropic liquid-vapour (neq 4);
+                     3 barotropic two-phase mixture with mass fraction (neq 5) */
+  int ns;         /* 1: viscous fluxes (residual_faces_AMR.c:129) */
+  int ischeme;    /* 1: first order face states (facevalues.c); 2: MUSCL (faceinterp.c) */
+  int limiter;    /* FVG_MINMOD | FVG_VANALBADA */
+  int precision;  /* FVG_PLAIN | FVG_MIXED */
+  int visc_work;  /* viscous work in the energy row: the reference's con->eos==1 (flux_viscous.c:227) */
+  double gamma;   /* par->matergama[0] */
+  double pinf;    /* par->materpinf[0] */
+  double mu;      /* par->matervisc[0] */
+  double ymin;    /* model 3 mass-fraction clip at ymin/1000 (facercnstr.c:43-44) */
+  /* model 2 constants (barotropic_eos.c:8-12) */
+  double baro_nexp, baro_beta, baro_cbar, baro_psat, baro_rsat;
+  /* model 3 constants (barotropic_eos.c:40-46) */
+  double mix_RG, mix_Tref, mix_rsat, mix_psat, mix_cliq, mix_cmix;
+  double inlet[5];        /* conserved ghost state of bc type 1 (bc_inlet.c) */
+  double vanalbada_eps;   /* epsilon of the smooth van Albada limiter */
+} fvg_params;
+
+/* Flattened mesh (all arrays host memory, read by fvg_set_mesh). */
+typedef struct fvg_mesh {
+  int32_t ncell;
+  int32_t nface;          /* unique faces */
+  int32_t nface_axis[3];  /* faces 0..nface_axis[0]-1 are x faces, then y, then z (incl. boundary) */
+  int32_t nquad;          /* coarse faces with 4 finer neighbours */
+  int32_t nlevel;         /* rows in the per-level tables */
+  int32_t neq;
+  const uint32_t *cflag;  /* [ncell] bits 2f..2f+1: class of face f (0 same level, 1 finer (quad),
+                             2 coarser, 3 domain boundary); bits 16-23: level */
+  const int32_t *nbr;     /* [6][ncell]: same/coarser -> neighbour cell; finer -> quad index;
+                             boundary -> -1-bc */
+  const int32_t *quad_nbr;  /* [nquad][4] fine neighbours in sub-face order */
+  const int32_t *face_L;  /* [nface] owner cell */
+  const int32_t *face_R;  /* [nface] other cell (= owner for boundary faces) */
+  const uint32_t *face_meta;/* [nface] bits 0-2 owner face id, 3-5 other face id,
+                               6-7 bc (0 interior, 1 inlet, 2 outflow), 8-15 level of the finer side */
+  const int32_t *gslot;   /* [6][ncell] (face<<1)|other_side; class finer: first of 4 consecutive
+                             owned faces; -1: no flux through this face */
+  const double *h;        /* [nlevel][3] cell size per level and axis */
+  const double *dist;     /* [nlevel][3][3][2] (d_cf, d_cc) per level, axis, class (distance.c) */
+  const double *area;     /* [nlevel][3] face area per level of the finer side and axis */
+  const double *vol;      /* [nlevel] cell volume */
+} fvg_mesh;
+
+typedef struct fvg_ctx fvg_ctx;
+typedef struct fvh_forest fvh_forest;
+typedef struct fvh_flat fvh_flat;
+
+/* ---------------------------------------------------------------- params */
+/* Fill defaults: gamma 1.4, pinf 0, mu 0, the reference's EOS constants, minmod, plain. */
+void fvg_params_default(fvg_params *p, int model);
+int fvg_neq(int model);     /* 5, 4, 5; -1 unsupported */
+
+/* ---------------------------------------------------------------- host forest */
+/* Build a forest of nroot[0]*nroot[1]*nroot[2] hexahedral roots over the box [x0,x1],
+ * refined uniformly by `level` (level 0 = the roots).  bc[6] are the side types in the
+ * reference's order -x,+x,-y,+y,-z,+z (inputset.c:99-104); periodic[a]!=0 pairs the two
+ * sides of axis a.  neq: conserved variables per leaf. */
+int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3], int level,
+                      const int bc[6], const int periodic[3], int neq, fvh_forest **out);
+void fvh_forest_destroy(fvh_forest *fr);
+RUN *fvh_forest_run(fvh_forest *fr);          /* the RUN whose topo/con/par describe this forest */
+int fvh_forest_nleaves(fvh_forest *fr);       /* returns the count (>=0) */
+int fvh_forest_max_level(fvh_forest *fr);     /* deepest leaf level (roots = 1) */
+/* Per leaf (list order): centre xc[3N], level[N] (reference levels, roots = 1). */
+int fvh_forest_leaf_info(fvh_forest *fr, double *xc, int32_t *level);
+/* Copy S/SN/RHS/SF/SG between leaf arrays and var-major host buffers (layouts as FVG_FIELD_*). */
+int fvh_forest_set_field(fvh_forest *fr, int which, const double *src);
+int fvh_forest_get_field(fvh_forest *fr, int which, double *dst);
+/* Adapt: flag[N] per leaf: +1 split, -1 merge request, 0 keep.  Splits copy the parent S
+ * (split.c:492-502); a parent whose 8 kids are all leaves flagged -1 is merged with the mean
+ * of the kids (merge.c:92-107).  2:1 face balance is restored by further splits.  Leaves
+ * deeper than max_level are not split.  Returns the new leaf count or an error. */
+int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level);
+
+/* ---------------------------------------------------------------- flattener */
+/* Flatten the leaf list of `run` (any RUN whose leaves carry the reference's neighbour and
+ * geometry fields).  The result owns its arrays; fvh_flat_mesh points into them. */
+int fvh_flatten_run(RUN *run, fvh_flat **out);
+const fvg_mesh *fvh_flat_mesh(const fvh_flat *fl);
+void fvh_flat_destroy(fvh_flat *fl);
+
+/* ---------------------------------------------------------------- device context */
+int fvg_create(int device, const fvg_params *p, fvg_ctx **out);
+int fvg_destroy(fvg_ctx *ctx);
+int fvg_set_params(fvg_ctx *ctx, const fvg_params *p);
+int fvg_set_mesh(fvg_ctx *ctx, const fvg_mesh *m);
+int fvg_ncell(const fvg_ctx *ctx);
+int fvg_set_state(fvg_ctx *ctx, const double *S);     /* host [neq][ncell] */
+int fvg_get_state(fvg_ctx *ctx, double *S);           /* host [neq][ncell] */
+int fvg_set_field(fvg_ctx *ctx, int which, const double *src);  /* S, SN, RHS, SF, SG from host f64 */
+int fvg_get_field(fvg_ctx *ctx, int which, double *dst);        /* to host f64 */
+/* Device pointer of a field (stream-ordered on fvg_stream).  SF/FACEFLUX are f32 when mixed. */
+int fvg_device_ptr(fvg_ctx *ctx, int which, void **ptr);
+void *fvg_stream(fvg_ctx *ctx);                       /* the context's hipStream_t */
+
+/* The reference's phases, each one stream-ordered kernel sequence. */
+int fvg_faceinterp(fvg_ctx *ctx);            /* SF from S (faceinterp.c:21-61) */
+int fvg_gradcalc(fvg_ctx *ctx);              /* SG[neq+1][3] from S (gradcalc.c:9-47) */
+int fvg_rhs(fvg_ctx *ctx);                   /* RHS from S / SF / SG (RHS.c:3-13) */
+int fvg_rk(fvg_ctx *ctx, int irk, double dt);/* stage formula irk = 0..4 (RK.c:16-51) */
+/* One time step: RK(0), then per stage [faceinterp] [gradcalc] RHS RK(k) with the RHS
+ * gather fused into the stage update.  method 1: Euler, 2: SSP-RK2, 3: SSP-RK3. */
+int fvg_step(fvg_ctx *ctx, int method, double dt);
+int fvg_steps(fvg_ctx *ctx, int nsteps, int method, double dt);
+/* dt = cfl / max_i sum_a (|u_a| + c_i) / h_a over the current S (extension; the reference
+ * has a fixed dt). */
+int fvg_cfl_dt(fvg_ctx *ctx, double cfl, double *dt);
+/* out[0..neq-1] = sum_i S_i Vol_i, out[neq] = sum_i Vol_i (conservationcalc.c:31-38),
+ * fixed-order device reduction. */
+int fvg_totals(fvg_ctx *ctx, double *out);
+/* Scan S and RHS for NaN/Inf or rho <= 0; *bad_cell = first offending cell or -1. */
+int fvg_check_state(fvg_ctx *ctx, int32_t *bad_cell);
+int fvg_sync(fvg_ctx *ctx);
+const char *fvg_last_error(const fvg_ctx *ctx);
+/* Average kernel durations (ms) of the last fvg_time_steps call, by kernel id
+ * (0 recon, 1 gradient, 2 face flux, 3 gather+RK, 4 RK(0) copy). */
+int fvg_time_steps(fvg_ctx *ctx, int nsteps, int method, double dt, double *ms_total,
+                   double *ms_kernel /* [8] */);
+
+/* ---------------------------------------------------------------- RUN binding */
+/* Flatten `run`, create a context on `device` and upload S; afterwards RHS(run), RK(irk,run),
+ * faceinterp(run) and gradcalc(run) run on the GPU.  compat != 0 copies every output back
+ * into the leaf arrays after each call (reference-shaped callers); compat == 0 keeps the
+ * state on the device until fv_gpu_download(run). */
+int fv_gpu_attach(RUN *run, int device, const fvg_params *p, int compat);
+int fv_gpu_detach(RUN *run);
+int fv_gpu_remesh(RUN *run);      /* after a mesh adaptation: re-flatten and upload S */
+int fv_gpu_upload(RUN *run);      /* leaves -> device S */
+int fv_gpu_download(RUN *run);    /* device S, SN, RHS -> leaves */
+fvg_ctx *fv_gpu_context(RUN *run);
+int fv_gpu_last_status(void);     /* status of the last RHS/RK/faceinterp/gradcalc call */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVGPU_H */

@@ -1,0 +1,141 @@
+"""ctypes wrapper of the CPU oracle (oracle/fv_oracle.c).
+
+Test infrastructure: imported only by tests/, the smoke check and the CPU
+baseline leg of bench.py.  The oracle works on the leaf arrays of a forest's RUN
+(the reference's structs), so inputs are set and outputs read through the
+forest (Forest.set_state / Forest.get_field).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", os.fspath(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(os.fspath(LIB_PATH))
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        for name in ("orc_faceinterp", "orc_gradcalc", "orc_rhs"):
+            getattr(L, name).argtypes = [vp, vp]
+            getattr(L, name).restype = C.c_int
+        L.orc_rk.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_rk.restype = C.c_int
+        L.orc_step.argtypes = [vp, vp, C.c_int, C.c_double]
+        L.orc_step.restype = C.c_int
+        L.orc_totals.argtypes = [vp, dp]
+        L.orc_totals.restype = C.c_int
+        L.orc_cfl_dt.argtypes = [vp, vp, C.c_double, dp]
+        L.orc_cfl_dt.restype = C.c_int
+        L.orc_face_flux.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_face_flux.restype = C.c_int
+        L.orc_eos.argtypes = [vp, dp, dp]
+        L.orc_eos.restype = C.c_int
+        L.orc_limited_slope.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double]
+        L.orc_limited_slope.restype = C.c_double
+        L.orc_last_bad_cell.argtypes = []
+        L.orc_last_bad_cell.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _ck(rc: int, what: str) -> None:
+    if rc != 0:
+        raise OracleError(f"{what} returned {rc} (cell {lib().orc_last_bad_cell()})")
+
+
+def _p(p):
+    return C.cast(C.byref(p), C.c_void_p)
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def faceinterp(forest, p) -> None:
+    _ck(lib().orc_faceinterp(forest.run, _p(p)), "orc_faceinterp")
+
+
+def gradcalc(forest, p) -> None:
+    _ck(lib().orc_gradcalc(forest.run, _p(p)), "orc_gradcalc")
+
+
+def rhs(forest, p) -> None:
+    _ck(lib().orc_rhs(forest.run, _p(p)), "orc_rhs")
+
+
+def rk(forest, irk: int, dt: float) -> None:
+    _ck(lib().orc_rk(forest.run, int(irk), float(dt)), "orc_rk")
+
+
+def step(forest, p, method: int, dt: float) -> None:
+    _ck(lib().orc_step(forest.run, _p(p), int(method), float(dt)), "orc_step")
+
+
+def totals(forest) -> np.ndarray:
+    out = np.zeros(forest.neq + 1)
+    _ck(lib().orc_totals(forest.run, _d(out)), "orc_totals")
+    return out
+
+
+def cfl_dt(forest, p, cfl: float) -> float:
+    d = C.c_double()
+    _ck(lib().orc_cfl_dt(forest.run, _p(p), float(cfl), C.byref(d)), "orc_cfl_dt")
+    return d.value
+
+
+def full_rhs(forest, p) -> np.ndarray:
+    """faceinterp (MUSCL) + gradcalc (viscous) + RHS of the forest's current S."""
+    from fv_140823_amd import FIELD_RHS
+    if p.ischeme == 2:
+        faceinterp(forest, p)
+    if p.ns:
+        gradcalc(forest, p)
+    rhs(forest, p)
+    return forest.get_field(FIELD_RHS)
+
+
+def face_flux(p, UL, UR, R9=(1, 0, 0, 0, 1, 0, 0, 0, 1)) -> np.ndarray:
+    """HLLC flux F.n + correction of two conserved states across a face (rows of R9 =
+    n, t1, t2)."""
+    n = 5 if p.model != 2 else 4
+    a = np.ascontiguousarray(UL, np.float64)
+    b = np.ascontiguousarray(UR, np.float64)
+    r = np.ascontiguousarray(R9, np.float64)
+    out = np.zeros(n)
+    _ck(lib().orc_face_flux(_p(p), _d(a), _d(b), _d(r), _d(out)), "orc_face_flux")
+    return out
+
+
+def eos(p, U) -> tuple[float, float]:
+    u = np.ascontiguousarray(U, np.float64)
+    pc = np.zeros(2)
+    _ck(lib().orc_eos(_p(p), _d(u), _d(pc)), "orc_eos")
+    return float(pc[0]), float(pc[1])
+
+
+def limited_slope(limiter: int, eps: float, boundary_face: bool, delta0: float, delta1: float,
+                  dp1cc: float, dm1cc: float) -> float:
+    return lib().orc_limited_slope(int(limiter), float(eps), int(bool(boundary_face)), float(delta0),
+                                   float(delta1), float(dp1cc), float(dm1cc))
