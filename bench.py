@@ -33,6 +33,8 @@ def _args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="blast", choices=["blast", "sod", "cavitation", "tgv", "vortex"])
     ap.add_argument("--evolve", type=int, default=120, help="untimed AMR steps before timing (blast)")
+    ap.add_argument("--level", type=int, default=None, help="uniform level (sod/cavitation/tgv) or base level (blast/vortex)")
+    ap.add_argument("--refine", type=float, default=0.02, help="blast: split where h|grad rho|/rho exceeds this")
     ap.add_argument("--acc-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
@@ -66,14 +68,16 @@ def alg_bytes(kernel, m, neq, rs, ns_visc, first_stage):
     raise KeyError(kernel)
 
 
-def build_workload(name, evolve, quiet):
+def build_workload(name, evolve, quiet, level=None, refine=0.02):
     import fv_140823_amd as fv
     from fv_140823_amd import configs, driver
     if name == "blast":
-        F, p, meta = configs.blast(5, 2)
+        F, p, meta = configs.blast(level or 5, 2)
         g = fv.Solver.from_forest(F, p)
         t0 = time.perf_counter()
-        flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, base_level=6)
+        base = (level or 5) + 1
+        flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, refine=refine,
+                                                                     coarsen=refine / 5, base_level=base)
         st = driver.evolve(F, g, evolve, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"],
                            log=lambda s: log(s, quiet))
         # final adaptation so the timed mesh matches the state
@@ -84,7 +88,7 @@ def build_workload(name, evolve, quiet):
         g.close()
         return F, p, meta
     builder = getattr(configs, name)
-    F, p, meta = builder()
+    F, p, meta = builder(level) if level else builder()
     return F, p, meta
 
 
@@ -175,7 +179,7 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     log(f"building workload {a.workload}", a.quiet)
-    F, p, meta = build_workload(a.workload, a.evolve, a.quiet)
+    F, p, meta = build_workload(a.workload, a.evolve, a.quiet, a.level, a.refine)
     mf = mesh_facts(F)
     log(f"mesh: {mf}", a.quiet)
     neq = fv.neq_of(p.model)

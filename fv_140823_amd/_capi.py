@@ -127,6 +127,17 @@ _DECLS = [
     ("fvg_sync", C.c_int, [_vp]),
     ("fvg_last_error", C.c_char_p, [_vp]),
     ("fvg_time_steps", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp]),
+    ("fvg_adj_set_probe", C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
+    ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),
+    ("fvg_tlm", C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp, _dp]),
+    ("fvg_face_flux_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp]),
+    ("fvg_face_vjp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    ("fvg_face_jvp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    ("fvg_limiter_host", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, _dp, _dp, _dp]),
+    ("fvg_set_state_device", C.c_int, [_vp, _vp]),
+    ("fvg_get_state_device", C.c_int, [_vp, _vp]),
     ("fv_gpu_attach", C.c_int, [_vp, C.c_int, C.POINTER(FvgParams), C.c_int]),
     ("fv_gpu_detach", C.c_int, [_vp]),
     ("fv_gpu_remesh", C.c_int, [_vp]),

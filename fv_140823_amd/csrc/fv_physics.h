@@ -19,7 +19,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
+
+#define FV_HD __host__ __device__
 
 struct DevParams {
   int model, ns, ischeme, limiter, visc_work, neq;
@@ -44,9 +47,12 @@ template <typename T> struct FState {
 
 /* EOS of a conserved vector, always in FP64 (returns false on a failed root) */
 template <int MODEL>
-__device__ __forceinline__ bool eos64(const DevParams &P, const double *U, bool clip,
+FV_HD __forceinline__ bool eos64(const DevParams &P, const double *U, bool clip,
                                       double &r, double u[3], double &e, double &p, double &c,
                                       double &y) {
+  /* no fused multiply-add here: the two-phase root -b + sqrt(b^2 - 4d) cancels
+   * about three digits, so every product is rounded as in the reference */
+#pragma clang fp contract(off)
   r = U[0];
   u[0] = U[1] / r;
   u[1] = U[2] / r;
@@ -98,7 +104,7 @@ __device__ __forceinline__ bool eos64(const DevParams &P, const double *U, bool 
 
 /* face state: EOS in FP64, then the primitive state in compute precision T */
 template <int MODEL, typename T>
-__device__ __forceinline__ bool make_state(const DevParams &P, const double *U, bool clip,
+FV_HD __forceinline__ bool make_state(const DevParams &P, const double *U, bool clip,
                                            FState<T> &s) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   double r, u[3], e, p, c, y;
@@ -119,7 +125,7 @@ __device__ __forceinline__ bool make_state(const DevParams &P, const double *U, 
 /* Davis-HLLC flux of (L, R) across a face of normal s*e_a, plus the viscous flux;
  * returns out[v] = (F_adv - F_visc).n + HLLC correction (before the -Area factor). */
 template <int MODEL, bool NS, int a, typename T>
-__device__ __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
+FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
                                           const FState<T> &R, T s, T *out) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const T zero = (T)0;
@@ -173,7 +179,10 @@ __device__ __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L
     tau[1][2] = tau[2][1] = mu * (gr[5] + gr[7]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) F[1 + i] = F[1 + i] - tau[a][i];
-    if constexpr (MODEL == 1) {
+    /* flux_viscous.c:227-232 writes row 4 whenever con->eos == 1, whatever the
+     * model: for model 3 that row is the rho Y flux, and the reference subtracts
+     * the viscous work from it too */
+    if constexpr (MODEL == 1 || MODEL == 3) {
       if (P.visc_work) F[4] = F[4] - (uf[0] * tau[a][0] + uf[1] * tau[a][1] + uf[2] * tau[a][2]);
     }
   }
@@ -196,4 +205,250 @@ __device__ __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) out[v] = F[v] * s + corr[v];
   return region != 0 && (SS == SS);
+}
+
+/* ------------------------------------------------------------------------
+ * Derivatives of the face flux (model 1, inviscid) for the discrete adjoint
+ * and the tangent-linear model.
+ * ---------------------------------------------------------------------- */
+
+/* forward-mode number with one tangent lane (tangent-linear model) */
+template <typename T> struct Dual {
+  T v, d;
+  FV_HD Dual() : v(0), d(0) {}
+  FV_HD Dual(double x) : v((T)x), d(0) {}
+  FV_HD Dual(T x, T dx) : v(x), d(dx) {}
+};
+template <typename T> FV_HD inline Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <typename T> FV_HD inline Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <typename T> FV_HD inline Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <typename T> FV_HD inline Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <typename T> FV_HD inline Dual<T> operator/(Dual<T> a, Dual<T> b) {
+  T q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+template <typename T> FV_HD inline bool operator<(Dual<T> a, Dual<T> b) { return a.v < b.v; }
+template <typename T> FV_HD inline bool operator>(Dual<T> a, Dual<T> b) { return a.v > b.v; }
+template <typename T> FV_HD inline bool operator<=(Dual<T> a, Dual<T> b) { return a.v <= b.v; }
+template <typename T> FV_HD inline bool operator==(Dual<T> a, Dual<T> b) { return a.v == b.v; }
+template <typename T> FV_HD inline Dual<T> sqrt(Dual<T> a) {
+  T r = sqrt(a.v);
+  return {r, a.d / (T(2) * r)};
+}
+
+/* ideal/stiffened-gas face state in any arithmetic T (facercnstr.c:20-35) */
+template <typename T>
+FV_HD __forceinline__ void make_state_ideal(const DevParams &P, const T *U, FState<T> &s) {
+  const T g = (T)P.gamma, pinf = (T)P.pinf;
+  s.r = U[0];
+  s.u[0] = U[1] / s.r;
+  s.u[1] = U[2] / s.r;
+  s.u[2] = U[3] / s.r;
+  s.e = U[4];
+  T ei = (s.e / s.r) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+  s.p = s.r * (g - (T)1.0) * ei - g * pinf;
+  s.c = sqrt(g * (s.p + pinf) / s.r);
+  s.y = (T)0.0;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) s.q[v] = U[v];
+}
+
+/* bars of the primitive state (r, u, e, p, c) -> bars of the conserved vector */
+template <typename T>
+FV_HD __forceinline__ void state_ideal_vjp(const DevParams &P, const FState<T> &s, T rb, T ub[3],
+                                           T eb, T pb, T cb, T *Ub) {
+  const T g = (T)P.gamma, pinf = (T)P.pinf;
+  /* c = sqrt(q), q = g (p + pinf) / r */
+  const T q = g * (s.p + pinf) / s.r;
+  const T qb = cb / ((T)2 * s.c);
+  pb += qb * g / s.r;
+  rb -= qb * q / s.r;
+  /* p = r (g - 1) ei - g pinf, ei = e / r - |u|^2 / 2 */
+  const T ei = (s.e / s.r) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+  rb += pb * (g - (T)1.0) * ei;
+  const T eib = pb * s.r * (g - (T)1.0);
+  eb += eib / s.r;
+  rb -= eib * s.e / (s.r * s.r);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ub[i] -= eib * s.u[i];
+    /* u_i = U_{1+i} / r */
+    Ub[1 + i] += ub[i] / s.r;
+    rb -= ub[i] * s.u[i] / s.r;
+  }
+  Ub[0] += rb;
+  Ub[4] += eb;
+}
+
+/* Hand-written reverse of make_state_ideal + hllc_face<1, false, a> (the same
+ * branch choices as the forward): given ob = d(objective)/d(out), accumulate
+ * ULb += ob . d out / d UL and URb += ob . d out / d UR. */
+template <int a, typename T>
+FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const T *UR, T s,
+                                          const T *ob, T *ULb, T *URb) {
+  FState<T> L, R;
+  make_state_ideal<T>(P, UL, L);
+  make_state_ideal<T>(P, UR, R);
+  const T zero = (T)0;
+  /* forward (hllc_face) */
+  const T uL = s * L.u[a], uR = s * R.u[a];
+  const T ruL = L.r * uL, ruR = R.r * uR;
+  const bool rmaxL = (uL + L.c) > (uR + R.c);
+  const T SR = rmaxL ? (uL + L.c) : (uR + R.c);
+  const bool lminL = (uL - L.c) < (uR - R.c);
+  const T SL = lminL ? (uL - L.c) : (uR - R.c);
+  const T num = (L.r * (uL * uL) + L.p) - (R.r * (uR * uR) + R.p) - (SL * ruL) + (SR * ruR);
+  const T den = (ruL - ruR - (SL * L.r) + (SR * R.r));
+  const T SS = num / den;
+  int region = 0;
+  if (zero <= SL) region = 1;
+  else if (SL <= zero && zero <= SS) region = 2;
+  else if (SS <= zero && zero <= SR) region = 3;
+  else if (SR <= zero) region = 4;
+  const bool useL = region <= 2;
+  const T Kr = useL ? L.r : R.r, Ke = useL ? L.e : R.e, Kp = useL ? L.p : R.p;
+  T Ku[3], Kq[5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Ku[i] = useL ? L.u[i] : R.u[i];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Kq[v] = useL ? L.q[v] : R.q[v];
+  const T SK = (region == 2) ? SL : SR;
+  const T uK = useL ? uL : uR;
+  /* reverse */
+  T Krb = 0, Keb = 0, Kpb = 0, Kub[3] = {0, 0, 0}, Kqb[5] = {0, 0, 0, 0, 0};
+  T uKb = 0, SKb = 0, SSb = 0;
+  T Fb[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Fb[v] = ob[v] * s;
+  if (region == 2 || region == 3) {
+    const T W = Kr * (SK - uK);
+    const T B = SK - SS;
+    const T cs = W / B;
+    const T Y = SS + (Kp / W);
+    const T X = (Ke / Kr) + (SS - uK) * Y;
+    T Us[5];
+    Us[0] = cs;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Us[1 + i] = cs * (i == a ? s * SS : Ku[i]);
+    Us[4] = cs * X;
+    T Usb[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      SKb += ob[v] * (Us[v] - Kq[v]);
+      Usb[v] = ob[v] * SK;
+      Kqb[v] -= ob[v] * SK;
+    }
+    T csb = Usb[0] + Usb[4] * X;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      csb += Usb[1 + i] * (i == a ? s * SS : Ku[i]);
+      if (i == a) SSb += Usb[1 + i] * cs * s;
+      else Kub[i] += Usb[1 + i] * cs;
+    }
+    const T Xb = Usb[4] * cs;
+    Keb += Xb / Kr;
+    Krb -= Xb * Ke / (Kr * Kr);
+    SSb += Xb * Y;
+    uKb -= Xb * Y;
+    const T Yb = Xb * (SS - uK);
+    SSb += Yb;
+    Kpb += Yb / W;
+    T Wb = -Yb * Kp / (W * W);
+    Wb += csb / B;
+    const T Bb = -csb * cs / B;
+    Krb += Wb * (SK - uK);
+    SKb += Wb * Kr;
+    uKb -= Wb * Kr;
+    SKb += Bb;
+    SSb -= Bb;
+  }
+  /* physical flux of K, component a */
+  Kub[a] += Fb[0] * Kr;
+  Krb += Fb[0] * Ku[a];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Krb += Fb[1 + i] * Ku[i] * Ku[a];
+    Kub[i] += Fb[1 + i] * Kr * Ku[a];
+    Kub[a] += Fb[1 + i] * Kr * Ku[i];
+  }
+  Kpb += Fb[1 + a];
+  Kub[a] += Fb[4] * (Ke + Kp);
+  Keb += Fb[4] * Ku[a];
+  Kpb += Fb[4] * Ku[a];
+  /* K -> L or R */
+  T Lrb = 0, Leb = 0, Lpb = 0, Lcb = 0, Lub[3] = {0, 0, 0};
+  T Rrb = 0, Reb = 0, Rpb = 0, Rcb = 0, Rub[3] = {0, 0, 0};
+  T uLb = 0, uRb = 0, SLb = 0, SRb = 0;
+  if (useL) {
+    Lrb += Krb; Leb += Keb; Lpb += Kpb; uLb += uKb;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Lub[i] += Kub[i];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) ULb[v] += Kqb[v];
+  } else {
+    Rrb += Krb; Reb += Keb; Rpb += Kpb; uRb += uKb;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Rub[i] += Kub[i];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) URb[v] += Kqb[v];
+  }
+  if (region == 2) SLb += SKb;
+  else if (region == 3) SRb += SKb;
+  /* S* = num / den */
+  const T numb = SSb / den;
+  const T denb = -SSb * SS / den;
+  Lrb += numb * (uL * uL);
+  uLb += numb * (T)2 * L.r * uL;
+  Lpb += numb;
+  Rrb -= numb * (uR * uR);
+  uRb -= numb * (T)2 * R.r * uR;
+  Rpb -= numb;
+  SLb -= numb * ruL;
+  T ruLb = -numb * SL;
+  SRb += numb * ruR;
+  T ruRb = numb * SR;
+  ruLb += denb;
+  ruRb -= denb;
+  SLb -= denb * L.r;
+  Lrb -= denb * SL;
+  SRb += denb * R.r;
+  Rrb += denb * SR;
+  Lrb += ruLb * uL;
+  uLb += ruLb * L.r;
+  Rrb += ruRb * uR;
+  uRb += ruRb * R.r;
+  if (rmaxL) { uLb += SRb; Lcb += SRb; } else { uRb += SRb; Rcb += SRb; }
+  if (lminL) { uLb += SLb; Lcb -= SLb; } else { uRb += SLb; Rcb -= SLb; }
+  Lub[a] += s * uLb;
+  Rub[a] += s * uRb;
+  state_ideal_vjp<T>(P, L, Lrb, Lub, Leb, Lpb, Lcb, ULb);
+  state_ideal_vjp<T>(P, R, Rrb, Rub, Reb, Rpb, Rcb, URb);
+}
+
+/* limited slope of reconctruct_V3.c with its partials w.r.t. delta0, delta1 */
+FV_HD __forceinline__ double limited_d(int lim, double eps, bool bnd, double d0, double d1,
+                                       double dp1cc, double dm1cc, double &g0, double &g1) {
+  g0 = 0.0;
+  g1 = 0.0;
+  if (bnd) return 0.0;
+  if (lim == 0) {
+    if (fabs(d0) > 1e-10) {
+      double a = -d1 / dm1cc, b = d0 / dp1cc;
+      if (a * b > 0.0) {
+        if (fabs(a) > fabs(b)) { g0 = 1.0 / dp1cc; return b; }
+        g1 = -1.0 / dm1cc;
+        return a;
+      }
+    }
+    return 0.0;
+  }
+  const double a = dm1cc != 0.0 ? -d1 / dm1cc : 0.0;
+  const double b = dp1cc != 0.0 ? d0 / dp1cc : 0.0;
+  const double N = a * (b * b + eps) + b * (a * a + eps);
+  const double D = a * a + b * b + 2.0 * eps;
+  const double ga = ((b * b + eps) + 2.0 * a * b) / D - N * 2.0 * a / (D * D);
+  const double gb = ((a * a + eps) + 2.0 * a * b) / D - N * 2.0 * b / (D * D);
+  if (dp1cc != 0.0) g0 = gb / dp1cc;
+  if (dm1cc != 0.0) g1 = -ga / dm1cc;
+  return N / D;
 }

@@ -1,0 +1,747 @@
+/*
+ * fvadj.hip — discrete adjoint and tangent-linear model of RHS + RK rollouts.
+ *
+ * Objective of a rollout of nsteps SSP-RK3 steps at a fixed dt (RK.c:24-47):
+ *     J = sum_{n=1..nsteps} dt * sum_{i in probe} p(S^n_i) Vol_i
+ * Adjoint (reverse, hand written): for each stage
+ *     S_k = a SN + b S_{k-1} + c dt/Vol R(S_{k-1})
+ * lambda_SN += a lambda, lambda_{k-1} = b lambda + R'(S_{k-1})^T (c dt/Vol lambda),
+ * with R'^T = recon^T . face^T . gather^T, each a deterministic gather:
+ *   k_adj_face   per face: mu_L - mu_R -> hand-written VJP of HLLC + EOS
+ *   k_adj_recon  per cell: face-state bars by gather slot (f = 0..5, k = 0..3),
+ *                limiter partials -> own bar and the bars sent to the +/- neighbours
+ *   k_adj_recv   per cell: own bar + what its neighbours sent (no atomics)
+ * Stage states are recomputed from checkpoints every `ckpt` steps.
+ * Tangent (forward, for the dot-product check): the same stage kernels on
+ * forward-mode dual numbers for the face flux and analytic limiter partials.
+ * Scope: model 1 (ideal/stiffened gas), inviscid, MUSCL (either limiter) or
+ * first order; bc outflow/periodic (an inlet ghost is a constant).
+ */
+#include "fv_common.h"
+
+struct fv_adj_state {
+  int nsteps = 0, ckpt = 0, nck = 0;
+  double dt = 0.0;
+  bool fwd_done = false;
+  size_t plane = 0; /* neq * ns doubles */
+  uint8_t *mask = nullptr;
+  double *ckpts = nullptr, *seg = nullptr, *Sfinal = nullptr;
+  double *lam = nullptr, *lamSN = nullptr, *mu = nullptr, *nu = nullptr, *self = nullptr;
+  double *Pp = nullptr, *Pm = nullptr;
+  void *lamUL = nullptr, *lamUR = nullptr;
+  double *Jsteps = nullptr, *Jdsteps = nullptr, *part = nullptr;
+  int part_cap = 0;
+  double *Sd = nullptr, *SNd = nullptr, *SFd = nullptr, *Gd = nullptr;
+};
+
+void fv_adj_release(fvg_ctx *c) {
+  if (!c || !c->adj) return;
+  fv_adj_state *a = c->adj;
+  void *ptrs[] = {a->mask, a->ckpts, a->seg, a->Sfinal, a->lam, a->lamSN, a->mu, a->nu, a->self,
+                  a->Pp, a->Pm, a->lamUL, a->lamUR, a->Jsteps, a->Jdsteps, a->part, a->Sd, a->SNd,
+                  a->SFd, a->Gd};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete a;
+  c->adj = nullptr;
+}
+
+/* ------------------------------------------------------------------ kernels */
+
+__device__ __forceinline__ double cell_vol(const DevMesh &M, int i) {
+  return M.vol[(M.cflag[i] >> 16) & 0xff];
+}
+
+/* face^T: lamUL/lamUR[v][j] = d(sum_i mu_i . R_i)/d(face states of face j) */
+template <int a, typename T, typename R>
+__device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams &P, int j,
+                                              const double *__restrict__ S,
+                                              const R *__restrict__ SF,
+                                              const double *__restrict__ mu, R *__restrict__ lUL,
+                                              R *__restrict__ lUR) {
+  const int iL = M.fL[j], iR = M.fR[j];
+  const uint32_t meta = M.fmeta[j];
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const T s = (T)c_face_sign[fl];
+  const bool first = P.ischeme == 1;
+  T UL[5], UR[5], ob[5], ULb[5], URb[5];
+  const T A = (T)M.area[3 * lev + a];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    UL[v] = first ? (T)S[(size_t)v * M.ns + iL] : (T)SF[((size_t)fl * 5 + v) * M.ns + iL];
+    if (bc == 0)
+      UR[v] = first ? (T)S[(size_t)v * M.ns + iR] : (T)SF[((size_t)fr * 5 + v) * M.ns + iR];
+    else if (bc == 1)
+      UR[v] = (T)P.inlet[v];
+    else
+      UR[v] = UL[v];
+    const double mg = mu[(size_t)v * M.ns + iL] - (bc == 0 ? mu[(size_t)v * M.ns + iR] : 0.0);
+    ob[v] = -A * (T)mg;
+    ULb[v] = URb[v] = (T)0;
+  }
+  hllc_vjp_ideal<a, T>(P, UL, UR, s, ob, ULb, URb);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    if (bc == 2) { ULb[v] += URb[v]; URb[v] = (T)0; }
+    if (bc == 1) URb[v] = (T)0;
+    lUL[(size_t)v * M.nfp + j] = (R)ULb[v];
+    lUR[(size_t)v * M.nfp + j] = (R)URb[v];
+  }
+}
+
+template <typename T, typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_face(DevMesh M, DevParams P, int b0, int b1,
+                                                       int n0, int n1, int n2,
+                                                       const double *__restrict__ S,
+                                                       const R *__restrict__ SF,
+                                                       const double *__restrict__ mu,
+                                                       R *__restrict__ lUL, R *__restrict__ lUR) {
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  if (lb < b0) {
+    const int j = lb * FV_BLOCK + t;
+    if (j < n0) adj_face_body<0, T, R>(M, P, j, S, SF, mu, lUL, lUR);
+  } else if (lb < b0 + b1) {
+    const int q = (lb - b0) * FV_BLOCK + t;
+    if (q < n1) adj_face_body<1, T, R>(M, P, n0 + q, S, SF, mu, lUL, lUR);
+  } else {
+    const int q = (lb - b0 - b1) * FV_BLOCK + t;
+    if (q < n2) adj_face_body<2, T, R>(M, P, n0 + n1 + q, S, SF, mu, lUL, lUR);
+  }
+}
+
+/* bar of the face state SF[f] of cell i, gathered from the face(s) of slot f */
+template <typename R>
+__device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ lUL,
+                                       const R *__restrict__ lUR, int i, uint32_t cf, int f,
+                                       double *out) {
+  const int g = M.gslot[(size_t)f * M.ns + i];
+  const int cls = (cf >> (2 * f)) & 3;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) out[v] = 0.0;
+  if (g < 0) return;
+  const int jf = g >> 1;
+  if (cls == CLS_FINER) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] += (double)lUL[(size_t)v * M.nfp + jf + k];
+  } else if (g & 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = (double)lUR[(size_t)v * M.nfp + jf];
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = (double)lUL[(size_t)v * M.nfp + jf];
+  }
+}
+
+/* recon^T, part 1: own bar and the bars sent to the +/- neighbour of each axis */
+template <typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
+                                                        const double *__restrict__ S,
+                                                        const R *__restrict__ lUL,
+                                                        const R *__restrict__ lUR,
+                                                        double *__restrict__ self,
+                                                        double *__restrict__ Pp,
+                                                        double *__restrict__ Pm) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  const size_t pl = (size_t)5 * M.ns;
+  double acc[5] = {0, 0, 0, 0, 0};
+  if (P.ischeme == 1) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      double b[5];
+      sf_bar<R>(M, lUL, lUR, i, cf, f, b);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += b[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      self[(size_t)v * M.ns + i] = acc[v];
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        Pp[ax * pl + (size_t)v * M.ns + i] = 0.0;
+        Pm[ax * pl + (size_t)v * M.ns + i] = 0.0;
+      }
+    }
+    return;
+  }
+  double U[5];
+  load_S<5>(S, M.ns, i, U);
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
+    const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
+    double Up[5], Um[5], lp[5], lm[5];
+    nb_state<5>(M, P, S, i, cf, fp, U, Up);
+    nb_state<5>(M, P, S, i, cf, fm, U, Um);
+    sf_bar<R>(M, lUL, lUR, i, cf, fp, lp);
+    sf_bar<R>(M, lUL, lUR, i, cf, fm, lm);
+    double pcf, pcc, mcf, mcc;
+    face_dist(M, cf, fp, pcf, pcc);
+    face_dist(M, cf, fm, mcf, mcc);
+    const int cp = (cf >> (2 * fp)) & 3, cm = (cf >> (2 * fm)) & 3;
+    const bool bp = cp == CLS_BOUNDARY, bm = cm == CLS_BOUNDARY;
+    const bool ghost_p = bp && (-1 - M.nbr[(size_t)fp * M.ns + i]) == 2;
+    const bool ghost_m = bm && (-1 - M.nbr[(size_t)fm * M.ns + i]) == 2;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double dP = Up[v] - U[v], dM = Um[v] - U[v];
+      double g0, g1, h0, h1;
+      limited_d(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc, g0, g1);
+      limited_d(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc, h0, h1);
+      /* SF[fp] = U + pcf g(dP, dM), SF[fm] = U + mcf g(dM, dP) */
+      double dPb = lp[v] * pcf * g0 + lm[v] * mcf * h1;
+      double dMb = lp[v] * pcf * g1 + lm[v] * mcf * h0;
+      /* a transmissive ghost equals the cell itself: its delta is identically 0 */
+      if (ghost_p) dPb = 0.0;
+      if (ghost_m) dMb = 0.0;
+      acc[v] += lp[v] + lm[v] - dPb - dMb;
+      Pp[ax * pl + (size_t)v * M.ns + i] = bp ? 0.0 : dPb;
+      Pm[ax * pl + (size_t)v * M.ns + i] = bm ? 0.0 : dMb;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
+}
+
+/* recon^T, part 2: nu_i = own bar + bars sent to i by its face neighbours.
+ * A neighbour whose delta used the mean of 4 finer cells sends 1/4 to each. */
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv(DevMesh M, const double *__restrict__ self,
+                                                       const double *__restrict__ Pp,
+                                                       const double *__restrict__ Pm,
+                                                       double *__restrict__ nu) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  const size_t pl = (size_t)5 * M.ns;
+  double acc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) acc[v] = self[(size_t)v * M.ns + i];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
+    const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      /* my - side neighbours sent Pp (their + side is me); my + side ones sent Pm */
+      const int f = side == 0 ? fm : fp;
+      const double *Pin = (side == 0 ? Pp : Pm) + ax * pl;
+      const int cls = (cf >> (2 * f)) & 3;
+      const int nb = M.nbr[(size_t)f * M.ns + i];
+      if (cls == CLS_SAME) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + nb];
+      } else if (cls == CLS_COARSER) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + nb] / ((double)4);
+      } else if (cls == CLS_FINER) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int q = M.quad[4 * nb + k];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + q];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) nu[(size_t)v * M.ns + i] = acc[v];
+}
+
+/* mu = c dt / Vol * lam */
+__global__ void k_adj_mu(DevMesh M, const double *__restrict__ lam, double cdt,
+                         double *__restrict__ mu) {
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const double V = cell_vol(M, i);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) mu[(size_t)v * M.ns + i] = cdt * lam[(size_t)v * M.ns + i] / V;
+}
+
+/* lamSN (=|+=) a lam;  lam = b lam + nu  (or, on the first stage, lam = lamSN + nu) */
+__global__ void k_adj_stage(DevMesh M, double *__restrict__ lam, double *__restrict__ lamSN,
+                            const double *__restrict__ nu, double a, double b, int mode) {
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const size_t o = (size_t)v * M.ns + i;
+    const double l = lam[o];
+    if (mode == 0) { lamSN[o] = a * l; lam[o] = b * l + nu[o]; }
+    else if (mode == 1) { lamSN[o] += a * l; lam[o] = b * l + nu[o]; }
+    else { lam[o] = lamSN[o] + a * l + nu[o]; }
+  }
+}
+
+/* objective: part[b] = sum over probe cells of block b of p Vol (fixed order) */
+__global__ __launch_bounds__(FV_BLOCK) void k_obj(DevMesh M, DevParams P,
+                                                  const double *__restrict__ S,
+                                                  const double *__restrict__ Sd,
+                                                  const uint8_t *__restrict__ mask,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[2][FV_BLOCK];
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  double pv = 0.0, pd = 0.0;
+  if (i < M.N && mask[i]) {
+    double U[5];
+    load_S<5>(S, M.ns, i, U);
+    const double V = cell_vol(M, i);
+    const double g1 = P.gamma - 1.0;
+    const double r = U[0], u = U[1] / r, v = U[2] / r, w = U[3] / r;
+    const double ei = (U[4] / r) - 0.5 * (u * u + v * v + w * w);
+    pv = (r * g1 * ei - P.gamma * P.pinf) * V;
+    if (Sd) {
+      double D[5];
+      load_S<5>(Sd, M.ns, i, D);
+      pd = V * g1 * (0.5 * (u * u + v * v + w * w) * D[0] - u * D[1] - v * D[2] - w * D[3] + D[4]);
+    }
+  }
+  sh[0][threadIdx.x] = pv;
+  sh[1][threadIdx.x] = pd;
+  __syncthreads();
+  for (int w = FV_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0];
+    part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+__global__ void k_obj_final(const double *__restrict__ part, int nb, double *__restrict__ J,
+                            double *__restrict__ Jd) {
+  if (threadIdx.x || blockIdx.x) return;
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < nb; ++k) { a += part[2 * k]; b += part[2 * k + 1]; }
+  *J = a;
+  if (Jd) *Jd = b;
+}
+
+/* lam += dt Vol dp/dS on the probe */
+__global__ void k_obj_grad(DevMesh M, DevParams P, const double *__restrict__ S,
+                           const uint8_t *__restrict__ mask, double dt, double *__restrict__ lam) {
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  if (i >= M.N || !mask[i]) return;
+  double U[5];
+  load_S<5>(S, M.ns, i, U);
+  const double c = dt * cell_vol(M, i) * (P.gamma - 1.0);
+  const double r = U[0], u = U[1] / r, v = U[2] / r, w = U[3] / r;
+  lam[i] += c * 0.5 * (u * u + v * v + w * w);
+  lam[(size_t)1 * M.ns + i] -= c * u;
+  lam[(size_t)2 * M.ns + i] -= c * v;
+  lam[(size_t)3 * M.ns + i] -= c * w;
+  lam[(size_t)4 * M.ns + i] += c;
+}
+
+/* tangent of the reconstruction: SFd = Sd + dcf (g0 dPd + g1 dMd) */
+__global__ __launch_bounds__(FV_BLOCK) void k_tlm_recon(DevMesh M, DevParams P,
+                                                        const double *__restrict__ S,
+                                                        const double *__restrict__ Sd,
+                                                        double *__restrict__ SFd) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  double U[5], D[5];
+  load_S<5>(S, M.ns, i, U);
+  load_S<5>(Sd, M.ns, i, D);
+  DevParams Pz = P;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Pz.inlet[v] = 0.0; /* the inlet ghost is a constant */
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
+    const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
+    double Up[5], Um[5], Dp[5], Dm[5];
+    nb_state<5>(M, P, S, i, cf, fp, U, Up);
+    nb_state<5>(M, P, S, i, cf, fm, U, Um);
+    nb_state<5>(M, Pz, Sd, i, cf, fp, D, Dp);
+    nb_state<5>(M, Pz, Sd, i, cf, fm, D, Dm);
+    double pcf, pcc, mcf, mcc;
+    face_dist(M, cf, fp, pcf, pcc);
+    face_dist(M, cf, fm, mcf, mcc);
+    const bool bp = ((cf >> (2 * fp)) & 3) == CLS_BOUNDARY;
+    const bool bm = ((cf >> (2 * fm)) & 3) == CLS_BOUNDARY;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double dP = Up[v] - U[v], dM = Um[v] - U[v];
+      const double dPd = Dp[v] - D[v], dMd = Dm[v] - D[v];
+      double g0, g1, h0, h1;
+      limited_d(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc, g0, g1);
+      limited_d(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc, h0, h1);
+      SFd[((size_t)fp * 5 + v) * M.ns + i] = D[v] + pcf * (g0 * dPd + g1 * dMd);
+      SFd[((size_t)fm * 5 + v) * M.ns + i] = D[v] + mcf * (h0 * dMd + h1 * dPd);
+    }
+  }
+}
+
+/* tangent of the face flux by forward-mode dual numbers */
+template <int a>
+__device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams &P, int j,
+                                              const double *__restrict__ S,
+                                              const double *__restrict__ Sd, const double *SF,
+                                              const double *SFd, double *__restrict__ Gd) {
+  typedef Dual<double> D;
+  const int iL = M.fL[j], iR = M.fR[j];
+  const uint32_t meta = M.fmeta[j];
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const bool first = P.ischeme == 1;
+  const double *X = first ? S : SF;
+  const double *Xd = first ? Sd : SFd;
+  const size_t oL = first ? (size_t)iL : (size_t)fl * 5 * M.ns + iL;
+  const size_t oR = first ? (size_t)iR : (size_t)fr * 5 * M.ns + iR;
+  D UL[5], UR[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    UL[v] = D(X[oL + (size_t)v * M.ns], Xd[oL + (size_t)v * M.ns]);
+    if (bc == 0) UR[v] = D(X[oR + (size_t)v * M.ns], Xd[oR + (size_t)v * M.ns]);
+    else if (bc == 1) UR[v] = D(P.inlet[v], 0.0);
+    else UR[v] = UL[v];
+  }
+  FState<D> L, R;
+  make_state_ideal<D>(P, UL, L);
+  make_state_ideal<D>(P, UR, R);
+  D out[5];
+  hllc_face<1, false, a, D>(P, L, R, D((double)c_face_sign[fl]), out);
+  const double A = M.area[3 * lev + a];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Gd[(size_t)v * M.nfp + j] = -(A)*out[v].d;
+}
+
+__global__ __launch_bounds__(FV_BLOCK) void k_tlm_face(DevMesh M, DevParams P, int b0, int b1,
+                                                       int n0, int n1, int n2,
+                                                       const double *__restrict__ S,
+                                                       const double *__restrict__ Sd,
+                                                       const double *__restrict__ SF,
+                                                       const double *__restrict__ SFd,
+                                                       double *__restrict__ Gd) {
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  if (lb < b0) {
+    const int j = lb * FV_BLOCK + t;
+    if (j < n0) tlm_face_body<0>(M, P, j, S, Sd, SF, SFd, Gd);
+  } else if (lb < b0 + b1) {
+    const int q = (lb - b0) * FV_BLOCK + t;
+    if (q < n1) tlm_face_body<1>(M, P, n0 + q, S, Sd, SF, SFd, Gd);
+  } else {
+    const int q = (lb - b0 - b1) * FV_BLOCK + t;
+    if (q < n2) tlm_face_body<2>(M, P, n0 + n1 + q, S, Sd, SF, SFd, Gd);
+  }
+}
+
+/* ------------------------------------------------------------------ host side */
+
+static inline int nblk(int n) { return (n + FV_BLOCK - 1) / FV_BLOCK; }
+
+static int adj_check(fvg_ctx *c) {
+  if (!c) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  if (c->prm.model != 1 || c->prm.ns)
+    return set_err(c, FVG_ERR_UNSUPPORTED, "adjoint/tangent: model 1, inviscid only");
+  return FVG_OK;
+}
+
+static int adj_alloc(fvg_ctx *c) {
+  if (c->adj) return FVG_OK;
+  fv_adj_state *a = new fv_adj_state();
+  c->adj = a;
+  a->plane = (size_t)5 * c->ns;
+  const size_t pl = a->plane * 8;
+  HIPCHK(c, hipMalloc(&a->mask, (size_t)c->ns));
+  HIPCHK(c, hipMemsetAsync(a->mask, 0, (size_t)c->ns, c->stream));
+  HIPCHK(c, hipMalloc(&a->Sfinal, pl));
+  HIPCHK(c, hipMalloc(&a->lam, pl));
+  HIPCHK(c, hipMalloc(&a->lamSN, pl));
+  HIPCHK(c, hipMalloc(&a->mu, pl));
+  HIPCHK(c, hipMalloc(&a->nu, pl));
+  HIPCHK(c, hipMalloc(&a->self, pl));
+  HIPCHK(c, hipMalloc(&a->Pp, 3 * pl));
+  HIPCHK(c, hipMalloc(&a->Pm, 3 * pl));
+  HIPCHK(c, hipMalloc(&a->lamUL, (size_t)5 * c->nfp * c->rsize));
+  HIPCHK(c, hipMalloc(&a->lamUR, (size_t)5 * c->nfp * c->rsize));
+  a->part_cap = nblk(c->N);
+  HIPCHK(c, hipMalloc(&a->part, (size_t)2 * a->part_cap * 8));
+  return FVG_OK;
+}
+
+/* J and (optionally) dJ contributions of the current S / Sd into slot n */
+static void obj_step(fvg_ctx *c, const double *Sd, int n) {
+  fv_adj_state *a = c->adj;
+  DevMesh M = dev_mesh(c);
+  const int nb = nblk(c->N);
+  hipLaunchKernelGGL(k_obj, dim3(nb), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, Sd, a->mask,
+                     a->part);
+  hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(64), 0, c->stream, a->part, nb, a->Jsteps + n,
+                     Sd ? a->Jdsteps + n : (double *)nullptr);
+}
+
+/* nu = R'(S)^T mu for the current S (reconstructs SF of S first) */
+static void residual_vjp(fvg_ctx *c) {
+  fv_adj_state *a = c->adj;
+  DevMesh M = dev_mesh(c);
+  if (c->prm.ischeme == 2) fv_launch_recon(c);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblk(n0), b1 = nblk(n1), b2 = nblk(n2);
+  if (c->rsize == 8)
+    hipLaunchKernelGGL((k_adj_face<double, double>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
+                       M, c->dp, b0, b1, n0, n1, n2, c->S, (const double *)c->SF, a->mu,
+                       (double *)a->lamUL, (double *)a->lamUR);
+  else
+    hipLaunchKernelGGL((k_adj_face<float, float>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
+                       M, c->dp, b0, b1, n0, n1, n2, c->S, (const float *)c->SF, a->mu,
+                       (float *)a->lamUL, (float *)a->lamUR);
+  const int nc = nblk(c->N);
+  if (c->rsize == 8)
+    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S,
+                       (const double *)a->lamUL, (const double *)a->lamUR, a->self, a->Pp, a->Pm);
+  else
+    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S,
+                       (const float *)a->lamUL, (const float *)a->lamUR, a->self, a->Pp, a->Pm);
+  hipLaunchKernelGGL(k_adj_recv, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->self, a->Pp, a->Pm,
+                     a->nu);
+}
+
+extern "C" {
+
+int fvg_adj_set_probe(fvg_ctx *c, const uint8_t *mask) {
+  int rc = adj_check(c);
+  if (rc) return rc;
+  if (!mask) return set_err(c, FVG_ERR_ARG, "null mask");
+  rc = adj_alloc(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->adj->mask, mask, (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
+}
+
+int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
+  int rc = adj_check(c);
+  if (rc) return rc;
+  if (nsteps < 1 || ckpt < 1 || !J) return set_err(c, FVG_ERR_ARG, "bad nsteps/ckpt");
+  (void)hipSetDevice(c->device);
+  rc = adj_alloc(c);
+  if (rc) return rc;
+  fv_adj_state *a = c->adj;
+  const int nck = (nsteps + ckpt - 1) / ckpt;
+  const size_t pl = a->plane * 8;
+  if (a->ckpts) (void)hipFree(a->ckpts);
+  if (a->seg) (void)hipFree(a->seg);
+  if (a->Jsteps) (void)hipFree(a->Jsteps);
+  a->ckpts = a->seg = a->Jsteps = nullptr;
+  HIPCHK(c, hipMalloc(&a->ckpts, (size_t)nck * pl));
+  HIPCHK(c, hipMalloc(&a->seg, (size_t)(3 * ckpt + 1) * pl));
+  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)nsteps * 8));
+  a->nsteps = nsteps;
+  a->ckpt = ckpt;
+  a->nck = nck;
+  a->dt = dt;
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % ckpt == 0)
+      HIPCHK(c, hipMemcpyAsync(a->ckpts + (size_t)(n / ckpt) * a->plane, c->S, pl,
+                               hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 1; k <= 3; ++k) fv_stage(c, k, dt, k == 1, false);
+    obj_step(c, nullptr, n);
+  }
+  HIPCHK(c, hipMemcpyAsync(a->Sfinal, c->S, pl, hipMemcpyDeviceToDevice, c->stream));
+  std::vector<double> js(nsteps);
+  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  for (int n = 0; n < nsteps; ++n) sum += dt * js[n];
+  *J = sum;
+  a->fwd_done = true;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  return FVG_OK;
+}
+
+int fvg_adj_backward(fvg_ctx *c, double *grad_host, void *grad_dev) {
+  int rc = adj_check(c);
+  if (rc) return rc;
+  fv_adj_state *a = c->adj;
+  if (!a || !a->fwd_done) return set_err(c, FVG_ERR_STATE, "fvg_adj_forward first");
+  (void)hipSetDevice(c->device);
+  DevMesh M = dev_mesh(c);
+  const size_t pl = a->plane * 8;
+  const int nc = nblk(c->N);
+  const double dt = a->dt;
+  HIPCHK(c, hipMemsetAsync(a->lam, 0, pl, c->stream));
+  for (int sg = a->nck - 1; sg >= 0; --sg) {
+    const int n0 = sg * a->ckpt, n1 = std::min(n0 + a->ckpt, a->nsteps);
+    HIPCHK(c, hipMemcpyAsync(c->S, a->ckpts + (size_t)sg * a->plane, pl, hipMemcpyDeviceToDevice,
+                             c->stream));
+    /* recompute the segment, keeping S^n, S1, S2 of every step and S^{n1} */
+    for (int n = n0; n < n1; ++n)
+      for (int k = 1; k <= 3; ++k) {
+        HIPCHK(c, hipMemcpyAsync(a->seg + (size_t)(3 * (n - n0) + k - 1) * a->plane, c->S, pl,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        fv_stage(c, k, dt, k == 1, false);
+      }
+    HIPCHK(c, hipMemcpyAsync(a->seg + (size_t)(3 * (n1 - n0)) * a->plane, c->S, pl,
+                             hipMemcpyDeviceToDevice, c->stream));
+    for (int n = n1 - 1; n >= n0; --n) {
+      /* d J_{n+1} / d S^{n+1} */
+      HIPCHK(c, hipMemcpyAsync(c->S, a->seg + (size_t)(3 * (n + 1 - n0)) * a->plane, pl,
+                               hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL(k_obj_grad, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, a->mask,
+                         dt, a->lam);
+      /* stage 3: S3 = 1/3 SN + 2/3 S2 + 2/3 dt/V R(S2);  stage 2: S2 = 3/4 SN + 1/4 S1 + ...;
+         stage 1: S1 = SN + dt/V R(SN) */
+      const double cst[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+      const double ast[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
+      const double bst[3] = {0.0, 1.0 / 4.0, 2.0 / 3.0};
+      for (int k = 3; k >= 1; --k) {
+        HIPCHK(c, hipMemcpyAsync(c->S, a->seg + (size_t)(3 * (n - n0) + k - 1) * a->plane, pl,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_adj_mu, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->lam,
+                           cst[k - 1] * dt, a->mu);
+        residual_vjp(c);
+        const int mode = k == 3 ? 0 : (k == 2 ? 1 : 2);
+        hipLaunchKernelGGL(k_adj_stage, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->lam, a->lamSN,
+                           a->nu, ast[k - 1], bst[k - 1], mode);
+      }
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(c->S, a->Sfinal, pl, hipMemcpyDeviceToDevice, c->stream));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  for (int v = 0; v < 5; ++v) {
+    if (grad_host)
+      HIPCHK(c, hipMemcpyAsync(grad_host + (size_t)v * c->N, a->lam + (size_t)v * c->ns,
+                               (size_t)c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    if (grad_dev)
+      HIPCHK(c, hipMemcpyAsync((double *)grad_dev + (size_t)v * c->N, a->lam + (size_t)v * c->ns,
+                               (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
+}
+
+int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, double *dJ) {
+  int rc = adj_check(c);
+  if (rc) return rc;
+  if (!dir || nsteps < 1 || !J || !dJ) return set_err(c, FVG_ERR_ARG, "bad tangent arguments");
+  if (c->rsize != 8) return set_err(c, FVG_ERR_UNSUPPORTED, "tangent runs in plain FP64");
+  (void)hipSetDevice(c->device);
+  rc = adj_alloc(c);
+  if (rc) return rc;
+  fv_adj_state *a = c->adj;
+  const size_t pl = a->plane * 8;
+  if (!a->Sd) {
+    HIPCHK(c, hipMalloc(&a->Sd, pl));
+    HIPCHK(c, hipMalloc(&a->SNd, pl));
+    HIPCHK(c, hipMalloc(&a->SFd, 6 * pl));
+    HIPCHK(c, hipMalloc(&a->Gd, (size_t)5 * c->nfp * 8));
+    HIPCHK(c, hipMemsetAsync(a->SFd, 0, 6 * pl, c->stream));
+  }
+  if (a->Jsteps) (void)hipFree(a->Jsteps);
+  if (a->Jdsteps) (void)hipFree(a->Jdsteps);
+  a->Jsteps = a->Jdsteps = nullptr;
+  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)nsteps * 8));
+  HIPCHK(c, hipMalloc(&a->Jdsteps, (size_t)nsteps * 8));
+  HIPCHK(c, hipMemsetAsync(a->Sd, 0, pl, c->stream));
+  for (int v = 0; v < 5; ++v)
+    HIPCHK(c, hipMemcpyAsync(a->Sd + (size_t)v * c->ns, dir + (size_t)v * c->N, (size_t)c->N * 8,
+                             hipMemcpyHostToDevice, c->stream));
+  DevMesh M = dev_mesh(c);
+  const int nc = nblk(c->N);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblk(n0), b1 = nblk(n1), b2 = nblk(n2);
+  for (int n = 0; n < nsteps; ++n) {
+    for (int k = 1; k <= 3; ++k) {
+      if (c->prm.ischeme == 2) {
+        fv_launch_recon(c);
+        hipLaunchKernelGGL(k_tlm_recon, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, a->Sd,
+                           a->SFd);
+      }
+      hipLaunchKernelGGL(k_tlm_face, dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream, M, c->dp, b0,
+                         b1, n0, n1, n2, c->S, a->Sd, (const double *)c->SF, a->SFd, a->Gd);
+      /* tangent stage update is the same linear combination (k_gather_rk on tangents) */
+      fv_launch_gather_ptrs(c, a->Gd, a->Sd, a->SNd, k, dt, k == 1);
+      fv_stage(c, k, dt, k == 1, false);
+    }
+    obj_step(c, a->Sd, n);
+  }
+  std::vector<double> js(nsteps), jd(nsteps);
+  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(jd.data(), a->Jdsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  double s0 = 0.0, s1 = 0.0;
+  for (int n = 0; n < nsteps; ++n) { s0 += dt * js[n]; s1 += dt * jd[n]; }
+  *J = s0;
+  *dJ = s1;
+  return FVG_OK;
+}
+
+/* host-side derivative checks of the face flux (no GPU needed) */
+int fvg_face_flux_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                       double *out) {
+  if (!p || axis < 0 || axis > 2 || !UL || !UR || !out || p->model != 1) return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  P.gamma = p->gamma;
+  P.pinf = p->pinf;
+  FState<double> L, R;
+  make_state_ideal<double>(P, UL, L);
+  make_state_ideal<double>(P, UR, R);
+  const double s = sign < 0 ? -1.0 : 1.0;
+  if (axis == 0) hllc_face<1, false, 0, double>(P, L, R, s, out);
+  else if (axis == 1) hllc_face<1, false, 1, double>(P, L, R, s, out);
+  else hllc_face<1, false, 2, double>(P, L, R, s, out);
+  return FVG_OK;
+}
+
+int fvg_face_vjp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                      const double *ob, double *ULb, double *URb) {
+  if (!p || axis < 0 || axis > 2 || !UL || !UR || !ob || !ULb || !URb || p->model != 1)
+    return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  P.gamma = p->gamma;
+  P.pinf = p->pinf;
+  for (int v = 0; v < 5; ++v) ULb[v] = URb[v] = 0.0;
+  const double s = sign < 0 ? -1.0 : 1.0;
+  if (axis == 0) hllc_vjp_ideal<0, double>(P, UL, UR, s, ob, ULb, URb);
+  else if (axis == 1) hllc_vjp_ideal<1, double>(P, UL, UR, s, ob, ULb, URb);
+  else hllc_vjp_ideal<2, double>(P, UL, UR, s, ob, ULb, URb);
+  return FVG_OK;
+}
+
+int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                      const double *ULd, const double *URd, double *outd) {
+  if (!p || axis < 0 || axis > 2 || !UL || !UR || !ULd || !URd || !outd || p->model != 1)
+    return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  P.gamma = p->gamma;
+  P.pinf = p->pinf;
+  typedef Dual<double> D;
+  D uL[5], uR[5], o[5];
+  for (int v = 0; v < 5; ++v) { uL[v] = D(UL[v], ULd[v]); uR[v] = D(UR[v], URd[v]); }
+  FState<D> L, R;
+  make_state_ideal<D>(P, uL, L);
+  make_state_ideal<D>(P, uR, R);
+  const D s(sign < 0 ? -1.0 : 1.0);
+  if (axis == 0) hllc_face<1, false, 0, D>(P, L, R, s, o);
+  else if (axis == 1) hllc_face<1, false, 1, D>(P, L, R, s, o);
+  else hllc_face<1, false, 2, D>(P, L, R, s, o);
+  for (int v = 0; v < 5; ++v) outd[v] = o[v].d;
+  return FVG_OK;
+}
+
+int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc, double dm1cc,
+                     double *g, double *g0, double *g1) {
+  if (!g || !g0 || !g1) return FVG_ERR_ARG;
+  *g = limited_d(limiter, eps, false, d0, d1, dp1cc, dm1cc, *g0, *g1);
+  return FVG_OK;
+}
+
+}  // extern "C"

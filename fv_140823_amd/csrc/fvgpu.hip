@@ -20,132 +20,7 @@
  * The gather reads each cell's face fluxes in the reference's order
  * f = 0..5, k = 0..3 (no atomics, bitwise deterministic).
  */
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdint.h>
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
-
-#include <string>
-#include <vector>
-
-#include "fvgpu.h"
-#include "fv_physics.h"
-
-/* ------------------------------------------------------------------ constants */
-
-__constant__ int c_face_axis[6] = {1, 0, 1, 0, 2, 2};
-__constant__ int c_face_sign[6] = {-1, 1, 1, -1, -1, 1};
-
-
-#define FV_BLOCK 256
-#define FV_NKT 8
-
-enum { CLS_SAME = 0, CLS_FINER = 1, CLS_COARSER = 2, CLS_BOUNDARY = 3 };
-
-static inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
-
-/* ------------------------------------------------------------------ mesh views */
-
-struct DevMesh {
-  int N, ns, F, nfp, Q, nlevel;
-  const uint32_t *cflag;
-  const int32_t *nbr;      /* [6][ns] */
-  const int32_t *quad;     /* [Q][4] */
-  const int32_t *fL, *fR;
-  const uint32_t *fmeta;
-  const int32_t *gslot;    /* [6][ns] */
-  const double *h, *dist, *area, *vol;
-};
-
-struct DevState {
-  double *S, *SN, *RHS, *SG;
-  void *SF, *SGm, *G;
-  int *err;        /* [0] first bad cell (min), [1] flags */
-  double *red;     /* reduction scratch */
-};
-
-/* ------------------------------------------------------------------ helpers */
-
-template <int NEQ>
-__device__ __forceinline__ void load_S(const double *__restrict__ S, int ns, int i, double *U) {
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) U[v] = S[(size_t)v * ns + i];
-}
-
-/* neighbour state across face f for MUSCL deltas (facevalues_MUSCL.c:18-49 and
- * the bc ghost of deltavalues.c:12-13) */
-template <int NEQ>
-__device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
-                                         const double *__restrict__ S, int i, uint32_t cf, int f,
-                                         const double *Uown, double *Un) {
-  int cls = (cf >> (2 * f)) & 3;
-  int nb = M.nbr[(size_t)f * M.ns + i];
-  if (cls == CLS_SAME || cls == CLS_COARSER) {
-    load_S<NEQ>(S, M.ns, nb, Un);
-  } else if (cls == CLS_FINER) {
-    int q[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = M.quad[4 * nb + k];
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      double acc = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) acc += S[(size_t)v * M.ns + q[k]] / ((double)4);
-      Un[v] = acc;
-    }
-  } else {
-    int bc = -1 - nb;
-    if (bc == 1) {
-#pragma unroll
-      for (int v = 0; v < NEQ; ++v) Un[v] = P.inlet[v];
-    } else {
-#pragma unroll
-      for (int v = 0; v < NEQ; ++v) Un[v] = Uown[v];
-    }
-  }
-}
-
-__device__ __forceinline__ void face_dist(const DevMesh &M, uint32_t cf, int f, double &dcf,
-                                          double &dcc) {
-  int cls = (cf >> (2 * f)) & 3;
-  if (cls == CLS_BOUNDARY) { dcf = 0.0; dcc = 0.0; return; }
-  int L = (cf >> 16) & 0xff;
-  int a = c_face_axis[f];
-  int c = cls == CLS_SAME ? 0 : (cls == CLS_FINER ? 1 : 2);
-  const double *d = M.dist + ((L * 3 + a) * 3 + c) * 2;
-  dcf = d[0];
-  dcc = d[1];
-}
-
-/* reconctruct_V3.c:11-45 for one variable; lim 1 = smooth van Albada (extension) */
-__device__ __forceinline__ double limited(int lim, double eps, bool bnd, double d0, double d1,
-                                          double dp1cc, double dm1cc) {
-  double grad = 0.0;
-  if (bnd) return grad;
-  if (lim == FVG_MINMOD) {
-    if (fabs(d0) > 1e-10) {
-      double a = -d1 / dm1cc, b = d0 / dp1cc;
-      double prod = a * b;
-      if (prod > 0.0) grad = fabs(a) > fabs(b) ? b : a;
-    }
-  } else {
-    double a = dm1cc != 0.0 ? -d1 / dm1cc : 0.0;
-    double b = dp1cc != 0.0 ? d0 / dp1cc : 0.0;
-    grad = (a * (b * b + eps) + b * (a * a + eps)) / (a * a + b * b + 2.0 * eps);
-  }
-  return grad;
-}
-
-/* XCD-aware block order: consecutive logical blocks (contiguous Morton ranges
- * of cells or faces) land on the same XCD and so share its L2. */
-__device__ __forceinline__ int logical_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int per = nb >> 3;
-  if (per == 0 || (nb & 7)) return b;
-  return (b & 7) * per + (b >> 3);
-}
+#include "fv_common.h"
 
 /* ------------------------------------------------------------------ kernels */
 
@@ -518,72 +393,6 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long val) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = val;
 }
 
-/* ------------------------------------------------------------------ context */
-
-struct fvg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  fvg_params prm;
-  DevParams dp;
-  int neq = 5;
-  bool has_mesh = false;
-  int N = 0, ns = 0, F = 0, nfp = 0, Q = 0, nlevel = 0;
-  int nfa[3] = {0, 0, 0};
-  /* device buffers */
-  uint32_t *cflag = nullptr;
-  int32_t *nbr = nullptr, *quad = nullptr, *fL = nullptr, *fR = nullptr, *gslot = nullptr;
-  uint32_t *fmeta = nullptr;
-  double *tab = nullptr;
-  double *S = nullptr, *SN = nullptr, *RHS = nullptr, *SG = nullptr;
-  void *SF = nullptr, *SGm = nullptr, *G = nullptr;
-  int *err = nullptr;
-  double *red = nullptr;
-  unsigned long long *redu = nullptr;
-  size_t red_cap = 0;
-  size_t rsize = 8;  /* bytes of R (8 plain, 4 mixed) */
-  std::string last;
-  hipEvent_t ev[2 * 64 + 2];
-  bool ev_ok = false;
-};
-
-static int set_err(fvg_ctx *c, int code, const char *msg) {
-  if (c) c->last = msg ? msg : "";
-  return code;
-}
-
-#define HIPCHK(c, call)                                                           \
-  do {                                                                            \
-    hipError_t e_ = (call);                                                       \
-    if (e_ != hipSuccess) {                                                       \
-      char b_[256];                                                               \
-      snprintf(b_, sizeof b_, "%s: %s", #call, hipGetErrorString(e_));            \
-      return set_err((c), FVG_ERR_HIP, b_);                                       \
-    }                                                                             \
-  } while (0)
-
-static DevMesh dev_mesh(const fvg_ctx *c) {
-  DevMesh m;
-  m.N = c->N;
-  m.ns = c->ns;
-  m.F = c->F;
-  m.nfp = c->nfp;
-  m.Q = c->Q;
-  m.nlevel = c->nlevel;
-  m.cflag = c->cflag;
-  m.nbr = c->nbr;
-  m.quad = c->quad;
-  m.fL = c->fL;
-  m.fR = c->fR;
-  m.fmeta = c->fmeta;
-  m.gslot = c->gslot;
-  const int nl = c->nlevel;
-  m.h = c->tab;
-  m.dist = c->tab + 3 * nl;
-  m.area = c->tab + 3 * nl + 18 * nl;
-  m.vol = c->tab + 3 * nl + 18 * nl + 3 * nl;
-  return m;
-}
-
 static void to_dev_params(const fvg_params *p, DevParams *d) {
   d->model = p->model;
   d->ns = p->ns;
@@ -621,6 +430,7 @@ static int check_params(const fvg_params *p) {
 }
 
 static void free_mesh(fvg_ctx *c) {
+  fv_adj_release(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->gslot, c->fmeta, c->tab,
                   c->S, c->SN, c->RHS, c->SG, c->SF, c->SGm, c->G};
   for (void *p : ptrs)
@@ -985,6 +795,24 @@ static const int *stage_list(int method, int *n) {
   return st1;
 }
 
+void fv_launch_recon(fvg_ctx *c) { dispatch_recon(c); }
+
+void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs) {
+  stage(c, irk, dt, first, write_rhs, nullptr, nullptr, nullptr);
+}
+
+void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
+                           bool first) {
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  if (first)
+    hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
+                       (double *)nullptr, irk, dt, c->err);
+  else
+    hipLaunchKernelGGL((k_gather_rk<5, double, false, true, false>), g, b, 0, c->stream, M, G, S, SN,
+                       (double *)nullptr, irk, dt, c->err);
+}
+
 extern "C" {
 
 int fvg_set_state(fvg_ctx *c, const double *S) { return copy_field(c, FVG_FIELD_S, (double *)S, true); }
@@ -993,6 +821,25 @@ int fvg_set_field(fvg_ctx *c, int which, const double *src) {
   return copy_field(c, which, (double *)src, true);
 }
 int fvg_get_field(fvg_ctx *c, int which, double *dst) { return copy_field(c, which, dst, false); }
+
+static int state_device_copy(fvg_ctx *c, void *d, bool to_ctx) {
+  if (!c || !d) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  (void)hipSetDevice(c->device);
+  for (int v = 0; v < c->neq; ++v) {
+    double *ctxp = c->S + (size_t)v * c->ns;
+    double *ext = (double *)d + (size_t)v * c->N;
+    if (to_ctx)
+      HIPCHK(c, hipMemcpyAsync(ctxp, ext, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIPCHK(c, hipMemcpyAsync(ext, ctxp, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
+}
+
+int fvg_set_state_device(fvg_ctx *c, const void *dS) { return state_device_copy(c, (void *)dS, true); }
+int fvg_get_state_device(fvg_ctx *c, void *dS) { return state_device_copy(c, dS, false); }
 
 int fvg_device_ptr(fvg_ctx *c, int which, void **ptr) {
   if (!c || !ptr) return FVG_ERR_ARG;

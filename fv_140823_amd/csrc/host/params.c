@@ -21,7 +21,9 @@ void fvg_params_default(fvg_params *p, int model) {
   p->ischeme = 2;
   p->limiter = FVG_MINMOD;
   p->precision = FVG_PLAIN;
-  p->visc_work = 1;
+  /* con->eos gates the viscous work (flux_viscous.c:227) and is never set; CON is
+   * malloc'd at ~2 MB (createrunstruct.c:27), i.e. fresh zero pages, so it reads 0 */
+  p->visc_work = 0;
   p->gamma = 1.4;
   p->pinf = 0.0;
   p->mu = 0.0;

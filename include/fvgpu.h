@@ -59,7 +59,8 @@ typedef struct fvg_params {
   int ischeme;    /* 1: first order face states (facevalues.c); 2: MUSCL (faceinterp.c) */
   int limiter;    /* FVG_MINMOD | FVG_VANALBADA */
   int precision;  /* FVG_PLAIN | FVG_MIXED */
-  int visc_work;  /* viscous work in the energy row: the reference's con->eos==1 (flux_viscous.c:227) */
+  int visc_work;  /* the reference's con->eos == 1: subtract u.tau from row 4 (flux_viscous.c:227-232);
+                     default 0, the value the never-set flag reads in the reference */
   double gamma;   /* par->matergama[0] */
   double pinf;    /* par->materpinf[0] */
   double mu;      /* par->matervisc[0] */
@@ -172,6 +173,35 @@ const char *fvg_last_error(const fvg_ctx *ctx);
  * (0 recon, 1 gradient, 2 face flux, 3 gather+RK, 4 RK(0) copy). */
 int fvg_time_steps(fvg_ctx *ctx, int nsteps, int method, double dt, double *ms_total,
                    double *ms_kernel /* [8] */);
+
+/* ---------------------------------------------------------------- adjoint / tangent */
+/* Rollout objective J = sum_{n=1..nsteps} dt sum_{i in probe} p(S^n_i) Vol_i of nsteps SSP-RK3
+ * steps at a fixed dt from the context's S (model 1, inviscid; either limiter).  This is an
+ * extension: the reference has no adjoint. */
+int fvg_adj_set_probe(fvg_ctx *ctx, const uint8_t *mask /* host [ncell], 1 = in the probe box */);
+/* Runs the rollout, keeping S every `ckpt` steps; *J = objective; S ends at the final state. */
+int fvg_adj_forward(fvg_ctx *ctx, int nsteps, int ckpt, double dt, double *J);
+/* Discrete adjoint dJ/dS^0 [neq][ncell] of the last forward rollout, recomputing stage
+ * states from the checkpoints; into host memory and/or device memory (either may be NULL). */
+int fvg_adj_backward(fvg_ctx *ctx, double *grad_host, void *grad_dev);
+/* Tangent-linear rollout from the context's S along dir [neq][ncell] (host): *J and the
+ * directional derivative *dJ (forward-mode, FP64 only).  S ends at the final state. */
+int fvg_tlm(fvg_ctx *ctx, const double *dir, int nsteps, double dt, double *J, double *dJ);
+/* Host evaluations of the device face-flux code for derivative checks (model 1, inviscid):
+ * flux (F.n + HLLC correction, before -Area) across a face of normal sign*e_axis, its
+ * vector-Jacobian product (reverse) and Jacobian-vector product (dual numbers), and the
+ * limited slope with its partials w.r.t. delta0 and delta1. */
+int fvg_face_flux_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                       double *out);
+int fvg_face_vjp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                      const double *ob, double *ULb, double *URb);
+int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
+                      const double *ULd, const double *URd, double *outd);
+int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc, double dm1cc,
+                     double *g, double *g0, double *g1);
+/* device-to-device state copies on the context's stream (torch interop) */
+int fvg_set_state_device(fvg_ctx *ctx, const void *dS /* device [neq][ncell] f64 */);
+int fvg_get_state_device(fvg_ctx *ctx, void *dS);
 
 /* ---------------------------------------------------------------- RUN binding */
 /* Flatten `run`, create a context on `device` and upload S; afterwards RHS(run), RK(irk,run),

@@ -20,6 +20,17 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
+def _tol(model, neq):
+    """1e-12 per row; the model-3 mass-fraction row gets 1e-11: its flux is ~1e-3 of
+    the momentum flux and its star-state correction S_K (C Y - rho Y) amplifies the
+    1-ulp difference between the reference's two evaluations of S* (one from each
+    side, compute_wave_speeds.c:81-82) and the single evaluation here."""
+    t = np.full(neq, TOL)
+    if model == 3:
+        t[4] = 1e-11
+    return t
+
+
 def _mesh(kind, neq):
     if kind == "uniform":
         return uniform(3, neq)
@@ -48,17 +59,17 @@ def test_rhs_parity(model, ischeme, ns, mesh):
     F = _mesh(mesh, neq)
     xc, _ = F.leaf_info()
     F.set_state(state(model, xc, seed=model * 10 + ischeme))
-    p = fv.params(model, ischeme=ischeme, ns=ns, mu=0.05 if ns else 0.0)
+    p = fv.params(model, ischeme=ischeme, ns=ns, mu=0.05 if ns else 0.0, visc_work=ns)
     ref = oracle.full_rhs(F, p)
     g = fv.Solver.from_forest(F, p)
     got = g.full_rhs()
-    err = rel_linf(got, ref)
+    err = rel_linf(got, ref, neq)
     print("rel Linf RHS", err, flush=True)
-    assert np.all(err <= TOL), err
+    assert np.all(err <= _tol(model, neq)), err
     if ischeme == 2:
         sf_ref = F.get_field(fv.FIELD_SF)
         sf = g.get_field(fv.FIELD_SF)
-        e = rel_linf(sf.reshape(6 * neq, -1), sf_ref.reshape(6 * neq, -1))
+        e = rel_linf(sf.reshape(6 * neq, -1), sf_ref.reshape(6 * neq, -1), neq)
         assert np.all(e <= TOL), e
     if ns:
         g.gradcalc()
@@ -103,7 +114,7 @@ def test_steps_parity_amr():
     for _ in range(4):
         oracle.step(F, p, 3, dt)
     g.steps(4, dt, 3)
-    e = rel_linf(g.get_state(), F.get_state())
+    e = rel_linf(g.get_state(), F.get_state(), 5)
     print("rel Linf S after 4 steps", e, flush=True)
     assert np.all(e <= 1e-11), e
 
@@ -121,7 +132,7 @@ def test_sod_per_step_residuals():
         F.set_state(S)
         ref = oracle.full_rhs(F, p)
         got = g.full_rhs()
-        worst = np.maximum(worst, rel_linf(got, ref))
+        worst = np.maximum(worst, rel_linf(got, ref, 5))
         g.step(dt, 3)
     print("worst per-step rel Linf", worst, flush=True)
     assert np.all(worst <= TOL), worst
@@ -168,7 +179,7 @@ def test_mixed_close_to_plain():
     gp = fv.Solver.from_forest(F, fv.params(1))
     gm = fv.Solver.from_forest(F, fv.params(1, precision="mixed"))
     rp, rm = gp.full_rhs(), gm.full_rhs()
-    e = rel_linf(rm, rp)
+    e = rel_linf(rm, rp, 5)
     print("mixed vs plain RHS rel Linf", e, flush=True)
     assert np.all(e < 1e-5), e
 
@@ -193,7 +204,7 @@ def test_run_binding_compat():
         got = F.get_field(fv.FIELD_RHS)
         F.set_state(S0)
         ref = oracle.full_rhs(F, p)
-        assert np.all(rel_linf(got, ref) <= TOL)
+        assert np.all(rel_linf(got, ref, 5) <= TOL)
     finally:
         L.fv_gpu_detach(run)
 
