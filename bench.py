@@ -39,6 +39,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (calibrates rocprof byte counters)")
     return ap.parse_args()
 
 
@@ -47,15 +49,22 @@ def log(msg, quiet=False):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes(kernel, m, neq, rs, ns_visc, first_stage):
+def alg_bytes(kernel, m, neq, rs, ns_visc, first_stage, fused=True):
     """Algorithmic HBM bytes of one launch: each input read once, each output
-    written once (per-unit figures in the design notes)."""
+    written once (per-unit figures in the design notes).  With `fused` the face
+    kernel reconstructs its face states from S (MUSCL in the flux kernel)."""
     N, F, Q, Fb = m["N"], m["F"], m["Q"], m["Fb"]
+    meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
     if kernel == "recon":
-        return 8 * neq * N + 4 * N + 24 * N + 16 * Q + 6 * neq * rs * N
+        return 8 * neq * N + meta_c + 6 * neq * rs * N
     if kernel == "grad":
-        return 8 * 3 * N + 4 * N + 24 * N + 16 * Q + 9 * rs * N
+        return 8 * 3 * N + meta_c + 9 * rs * N
     if kernel == "face":
+        if fused:
+            b = 8 * neq * N + meta_c + 12 * F + neq * rs * F
+            if ns_visc:
+                b += 9 * rs * N
+            return b
         b = 12 * F + neq * rs * (2 * F - Fb) + neq * rs * F
         if ns_visc:
             b += 9 * rs * (2 * F - Fb)
@@ -175,6 +184,16 @@ def main():
     import torch
 
     import fv_140823_amd as fv
+    if a.copy_calibration:
+        n = a.copy_calibration * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "bytes_read": n * 8,
+                          "bytes_written": n * 8}))
+        return
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
@@ -200,14 +219,14 @@ def main():
     per_step = {KERNELS[i]: k_p[i] * (3 if i < 4 else 1) for i in range(4) if k_p[i] > 0}
     dom = max(per_step, key=per_step.get)
     di = KERNELS.index(dom)
-    nb = alg_bytes(dom, mf, neq, 8, p.ns, False)
+    nb = alg_bytes(dom, mf, neq, 8, p.ns, False, p.ischeme == 2)
     achieved = nb / (k_p[di] * 1e-3) / 1e9
     roof = dict(bound="hbm", kernel=f"k_{dom}", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                 alg_bytes_per_launch=int(nb), avg_launch_ms=round(k_p[di], 5))
     kern = {KERNELS[i]: dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5),
-                             plain_gbs=round(alg_bytes(KERNELS[i], mf, neq, 8, p.ns, False) / (k_p[i] * 1e-3) / 1e9, 1) if k_p[i] else None,
-                             mixed_gbs=round(alg_bytes(KERNELS[i], mf, neq, 4, p.ns, False) / (k_m[i] * 1e-3) / 1e9, 1) if k_m[i] else None)
+                             plain_gbs=round(alg_bytes(KERNELS[i], mf, neq, 8, p.ns, False, p.ischeme == 2) / (k_p[i] * 1e-3) / 1e9, 1) if k_p[i] else None,
+                             mixed_gbs=round(alg_bytes(KERNELS[i], mf, neq, 4, p.ns, False, p.ischeme == 2) / (k_m[i] * 1e-3) / 1e9, 1) if k_m[i] else None)
             for i in range(4) if k_p[i] > 0}
     acc = l1_rel_density(F, p, pm, a.acc_steps, dt)
     log(f"mixed vs plain after {a.acc_steps} steps: {acc}", a.quiet)

@@ -165,6 +165,14 @@ def lib() -> C.CDLL:
                 f"{_LIB_PATH} is missing: build it with `python -m fv_140823_amd.build` "
                 "(or __graft_entry__.build()); the HIP path has no fallback"
             )
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+        # but asks for it by its unversioned name, so if this library were loaded
+        # first the process would end up with two HIP runtimes and torch would see
+        # no GPU.  Loading torch first makes this library bind to torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(os.fspath(_LIB_PATH))
         for name, res, args in _DECLS:
             fn = getattr(L, name)

@@ -294,6 +294,42 @@ class Solver:
                  "fvg_time_steps")
         return tot.value, list(k)
 
+    # ---- device-pointer state copies (torch interop, stream-synchronous)
+    def set_state_device(self, ptr: int) -> None:
+        self._ck(lib().fvg_set_state_device(self._h, C.c_void_p(ptr)), "fvg_set_state_device")
+
+    def get_state_device(self, ptr: int) -> None:
+        self._ck(lib().fvg_get_state_device(self._h, C.c_void_p(ptr)), "fvg_get_state_device")
+
+    # ---- adjoint / tangent of rollouts
+    def adj_set_probe(self, mask: np.ndarray) -> None:
+        m = np.ascontiguousarray(mask, np.uint8)
+        if m.shape != (self.ncell,):
+            raise ValueError("one mask byte per cell")
+        self._ck(lib().fvg_adj_set_probe(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))),
+                 "fvg_adj_set_probe")
+
+    def adj_forward(self, nsteps: int, ckpt: int, dt: float) -> float:
+        J = C.c_double()
+        self._ck(lib().fvg_adj_forward(self._h, int(nsteps), int(ckpt), float(dt), C.byref(J)),
+                 "fvg_adj_forward")
+        return J.value
+
+    def adj_backward(self) -> np.ndarray:
+        g = np.empty((self.neq, self.ncell), np.float64)
+        self._ck(lib().fvg_adj_backward(self._h, _dptr(g), None), "fvg_adj_backward")
+        return g
+
+    def adj_backward_device(self, ptr: int) -> None:
+        self._ck(lib().fvg_adj_backward(self._h, None, C.c_void_p(ptr)), "fvg_adj_backward")
+
+    def tlm(self, direction: np.ndarray, nsteps: int, dt: float):
+        d = np.ascontiguousarray(direction, np.float64).reshape(self.neq, self.ncell)
+        J, dJ = C.c_double(), C.c_double()
+        self._ck(lib().fvg_tlm(self._h, _dptr(d), int(nsteps), float(dt), C.byref(J), C.byref(dJ)),
+                 "fvg_tlm")
+        return J.value, dJ.value
+
     def full_rhs(self) -> np.ndarray:
         """faceinterp (MUSCL) + gradcalc (viscous) + RHS of the current S."""
         if self.params.ischeme == 2:

@@ -41,15 +41,20 @@ struct DevMesh {
   const int32_t *fL, *fR;
   const uint32_t *fmeta;
   const int32_t *gslot;    /* [6][ns] */
-  const double *h, *dist, *area, *vol;
+  const double *h, *dist, *area, *vol, *idist;
 };
 
 /* ------------------------------------------------------------------ helpers */
 
+/* The conserved-state arrays S, SN, RHS are cell-major (array of structs):
+ * S[i*NEQ + v], so the NEQ values of a gathered neighbour share one or two cache
+ * lines.  The face flux G is face-major: G[j*NEQ + v]. */
 template <int NEQ>
 __device__ __forceinline__ void load_S(const double *__restrict__ S, int ns, int i, double *U) {
+  (void)ns;
+  const double *p = S + (size_t)i * NEQ;
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) U[v] = S[(size_t)v * ns + i];
+  for (int v = 0; v < NEQ; ++v) U[v] = p[v];
 }
 
 /* neighbour state across face f for MUSCL deltas (facevalues_MUSCL.c:18-49 and
@@ -70,7 +75,7 @@ __device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
     for (int v = 0; v < NEQ; ++v) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc += S[(size_t)v * M.ns + q[k]] / ((double)4);
+      for (int k = 0; k < 4; ++k) acc += S[(size_t)q[k] * NEQ + v] / ((double)4);
       Un[v] = acc;
     }
   } else {
@@ -86,31 +91,35 @@ __device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
 }
 
 __device__ __forceinline__ void face_dist(const DevMesh &M, uint32_t cf, int f, double &dcf,
-                                          double &dcc) {
+                                          double &dcc, double &idcc) {
   int cls = (cf >> (2 * f)) & 3;
-  if (cls == CLS_BOUNDARY) { dcf = 0.0; dcc = 0.0; return; }
+  if (cls == CLS_BOUNDARY) { dcf = 0.0; dcc = 0.0; idcc = __builtin_inf(); return; }
   int L = (cf >> 16) & 0xff;
   int a = c_face_axis[f];
   int c = cls == CLS_SAME ? 0 : (cls == CLS_FINER ? 1 : 2);
-  const double *d = M.dist + ((L * 3 + a) * 3 + c) * 2;
-  dcf = d[0];
-  dcc = d[1];
+  const int k = (L * 3 + a) * 3 + c;
+  dcf = M.dist[2 * k];
+  dcc = M.dist[2 * k + 1];
+  idcc = M.idist[k];
 }
 
-/* reconctruct_V3.c:11-45 for one variable; lim 1 = smooth van Albada (extension) */
+/* reconctruct_V3.c:11-45 for one variable, with the distances d_cc of the two
+ * faces given as reciprocals (ip = 1/d_cc(f), im = 1/d_cc(fo)); a boundary's
+ * 1/0 = inf keeps the reference's 0/0 -> NaN -> no slope and inf -> the finite
+ * slope.  lim 1 = smooth van Albada (extension). */
 __device__ __forceinline__ double limited(int lim, double eps, bool bnd, double d0, double d1,
-                                          double dp1cc, double dm1cc) {
+                                          double ip, double im) {
   double grad = 0.0;
   if (bnd) return grad;
   if (lim == FVG_MINMOD) {
     if (fabs(d0) > 1e-10) {
-      double a = -d1 / dm1cc, b = d0 / dp1cc;
+      double a = -d1 * im, b = d0 * ip;
       double prod = a * b;
       if (prod > 0.0) grad = fabs(a) > fabs(b) ? b : a;
     }
   } else {
-    double a = dm1cc != 0.0 ? -d1 / dm1cc : 0.0;
-    double b = dp1cc != 0.0 ? d0 / dp1cc : 0.0;
+    double a = isinf(im) ? 0.0 : -d1 * im;
+    double b = isinf(ip) ? 0.0 : d0 * ip;
     grad = (a * (b * b + eps) + b * (a * a + eps)) / (a * a + b * b + 2.0 * eps);
   }
   return grad;
@@ -198,6 +207,7 @@ static inline DevMesh dev_mesh(const fvg_ctx *c) {
   m.dist = c->tab + 3 * nl;
   m.area = c->tab + 3 * nl + 18 * nl;
   m.vol = c->tab + 3 * nl + 18 * nl + 3 * nl;
+  m.idist = c->tab + 25 * nl;
   return m;
 }
 

@@ -53,17 +53,19 @@ FV_HD __forceinline__ bool eos64(const DevParams &P, const double *U, bool clip,
   /* no fused multiply-add here: the two-phase root -b + sqrt(b^2 - 4d) cancels
    * about three digits, so every product is rounded as in the reference */
 #pragma clang fp contract(off)
+  /* one reciprocal of rho replaces the reference's four divisions by it */
   r = U[0];
-  u[0] = U[1] / r;
-  u[1] = U[2] / r;
-  u[2] = U[3] / r;
+  const double ir = 1.0 / r;
+  u[0] = U[1] * ir;
+  u[1] = U[2] * ir;
+  u[2] = U[3] * ir;
   e = 0.0;
   y = 0.0;
   if constexpr (MODEL == 1) {
     e = U[4];
-    double ei = (e / r) - 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    double ei = (e * ir) - 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     p = r * (P.gamma - 1.0) * ei - P.gamma * P.pinf;
-    c = sqrt(P.gamma * (p + P.pinf) / r);
+    c = sqrt(P.gamma * (p + P.pinf) * ir);
     return true;
   } else if constexpr (MODEL == 2) {
     const double rsat = P.baro_rsat;
@@ -76,7 +78,7 @@ FV_HD __forceinline__ bool eos64(const DevParams &P, const double *U, bool clip,
     }
     return true;
   } else {
-    y = U[4] / r;
+    y = U[4] * ir;
     if (clip) {
       if (y < (P.ymin / 1000.0)) y = (P.ymin / 1000.0);
       if (y > (1.0 - (P.ymin / 1000.0))) y = 1.0 - (P.ymin / 1000.0);
@@ -241,13 +243,14 @@ template <typename T>
 FV_HD __forceinline__ void make_state_ideal(const DevParams &P, const T *U, FState<T> &s) {
   const T g = (T)P.gamma, pinf = (T)P.pinf;
   s.r = U[0];
-  s.u[0] = U[1] / s.r;
-  s.u[1] = U[2] / s.r;
-  s.u[2] = U[3] / s.r;
+  const T ir = (T)1.0 / s.r;
+  s.u[0] = U[1] * ir;
+  s.u[1] = U[2] * ir;
+  s.u[2] = U[3] * ir;
   s.e = U[4];
-  T ei = (s.e / s.r) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+  T ei = (s.e * ir) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
   s.p = s.r * (g - (T)1.0) * ei - g * pinf;
-  s.c = sqrt(g * (s.p + pinf) / s.r);
+  s.c = sqrt(g * (s.p + pinf) * ir);
   s.y = (T)0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) s.q[v] = U[v];

@@ -68,9 +68,9 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    UL[v] = first ? (T)S[(size_t)v * M.ns + iL] : (T)SF[((size_t)fl * 5 + v) * M.ns + iL];
+    UL[v] = first ? (T)S[(size_t)iL * 5 + v] : (T)SF[((size_t)fl * 5 + v) * M.ns + iL];
     if (bc == 0)
-      UR[v] = first ? (T)S[(size_t)v * M.ns + iR] : (T)SF[((size_t)fr * 5 + v) * M.ns + iR];
+      UR[v] = first ? (T)S[(size_t)iR * 5 + v] : (T)SF[((size_t)fr * 5 + v) * M.ns + iR];
     else if (bc == 1)
       UR[v] = (T)P.inlet[v];
     else
@@ -179,9 +179,9 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
     nb_state<5>(M, P, S, i, cf, fm, U, Um);
     sf_bar<R>(M, lUL, lUR, i, cf, fp, lp);
     sf_bar<R>(M, lUL, lUR, i, cf, fm, lm);
-    double pcf, pcc, mcf, mcc;
-    face_dist(M, cf, fp, pcf, pcc);
-    face_dist(M, cf, fm, mcf, mcc);
+    double pcf, pcc, mcf, mcc, pic, mic;
+    face_dist(M, cf, fp, pcf, pcc, pic);
+    face_dist(M, cf, fm, mcf, mcc, mic);
     const int cp = (cf >> (2 * fp)) & 3, cm = (cf >> (2 * fm)) & 3;
     const bool bp = cp == CLS_BOUNDARY, bm = cm == CLS_BOUNDARY;
     const bool ghost_p = bp && (-1 - M.nbr[(size_t)fp * M.ns + i]) == 2;
@@ -363,9 +363,9 @@ __global__ __launch_bounds__(FV_BLOCK) void k_tlm_recon(DevMesh M, DevParams P,
     nb_state<5>(M, P, S, i, cf, fm, U, Um);
     nb_state<5>(M, Pz, Sd, i, cf, fp, D, Dp);
     nb_state<5>(M, Pz, Sd, i, cf, fm, D, Dm);
-    double pcf, pcc, mcf, mcc;
-    face_dist(M, cf, fp, pcf, pcc);
-    face_dist(M, cf, fm, mcf, mcc);
+    double pcf, pcc, mcf, mcc, pic, mic;
+    face_dist(M, cf, fp, pcf, pcc, pic);
+    face_dist(M, cf, fm, mcf, mcc, mic);
     const bool bp = ((cf >> (2 * fp)) & 3) == CLS_BOUNDARY;
     const bool bm = ((cf >> (2 * fm)) & 3) == CLS_BOUNDARY;
 #pragma unroll
@@ -392,15 +392,17 @@ __device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams 
   const uint32_t meta = M.fmeta[j];
   const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
   const bool first = P.ischeme == 1;
+  /* first order reads the cell-major S; MUSCL the face states [6][5][ns] */
   const double *X = first ? S : SF;
   const double *Xd = first ? Sd : SFd;
-  const size_t oL = first ? (size_t)iL : (size_t)fl * 5 * M.ns + iL;
-  const size_t oR = first ? (size_t)iR : (size_t)fr * 5 * M.ns + iR;
+  const size_t oL = first ? (size_t)iL * 5 : (size_t)fl * 5 * M.ns + iL;
+  const size_t oR = first ? (size_t)iR * 5 : (size_t)fr * 5 * M.ns + iR;
+  const size_t st = first ? 1 : (size_t)M.ns;
   D UL[5], UR[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    UL[v] = D(X[oL + (size_t)v * M.ns], Xd[oL + (size_t)v * M.ns]);
-    if (bc == 0) UR[v] = D(X[oR + (size_t)v * M.ns], Xd[oR + (size_t)v * M.ns]);
+    UL[v] = D(X[oL + v * st], Xd[oL + v * st]);
+    if (bc == 0) UR[v] = D(X[oR + v * st], Xd[oR + v * st]);
     else if (bc == 1) UR[v] = D(P.inlet[v], 0.0);
     else UR[v] = UL[v];
   }
@@ -411,7 +413,7 @@ __device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams 
   hllc_face<1, false, a, D>(P, L, R, D((double)c_face_sign[fl]), out);
   const double A = M.area[3 * lev + a];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) Gd[(size_t)v * M.nfp + j] = -(A)*out[v].d;
+  for (int v = 0; v < 5; ++v) Gd[(size_t)j * 5 + v] = -(A)*out[v].d;
 }
 
 __global__ __launch_bounds__(FV_BLOCK) void k_tlm_face(DevMesh M, DevParams P, int b0, int b1,
@@ -646,9 +648,14 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
   HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)nsteps * 8));
   HIPCHK(c, hipMalloc(&a->Jdsteps, (size_t)nsteps * 8));
   HIPCHK(c, hipMemsetAsync(a->Sd, 0, pl, c->stream));
-  for (int v = 0; v < 5; ++v)
-    HIPCHK(c, hipMemcpyAsync(a->Sd + (size_t)v * c->ns, dir + (size_t)v * c->N, (size_t)c->N * 8,
-                             hipMemcpyHostToDevice, c->stream));
+  {
+    /* the tangent state is cell-major like S */
+    std::vector<double> t((size_t)5 * c->N);
+    for (int v = 0; v < 5; ++v)
+      for (int i = 0; i < c->N; ++i) t[(size_t)i * 5 + v] = dir[(size_t)v * c->N + i];
+    HIPCHK(c, hipMemcpyAsync(a->Sd, t.data(), t.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   DevMesh M = dev_mesh(c);
   const int nc = nblk(c->N);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];

@@ -42,17 +42,17 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
     double Up[NEQ], Um[NEQ];
     nb_state<NEQ>(M, P, S, i, cf, fp, U, Up);
     nb_state<NEQ>(M, P, S, i, cf, fm, U, Um);
-    double pcf, pcc, mcf, mcc;
-    face_dist(M, cf, fp, pcf, pcc);
-    face_dist(M, cf, fm, mcf, mcc);
+    double pcf, pcc, mcf, mcc, pic, mic;
+    face_dist(M, cf, fp, pcf, pcc, pic);
+    face_dist(M, cf, fm, mcf, mcc, mic);
     const bool bp = ((cf >> (2 * fp)) & 3) == CLS_BOUNDARY;
     const bool bm = ((cf >> (2 * fm)) & 3) == CLS_BOUNDARY;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       double dP = Up[v] - U[v];
       double dM = Um[v] - U[v];
-      double gp = limited(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc);
-      double gm = limited(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc);
+      double gp = limited(P.limiter, P.va_eps, bp, dP, dM, pic, mic);
+      double gm = limited(P.limiter, P.va_eps, bm, dM, dP, mic, pic);
       SF[((size_t)fp * NEQ + v) * M.ns + i] = (R)(U[v] + pcf * gp);
       SF[((size_t)fm * NEQ + v) * M.ns + i] = (R)(U[v] + mcf * gm);
     }
@@ -144,11 +144,11 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
   const bool first = P.ischeme == 1;
 #pragma unroll
   for (int v = 0; v < NEQ; ++v)
-    UL[v] = first ? S[(size_t)v * M.ns + iL] : (double)SF[((size_t)fl * NEQ + v) * M.ns + iL];
+    UL[v] = first ? S[(size_t)iL * NEQ + v] : (double)SF[((size_t)fl * NEQ + v) * M.ns + iL];
   if (bc == 0) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v)
-      UR[v] = first ? S[(size_t)v * M.ns + iR] : (double)SF[((size_t)fr * NEQ + v) * M.ns + iR];
+      UR[v] = first ? S[(size_t)iR * NEQ + v] : (double)SF[((size_t)fr * NEQ + v) * M.ns + iR];
   } else if (bc == 1) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
@@ -173,9 +173,93 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
     T g = -(A) * out[v];
-    G[(size_t)v * M.nfp + j] = (R)g;
+    G[(size_t)j * NEQ + v] = (R)g;
   }
   if (!ok) atomicMin(err, iL);
+}
+
+/* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face): the same
+ * operations as k_recon, rounded to the storage type R of the unfused path */
+template <int NEQ, typename R>
+__device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
+                                           const double *__restrict__ S, int c, int f,
+                                           double *out) {
+  const uint32_t cf = M.cflag[c];
+  const int fo = f ^ (f < 4 ? 2 : 1); /* fcopp: 0<->2, 1<->3, 4<->5 */
+  double U[NEQ], U0[NEQ], U1[NEQ];
+  load_S<NEQ>(S, M.ns, c, U);
+  nb_state<NEQ>(M, P, S, c, cf, f, U, U0);
+  nb_state<NEQ>(M, P, S, c, cf, fo, U, U1);
+  double pcf, pcc, mcf, mcc, pic, mic;
+  face_dist(M, cf, f, pcf, pcc, pic);
+  face_dist(M, cf, fo, mcf, mcc, mic);
+  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const double g = limited(P.limiter, P.va_eps, bnd, U0[v] - U[v], U1[v] - U[v], pic, mic);
+    out[v] = (double)(R)(U[v] + pcf * g);
+  }
+}
+
+/* face flux with the reconstruction fused in: reads S (the 2-ring through the
+ * caches) instead of a stored SF, same results as recon + face */
+template <int MODEL, bool NS, int a, typename T, typename R>
+__device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParams &P, int j,
+                                                const double *__restrict__ S,
+                                                const R *__restrict__ SGm, R *__restrict__ G,
+                                                int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int iL = M.fL[j], iR = M.fR[j];
+  const uint32_t meta = M.fmeta[j];
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const T s = (T)c_face_sign[fl];
+  double UL[NEQ], UR[NEQ];
+  recon_face<NEQ, R>(M, P, S, iL, fl, UL);
+  if (bc == 0) {
+    recon_face<NEQ, R>(M, P, S, iR, fr, UR);
+  } else if (bc == 1) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) UR[v] = UL[v];
+  }
+  FState<T> L, Rs;
+  bool ok = make_state<MODEL, T>(P, UL, true, L);
+  ok = make_state<MODEL, T>(P, UR, bc != 1, Rs) && ok;
+  if constexpr (NS) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
+      Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
+    }
+  }
+  T out[NEQ];
+  ok = hllc_face<MODEL, NS, a, T>(P, L, Rs, s, out) && ok;
+  const T A = (T)M.area[3 * lev + a];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) G[(size_t)j * NEQ + v] = (R)(-(A) * out[v]);
+  if (!ok) atomicMin(err, iL);
+}
+
+template <int MODEL, bool NS, typename T, typename R>
+__global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_face_fused(DevMesh M, DevParams P, int b0, int b1,
+                                                         int n0, int n1, int n2,
+                                                         const double *__restrict__ S,
+                                                         const R *__restrict__ SGm,
+                                                         R *__restrict__ G, int *__restrict__ err) {
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  if (lb < b0) {
+    const int j = lb * FV_BLOCK + t;
+    if (j < n0) face_fused_body<MODEL, NS, 0, T, R>(M, P, j, S, SGm, G, err);
+  } else if (lb < b0 + b1) {
+    const int q = (lb - b0) * FV_BLOCK + t;
+    if (q < n1) face_fused_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SGm, G, err);
+  } else {
+    const int q = (lb - b0 - b1) * FV_BLOCK + t;
+    if (q < n2) face_fused_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SGm, G, err);
+  }
 }
 
 /* Faces are grouped by axis; blocks [0, b0) take x faces, [b0, b0+b1) y faces,
@@ -202,71 +286,111 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
 }
 
 /* RHS gather (residual_faces_AMR.c:173-175, order f = 0..5, k = 0..3) fused with
- * the RK stage (RK.c:24-47).  irk 0 < stage: 1..4; FIRST: also SN = S (RK(0)). */
+ * the RK stage (RK.c:24-47).  irk 1..4; FIRST: also SN = S (RK(0)).
+ * A block owns 256 consecutive cells, whose cell-major S / SN / RHS rows form one
+ * contiguous span: it is moved between HBM and LDS with fully coalesced accesses
+ * and each thread works on its own row in LDS. */
 template <int NEQ, typename R, bool WRITE_RHS, bool DO_RK, bool FIRST>
 __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__restrict__ G,
                                                         double *__restrict__ S,
                                                         double *__restrict__ SN,
                                                         double *__restrict__ RHS, int irk,
                                                         double dt, int *__restrict__ err) {
-  const int i = logical_block() * FV_BLOCK + threadIdx.x;
-  if (i >= M.N) return;
-  const uint32_t cf = M.cflag[i];
+  __shared__ double shS[FV_BLOCK * NEQ];
+  __shared__ double shN[FV_BLOCK * NEQ];
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  const int i0 = lb * FV_BLOCK;
+  const int i = i0 + t;
+  const int nrow = min(FV_BLOCK, M.N - i0);
+  const int nval = nrow * NEQ;
+  const size_t base = (size_t)i0 * NEQ;
+  if constexpr (DO_RK) {
+    for (int k = t; k < nval; k += FV_BLOCK) {
+      shS[k] = S[base + k];
+      if constexpr (!FIRST) shN[k] = SN[base + k];
+    }
+  }
   double rhs[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) rhs[v] = 0.0;
+  const bool live = i < M.N;
+  uint32_t cf = 0;
+  if (live) {
+    cf = M.cflag[i];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int g = M.gslot[(size_t)f * M.ns + i];
-    const int cls = (cf >> (2 * f)) & 3;
-    double ff[NEQ];
+    for (int f = 0; f < 6; ++f) {
+      const int g = M.gslot[(size_t)f * M.ns + i];
+      const int cls = (cf >> (2 * f)) & 3;
+      double ff[NEQ];
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) ff[v] = 0.0;
-    if (g >= 0) {
-      const int jf = g >> 1;
-      if (cls == CLS_FINER) {
+      for (int v = 0; v < NEQ; ++v) ff[v] = 0.0;
+      if (g >= 0) {
+        const int jf = g >> 1;
+        if (cls == CLS_FINER) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+          for (int k = 0; k < 4; ++k)
 #pragma unroll
-          for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)v * M.nfp + jf + k];
-      } else if (g & 1) {
+            for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)(jf + k) * NEQ + v];
+        } else if (g & 1) {
 #pragma unroll
-        for (int v = 0; v < NEQ; ++v) ff[v] += -(double)G[(size_t)v * M.nfp + jf];
-      } else {
+          for (int v = 0; v < NEQ; ++v) ff[v] += -(double)G[(size_t)jf * NEQ + v];
+        } else {
 #pragma unroll
-        for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)v * M.nfp + jf];
+          for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)jf * NEQ + v];
+        }
       }
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) rhs[v] += ff[v];
     }
+    bool bad = false;
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) rhs[v] += ff[v];
+    for (int v = 0; v < NEQ; ++v) bad |= (rhs[v] != rhs[v]);
+    if (bad) atomicMin(err, i);
   }
-  bool bad = false;
+  __syncthreads();
+  if (live) {
+    if constexpr (DO_RK) {
+      const double Vol = M.vol[(cf >> 16) & 0xff];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) bad |= (rhs[v] != rhs[v]);
-  if (bad) atomicMin(err, i);
-  if constexpr (WRITE_RHS) {
+      for (int v = 0; v < NEQ; ++v) {
+        const int o = t * NEQ + v;
+        const double s = shS[o];
+        double sn;
+        if constexpr (FIRST) { sn = s; shN[o] = s; } else { sn = shN[o]; }
+        const double r = rhs[v];
+        double out;
+        switch (irk) {
+          case 1: out = sn + dt * (1.0 / Vol) * r; break;
+          case 2: out = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
+          case 3: out = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
+          case 4: out = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
+          default: out = s; break;
+        }
+        shS[o] = out;
+      }
+    } else if constexpr (WRITE_RHS) {
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) RHS[(size_t)v * M.ns + i] = rhs[v];
+      for (int v = 0; v < NEQ; ++v) shS[t * NEQ + v] = rhs[v];
+    }
   }
+  __syncthreads();
   if constexpr (DO_RK) {
-    const double Vol = M.vol[(cf >> 16) & 0xff];
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      const size_t o = (size_t)v * M.ns + i;
-      double s = S[o];
-      double sn;
-      if constexpr (FIRST) { sn = s; SN[o] = s; } else { sn = SN[o]; }
-      double r = rhs[v];
-      double out;
-      switch (irk) {
-        case 1: out = sn + dt * (1.0 / Vol) * r; break;
-        case 2: out = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
-        case 3: out = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
-        case 4: out = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
-        default: out = s; break;
-      }
-      S[o] = out;
+    for (int k = t; k < nval; k += FV_BLOCK) {
+      S[base + k] = shS[k];
+      if constexpr (FIRST) SN[base + k] = shN[k];
     }
+    if constexpr (WRITE_RHS) {
+      __syncthreads();
+      if (live) {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) shS[t * NEQ + v] = rhs[v];
+      }
+      __syncthreads();
+      for (int k = t; k < nval; k += FV_BLOCK) RHS[base + k] = shS[k];
+    }
+  } else if constexpr (WRITE_RHS) {
+    for (int k = t; k < nval; k += FV_BLOCK) RHS[base + k] = shS[k];
   }
 }
 
@@ -281,7 +405,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_rk(DevMesh M, double *__restrict__
   const double Vol = M.vol[(M.cflag[i] >> 16) & 0xff];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    const size_t o = (size_t)v * M.ns + i;
+    const size_t o = (size_t)i * NEQ + v;
     const double s = S[o];
     switch (irk) {
       case 0: SN[o] = s; break;
@@ -341,7 +465,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_totals_part(DevMesh M, const doubl
   for (int i = i0 + t; i < i1; i += FV_BLOCK) {
     const double V = M.vol[(M.cflag[i] >> 16) & 0xff];
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) acc[v] += S[(size_t)v * M.ns + i] * V;
+    for (int v = 0; v < NEQ; ++v) acc[v] += S[(size_t)i * NEQ + v] * V;
     acc[NEQ] += V;
   }
 #pragma unroll
@@ -376,10 +500,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_check(DevMesh M, const double *__r
                                                     int *__restrict__ err) {
   const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
-  bool bad = !(S[i] > 0.0);
+  bool bad = !(S[(size_t)i * NEQ] > 0.0);
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    double s = S[(size_t)v * M.ns + i], r = RHS[(size_t)v * M.ns + i];
+    double s = S[(size_t)i * NEQ + v], r = RHS[(size_t)i * NEQ + v];
     bad |= !(s - s == 0.0) || !(r - r == 0.0);
   }
   if (bad) atomicMin(err, i);
@@ -558,7 +682,7 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
   HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
   HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->tab, (size_t)25 * nl * 8));
+  HIPCHK(c, hipMalloc(&c->tab, (size_t)34 * nl * 8));
   HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
   HIPCHK(c, hipMalloc(&c->SN, (size_t)neq * ns * 8));
   HIPCHK(c, hipMalloc(&c->RHS, (size_t)neq * ns * 8));
@@ -578,11 +702,16 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   HIPCHK(c, hipMemcpyAsync(c->fL, m->face_L, (size_t)F * 4, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->fR, m->face_R, (size_t)F * 4, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->fmeta, m->face_meta, (size_t)F * 4, hipMemcpyHostToDevice, st));
-  std::vector<double> tab((size_t)25 * nl, 0.0);
+  std::vector<double> tab((size_t)34 * nl, 0.0);
   memcpy(tab.data(), m->h, (size_t)3 * nl * 8);
   memcpy(tab.data() + 3 * nl, m->dist, (size_t)18 * nl * 8);
   memcpy(tab.data() + 21 * nl, m->area, (size_t)3 * nl * 8);
   memcpy(tab.data() + 24 * nl, m->vol, (size_t)nl * 8);
+  /* reciprocals of the d_cc factors (a zero distance gives inf, as 1/0 does) */
+  for (int k = 0; k < 9 * nl; ++k) {
+    const double dcc = m->dist[2 * k + 1];
+    tab[25 * nl + k] = dcc != 0.0 ? 1.0 / dcc : HUGE_VAL;
+  }
   HIPCHK(c, hipMemcpyAsync(c->tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st));
   /* zero the padding and every state array so no kernel reads uninitialised memory */
   HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
@@ -614,6 +743,12 @@ static size_t field_rows(const fvg_ctx *c, int which, int *len, void **dptr, boo
   }
 }
 
+/* S, SN, RHS and the face flux are stored cell- (face-) major on the device */
+static bool cell_major(int which) {
+  return which == FVG_FIELD_S || which == FVG_FIELD_SN || which == FVG_FIELD_RHS ||
+         which == FVG_FIELD_FACEFLUX;
+}
+
 static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
   if (!c || !host) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
@@ -625,6 +760,28 @@ static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
   const int n = which == FVG_FIELD_FACEFLUX ? c->F : c->N;
   (void)hipSetDevice(c->device);
   const size_t es = isR ? c->rsize : 8;
+  if (cell_major(which)) {
+    /* host var-major [rows][n] <-> device [n][rows] */
+    std::vector<char> tmp(rows * (size_t)n * es);
+    if (to_dev) {
+      for (size_t r = 0; r < rows; ++r)
+        for (int i = 0; i < n; ++i) {
+          const double x = host[r * (size_t)n + i];
+          if (es == 8) ((double *)tmp.data())[(size_t)i * rows + r] = x;
+          else ((float *)tmp.data())[(size_t)i * rows + r] = (float)x;
+        }
+      HIPCHK(c, hipMemcpyAsync(d, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+      HIPCHK(c, hipMemcpyAsync(tmp.data(), d, tmp.size(), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t r = 0; r < rows; ++r)
+        for (int i = 0; i < n; ++i)
+          host[r * (size_t)n + i] = es == 8 ? ((double *)tmp.data())[(size_t)i * rows + r]
+                                            : (double)((float *)tmp.data())[(size_t)i * rows + r];
+    }
+    return FVG_OK;
+  }
   std::vector<char> tmp((size_t)n * es);
   for (size_t r = 0; r < rows; ++r) {
     char *dp = (char *)d + r * (size_t)len * es;
@@ -642,6 +799,17 @@ static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
     }
   }
   return FVG_OK;
+}
+
+/* var-major [neq][N] <-> cell-major [N][neq] on the device */
+__global__ void k_transpose_state(const double *__restrict__ src, double *__restrict__ dst, int N,
+                                  int neq, int to_cell_major) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int v = 0; v < neq; ++v) {
+    if (to_cell_major) dst[(size_t)i * neq + v] = src[(size_t)v * N + i];
+    else dst[(size_t)v * N + i] = src[(size_t)i * neq + v];
+  }
 }
 
 /* ------------------------------------------------------------------ launches */
@@ -687,6 +855,26 @@ static void launch_face_m(fvg_ctx *c) {
     hipLaunchKernelGGL((k_face<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
                        n1, n2, c->S, (const float *)c->SF, (const float *)c->SGm, (float *)c->G,
                        c->err);
+}
+
+template <int MODEL, bool NS>
+static void launch_face_fused_m(fvg_ctx *c) {
+  DevMesh M = dev_mesh(c);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
+  dim3 g(b0 + b1 + b2), b(FV_BLOCK);
+  if (c->rsize == 8)
+    hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0,
+                       b1, n0, n1, n2, c->S, (const double *)c->SGm, (double *)c->G, c->err);
+  else
+    hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1,
+                       n0, n1, n2, c->S, (const float *)c->SGm, (float *)c->G, c->err);
+}
+
+template <int MODEL>
+static void launch_face_fused(fvg_ctx *c) {
+  if (c->prm.ns) launch_face_fused_m<MODEL, true>(c);
+  else launch_face_fused_m<MODEL, false>(c);
 }
 
 template <int MODEL>
@@ -741,6 +929,13 @@ static void dispatch_grad(fvg_ctx *c, bool full) {
     default: launch_grad<3>(c, full); break;
   }
 }
+static void dispatch_face_fused(fvg_ctx *c) {
+  switch (c->prm.model) {
+    case 1: launch_face_fused<1>(c); break;
+    case 2: launch_face_fused<2>(c); break;
+    default: launch_face_fused<3>(c); break;
+  }
+}
 static void dispatch_face(fvg_ctx *c) {
   switch (c->prm.model) {
     case 1: launch_face<1>(c); break;
@@ -778,10 +973,11 @@ static void stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs, hi
   auto mark = [&](int k) {
     if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
   };
-  if (c->prm.ischeme == 2) { mark(0); dispatch_recon(c); }
   if (c->prm.ns) { mark(1); dispatch_grad(c, false); }
   mark(2);
-  dispatch_face(c);
+  /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
+  if (c->prm.ischeme == 2) dispatch_face_fused(c);
+  else dispatch_face(c);
   mark(3);
   launch_gather(c, write_rhs, true, first, irk, dt);
   mark(-1);
@@ -826,14 +1022,14 @@ static int state_device_copy(fvg_ctx *c, void *d, bool to_ctx) {
   if (!c || !d) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
   (void)hipSetDevice(c->device);
-  for (int v = 0; v < c->neq; ++v) {
-    double *ctxp = c->S + (size_t)v * c->ns;
-    double *ext = (double *)d + (size_t)v * c->N;
-    if (to_ctx)
-      HIPCHK(c, hipMemcpyAsync(ctxp, ext, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
-    else
-      HIPCHK(c, hipMemcpyAsync(ext, ctxp, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
-  }
+  if (to_ctx)
+    hipLaunchKernelGGL(k_transpose_state, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
+                       (const double *)d, c->S, c->N, c->neq, 1);
+  else
+    hipLaunchKernelGGL(k_transpose_state, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
+                       (const double *)c->S, (double *)d, c->N, c->neq, 0);
+  int rc = finish(c, "state copy");
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVG_OK;
 }
