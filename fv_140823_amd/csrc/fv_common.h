@@ -107,20 +107,21 @@ __device__ __forceinline__ void face_dist(const DevMesh &M, uint32_t cf, int f, 
  * faces given as reciprocals (ip = 1/d_cc(f), im = 1/d_cc(fo)); a boundary's
  * 1/0 = inf keeps the reference's 0/0 -> NaN -> no slope and inf -> the finite
  * slope.  lim 1 = smooth van Albada (extension). */
-__device__ __forceinline__ double limited(int lim, double eps, bool bnd, double d0, double d1,
-                                          double ip, double im) {
-  double grad = 0.0;
+template <typename T>
+__device__ __forceinline__ T limited(int lim, double eps, bool bnd, T d0, T d1, T ip, T im) {
+  T grad = (T)0;
   if (bnd) return grad;
   if (lim == FVG_MINMOD) {
-    if (fabs(d0) > 1e-10) {
-      double a = -d1 * im, b = d0 * ip;
-      double prod = a * b;
-      if (prod > 0.0) grad = fabs(a) > fabs(b) ? b : a;
+    if (fabs(d0) > (T)1e-10) {
+      T a = -d1 * im, b = d0 * ip;
+      T prod = a * b;
+      if (prod > (T)0) grad = fabs(a) > fabs(b) ? b : a;
     }
   } else {
-    double a = isinf(im) ? 0.0 : -d1 * im;
-    double b = isinf(ip) ? 0.0 : d0 * ip;
-    grad = (a * (b * b + eps) + b * (a * a + eps)) / (a * a + b * b + 2.0 * eps);
+    const T e = (T)eps;
+    T a = isinf(im) ? (T)0 : -d1 * im;
+    T b = isinf(ip) ? (T)0 : d0 * ip;
+    grad = (a * (b * b + e) + b * (a * a + e)) / (a * a + b * b + (T)2 * e);
   }
   return grad;
 }

@@ -51,8 +51,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
     for (int v = 0; v < NEQ; ++v) {
       double dP = Up[v] - U[v];
       double dM = Um[v] - U[v];
-      double gp = limited(P.limiter, P.va_eps, bp, dP, dM, pic, mic);
-      double gm = limited(P.limiter, P.va_eps, bm, dM, dP, mic, pic);
+      double gp = limited<double>(P.limiter, P.va_eps, bp, dP, dM, pic, mic);
+      double gm = limited<double>(P.limiter, P.va_eps, bm, dM, dP, mic, pic);
       SF[((size_t)fp * NEQ + v) * M.ns + i] = (R)(U[v] + pcf * gp);
       SF[((size_t)fm * NEQ + v) * M.ns + i] = (R)(U[v] + mcf * gm);
     }
@@ -178,9 +178,12 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
   if (!ok) atomicMin(err, iL);
 }
 
-/* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face): the same
- * operations as k_recon, rounded to the storage type R of the unfused path */
-template <int NEQ, typename R>
+/* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face).  The
+ * limited slope is computed in T (FP32 in the mixed form, from FP64 deltas) and the
+ * face state is the FP64 cell state plus that increment, so the FP64 equation of
+ * state sees an unrounded density (the barotropic EOS turns a density error into
+ * a pressure error c^2 times larger). */
+template <int NEQ, typename T>
 __device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
                                            const double *__restrict__ S, int c, int f,
                                            double *out) {
@@ -196,8 +199,9 @@ __device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
   const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    const double g = limited(P.limiter, P.va_eps, bnd, U0[v] - U[v], U1[v] - U[v], pic, mic);
-    out[v] = (double)(R)(U[v] + pcf * g);
+    const T g = limited<T>(P.limiter, P.va_eps, bnd, (T)(U0[v] - U[v]), (T)(U1[v] - U[v]), (T)pic,
+                           (T)mic);
+    out[v] = U[v] + (double)((T)pcf * g);
   }
 }
 
@@ -214,9 +218,9 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
   const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
   const T s = (T)c_face_sign[fl];
   double UL[NEQ], UR[NEQ];
-  recon_face<NEQ, R>(M, P, S, iL, fl, UL);
+  recon_face<NEQ, T>(M, P, S, iL, fl, UL);
   if (bc == 0) {
-    recon_face<NEQ, R>(M, P, S, iR, fr, UR);
+    recon_face<NEQ, T>(M, P, S, iR, fr, UR);
   } else if (bc == 1) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
