@@ -129,6 +129,7 @@ _DECLS = [
     ("fvg_time_steps", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp]),
     ("fvg_adj_set_probe", C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
+    ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),
     ("fvg_tlm", C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp, _dp]),
     ("fvg_face_flux_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp]),

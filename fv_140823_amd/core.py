@@ -315,6 +315,12 @@ class Solver:
                  "fvg_adj_forward")
         return J.value
 
+    def adj_objective(self):
+        """(hi, lo) of the last rollout's objective, J = hi + lo unevaluated."""
+        out = np.empty(2, np.float64)
+        self._ck(lib().fvg_adj_objective(self._h, _dptr(out)), "fvg_adj_objective")
+        return float(out[0]), float(out[1])
+
     def adj_backward(self) -> np.ndarray:
         g = np.empty((self.neq, self.ncell), np.float64)
         self._ck(lib().fvg_adj_backward(self._h, _dptr(g), None), "fvg_adj_backward")

@@ -137,6 +137,33 @@ __device__ __forceinline__ int logical_block() {
 
 /* ------------------------------------------------------------------ context */
 
+/* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face).  The
+ * limited slope is computed in T (FP32 in the mixed form, from FP64 deltas) and the
+ * face state is the FP64 cell state plus that increment, so the FP64 equation of
+ * state sees an unrounded density (the barotropic EOS turns a density error into
+ * a pressure error c^2 times larger). */
+template <int NEQ, typename T>
+__device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
+                                           const double *__restrict__ S, int c, int f,
+                                           double *out) {
+  const uint32_t cf = M.cflag[c];
+  const int fo = f ^ (f < 4 ? 2 : 1); /* fcopp: 0<->2, 1<->3, 4<->5 */
+  double U[NEQ], U0[NEQ], U1[NEQ];
+  load_S<NEQ>(S, M.ns, c, U);
+  nb_state<NEQ>(M, P, S, c, cf, f, U, U0);
+  nb_state<NEQ>(M, P, S, c, cf, fo, U, U1);
+  double pcf, pcc, mcf, mcc, pic, mic;
+  face_dist(M, cf, f, pcf, pcc, pic);
+  face_dist(M, cf, fo, mcf, mcc, mic);
+  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const T g = limited<T>(P.limiter, P.va_eps, bnd, (T)(U0[v] - U[v]), (T)(U1[v] - U[v]), (T)pic,
+                           (T)mic);
+    out[v] = U[v] + (double)((T)pcf * g);
+  }
+}
+
 struct fvg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -168,8 +195,10 @@ struct fvg_ctx {
 /* releases ctx->adj; defined in fvadj.hip */
 void fv_adj_release(fvg_ctx *c);
 /* forward pieces shared with fvadj.hip (defined in fvgpu.hip) */
-void fv_launch_recon(fvg_ctx *c);
+void fv_launch_recon(fvg_ctx *c, const double *S);
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs);
+/* one stage Sout = RK(irk; SN, S, R(S)) without touching c->S / c->SN */
+void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int irk, double dt);
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
                            bool first);
 

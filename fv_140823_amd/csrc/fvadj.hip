@@ -7,16 +7,22 @@
  *     S_k = a SN + b S_{k-1} + c dt/Vol R(S_{k-1})
  * lambda_SN += a lambda, lambda_{k-1} = b lambda + R'(S_{k-1})^T (c dt/Vol lambda),
  * with R'^T = recon^T . face^T . gather^T, each a deterministic gather:
- *   k_adj_face   per face: mu_L - mu_R -> hand-written VJP of HLLC + EOS
- *   k_adj_recon  per cell: face-state bars by gather slot (f = 0..5, k = 0..3),
- *                limiter partials -> own bar and the bars sent to the +/- neighbours
- *   k_adj_recv   per cell: own bar + what its neighbours sent (no atomics)
- * Stage states are recomputed from checkpoints every `ckpt` steps.
+ *   k_adj_face        per face: MUSCL face states (the forward's own
+ *                     reconstruction), mu = c dt lam / Vol on both sides,
+ *                     mu_L - mu_R -> hand-written VJP of HLLC + EOS
+ *   k_adj_recon       per cell: face-state bars by gather slot (f = 0..5, k = 0..3),
+ *                     limiter partials -> own bar and the bars sent to the +/- neighbours
+ *   k_adj_recv_stage  per cell: own bar + what its neighbours sent (no atomics),
+ *                     then the stage's reverse update of lam and lamSN
+ * Stage states are recomputed from checkpoints every `ckpt` steps, written
+ * straight into a segment store that the reverse sweep reads in place.
  * Tangent (forward, for the dot-product check): the same stage kernels on
  * forward-mode dual numbers for the face flux and analytic limiter partials.
  * Scope: model 1 (ideal/stiffened gas), inviscid, MUSCL (either limiter) or
  * first order; bc outflow/periodic (an inlet ghost is a constant).
  */
+#include <cmath>
+
 #include "fv_common.h"
 
 struct fv_adj_state {
@@ -24,12 +30,14 @@ struct fv_adj_state {
   double dt = 0.0;
   bool fwd_done = false;
   size_t plane = 0; /* neq * ns doubles */
+  int seg_valid = -1; /* segment whose stage states the store holds, or -1 */
   uint8_t *mask = nullptr;
-  double *ckpts = nullptr, *seg = nullptr, *Sfinal = nullptr;
-  double *lam = nullptr, *lamSN = nullptr, *mu = nullptr, *nu = nullptr, *self = nullptr;
+  double *ckpts = nullptr, *seg = nullptr;
+  double *lam = nullptr, *lamSN = nullptr, *self = nullptr;
   double *Pp = nullptr, *Pm = nullptr;
   void *lamUL = nullptr, *lamUR = nullptr;
-  double *Jsteps = nullptr, *Jdsteps = nullptr, *part = nullptr;
+  double *Jsteps = nullptr, *Jdsteps = nullptr, *part = nullptr; /* (hi, lo) pairs */
+  double J_hi = 0.0, J_lo = 0.0; /* objective of the last rollout, unevaluated sum */
   int part_cap = 0;
   double *Sd = nullptr, *SNd = nullptr, *SFd = nullptr, *Gd = nullptr;
 };
@@ -37,9 +45,8 @@ struct fv_adj_state {
 void fv_adj_release(fvg_ctx *c) {
   if (!c || !c->adj) return;
   fv_adj_state *a = c->adj;
-  void *ptrs[] = {a->mask, a->ckpts, a->seg, a->Sfinal, a->lam, a->lamSN, a->mu, a->nu, a->self,
-                  a->Pp, a->Pm, a->lamUL, a->lamUR, a->Jsteps, a->Jdsteps, a->part, a->Sd, a->SNd,
-                  a->SFd, a->Gd};
+  void *ptrs[] = {a->mask, a->ckpts, a->seg, a->lam, a->lamSN, a->self, a->Pp, a->Pm, a->lamUL,
+                  a->lamUR, a->Jsteps, a->Jdsteps, a->part, a->Sd, a->SNd, a->SFd, a->Gd};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete a;
@@ -52,31 +59,37 @@ __device__ __forceinline__ double cell_vol(const DevMesh &M, int i) {
   return M.vol[(M.cflag[i] >> 16) & 0xff];
 }
 
-/* face^T: lamUL/lamUR[v][j] = d(sum_i mu_i . R_i)/d(face states of face j) */
+/* face^T: lamUL/lamUR[j][v] = d(sum_i mu_i . R_i)/d(face states of face j), with
+ * mu_i = cdt lam_i / Vol_i.  The face states are rebuilt exactly as the forward
+ * face kernel builds them (recon_face), so the VJP is taken at the forward's
+ * own face states in either precision. */
 template <int a, typename T, typename R>
 __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams &P, int j,
                                               const double *__restrict__ S,
-                                              const R *__restrict__ SF,
-                                              const double *__restrict__ mu, R *__restrict__ lUL,
-                                              R *__restrict__ lUR) {
+                                              const double *__restrict__ lam, double cdt,
+                                              R *__restrict__ lUL, R *__restrict__ lUR) {
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
   const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
   const T s = (T)c_face_sign[fl];
   const bool first = P.ischeme == 1;
+  double XL[5], XR[5];
+  if (first) load_S<5>(S, M.ns, iL, XL);
+  else recon_face<5, T>(M, P, S, iL, fl, XL);
+  if (bc == 0) {
+    if (first) load_S<5>(S, M.ns, iR, XR);
+    else recon_face<5, T>(M, P, S, iR, fr, XR);
+  }
+  const double VL = cell_vol(M, iL), VR = cell_vol(M, iR);
   T UL[5], UR[5], ob[5], ULb[5], URb[5];
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    UL[v] = first ? (T)S[(size_t)iL * 5 + v] : (T)SF[((size_t)fl * 5 + v) * M.ns + iL];
-    if (bc == 0)
-      UR[v] = first ? (T)S[(size_t)iR * 5 + v] : (T)SF[((size_t)fr * 5 + v) * M.ns + iR];
-    else if (bc == 1)
-      UR[v] = (T)P.inlet[v];
-    else
-      UR[v] = UL[v];
-    const double mg = mu[(size_t)v * M.ns + iL] - (bc == 0 ? mu[(size_t)v * M.ns + iR] : 0.0);
-    ob[v] = -A * (T)mg;
+    UL[v] = (T)XL[v];
+    UR[v] = bc == 0 ? (T)XR[v] : (bc == 1 ? (T)P.inlet[v] : UL[v]);
+    const double muL = cdt * lam[(size_t)v * M.ns + iL] / VL;
+    const double muR = bc == 0 ? cdt * lam[(size_t)v * M.ns + iR] / VR : 0.0;
+    ob[v] = -A * (T)(muL - muR);
     ULb[v] = URb[v] = (T)0;
   }
   hllc_vjp_ideal<a, T>(P, UL, UR, s, ob, ULb, URb);
@@ -84,29 +97,28 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
   for (int v = 0; v < 5; ++v) {
     if (bc == 2) { ULb[v] += URb[v]; URb[v] = (T)0; }
     if (bc == 1) URb[v] = (T)0;
-    lUL[(size_t)v * M.nfp + j] = (R)ULb[v];
-    lUR[(size_t)v * M.nfp + j] = (R)URb[v];
+    lUL[(size_t)j * 5 + v] = (R)ULb[v];
+    lUR[(size_t)j * 5 + v] = (R)URb[v];
   }
 }
 
 template <typename T, typename R>
-__global__ __launch_bounds__(FV_BLOCK) void k_adj_face(DevMesh M, DevParams P, int b0, int b1,
+__global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_face(DevMesh M, DevParams P, int b0, int b1,
                                                        int n0, int n1, int n2,
                                                        const double *__restrict__ S,
-                                                       const R *__restrict__ SF,
-                                                       const double *__restrict__ mu,
+                                                       const double *__restrict__ lam, double cdt,
                                                        R *__restrict__ lUL, R *__restrict__ lUR) {
   const int lb = logical_block();
   const int t = threadIdx.x;
   if (lb < b0) {
     const int j = lb * FV_BLOCK + t;
-    if (j < n0) adj_face_body<0, T, R>(M, P, j, S, SF, mu, lUL, lUR);
+    if (j < n0) adj_face_body<0, T, R>(M, P, j, S, lam, cdt, lUL, lUR);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) adj_face_body<1, T, R>(M, P, n0 + q, S, SF, mu, lUL, lUR);
+    if (q < n1) adj_face_body<1, T, R>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) adj_face_body<2, T, R>(M, P, n0 + n1 + q, S, SF, mu, lUL, lUR);
+    if (q < n2) adj_face_body<2, T, R>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
   }
 }
 
@@ -125,13 +137,13 @@ __device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ l
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int v = 0; v < 5; ++v) out[v] += (double)lUL[(size_t)v * M.nfp + jf + k];
+      for (int v = 0; v < 5; ++v) out[v] += (double)lUL[(size_t)(jf + k) * 5 + v];
   } else if (g & 1) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = (double)lUR[(size_t)v * M.nfp + jf];
+    for (int v = 0; v < 5; ++v) out[v] = (double)lUR[(size_t)jf * 5 + v];
   } else {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = (double)lUL[(size_t)v * M.nfp + jf];
+    for (int v = 0; v < 5; ++v) out[v] = (double)lUL[(size_t)jf * 5 + v];
   }
 }
 
@@ -170,7 +182,9 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
   }
   double U[5];
   load_S<5>(S, M.ns, i, U);
-#pragma unroll
+  /* one axis at a time (not unrolled): the three axes' 20 live values each would
+   * otherwise be interleaved and cost the kernel most of its occupancy */
+#pragma unroll 1
   for (int ax = 0; ax < 3; ++ax) {
     const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
     const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
@@ -207,12 +221,19 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
   for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
 }
 
-/* recon^T, part 2: nu_i = own bar + bars sent to i by its face neighbours.
- * A neighbour whose delta used the mean of 4 finer cells sends 1/4 to each. */
-__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv(DevMesh M, const double *__restrict__ self,
-                                                       const double *__restrict__ Pp,
-                                                       const double *__restrict__ Pm,
-                                                       double *__restrict__ nu) {
+/* recon^T, part 2: nu_i = own bar + bars sent to i by its face neighbours (a
+ * neighbour whose delta used the mean of 4 finer cells sends 1/4 to each), then
+ * the stage's reverse update:
+ *   mode 0 (last stage):  lamSN  = a lam;  lam = b lam + nu
+ *   mode 1:               lamSN += a lam;  lam = b lam + nu
+ *   mode 2 (first stage): lam = lamSN + a lam + nu */
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M,
+                                                             const double *__restrict__ self,
+                                                             const double *__restrict__ Pp,
+                                                             const double *__restrict__ Pm,
+                                                             double *__restrict__ lam,
+                                                             double *__restrict__ lamSN, double a,
+                                                             double b, int mode) {
   const int i = logical_block() * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
   const uint32_t cf = M.cflag[i];
@@ -248,41 +269,36 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv(DevMesh M, const double *
     }
   }
 #pragma unroll
-  for (int v = 0; v < 5; ++v) nu[(size_t)v * M.ns + i] = acc[v];
-}
-
-/* mu = c dt / Vol * lam */
-__global__ void k_adj_mu(DevMesh M, const double *__restrict__ lam, double cdt,
-                         double *__restrict__ mu) {
-  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
-  if (i >= M.N) return;
-  const double V = cell_vol(M, i);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) mu[(size_t)v * M.ns + i] = cdt * lam[(size_t)v * M.ns + i] / V;
-}
-
-/* lamSN (=|+=) a lam;  lam = b lam + nu  (or, on the first stage, lam = lamSN + nu) */
-__global__ void k_adj_stage(DevMesh M, double *__restrict__ lam, double *__restrict__ lamSN,
-                            const double *__restrict__ nu, double a, double b, int mode) {
-  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
-  if (i >= M.N) return;
-#pragma unroll
   for (int v = 0; v < 5; ++v) {
     const size_t o = (size_t)v * M.ns + i;
     const double l = lam[o];
-    if (mode == 0) { lamSN[o] = a * l; lam[o] = b * l + nu[o]; }
-    else if (mode == 1) { lamSN[o] += a * l; lam[o] = b * l + nu[o]; }
-    else { lam[o] = lamSN[o] + a * l + nu[o]; }
+    if (mode == 0) { lamSN[o] = a * l; lam[o] = b * l + acc[v]; }
+    else if (mode == 1) { lamSN[o] += a * l; lam[o] = b * l + acc[v]; }
+    else { lam[o] = lamSN[o] + a * l + acc[v]; }
   }
 }
 
-/* objective: part[b] = sum over probe cells of block b of p Vol (fixed order) */
+/* double-double accumulation (Knuth's TwoSum): (ah, al) += (bh, bl).  The
+ * objective is summed this way so that J(S0 + eps v) - J(S0 - eps v) is not
+ * swamped by the rounding of J itself (about 1e-16 |J|, which a central
+ * difference with eps = 1e-6 turns into 1e-10 |J| of noise). */
+__host__ __device__ __forceinline__ void dd_add(double &ah, double &al, double bh, double bl) {
+  const double s = ah + bh;
+  const double bb = s - ah;
+  double e = (ah - (s - bb)) + (bh - bb);
+  e += al + bl;
+  ah = s + e;
+  al = e - (ah - s);
+}
+
+/* objective: part[4b..4b+3] = (J hi, J lo, dJ hi, dJ lo) over the probe cells of
+ * block b (fixed pairwise order) */
 __global__ __launch_bounds__(FV_BLOCK) void k_obj(DevMesh M, DevParams P,
                                                   const double *__restrict__ S,
                                                   const double *__restrict__ Sd,
                                                   const uint8_t *__restrict__ mask,
                                                   double *__restrict__ part) {
-  __shared__ double sh[2][FV_BLOCK];
+  __shared__ double sh[4][FV_BLOCK];
   const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
   double pv = 0.0, pd = 0.0;
   if (i < M.N && mask[i]) {
@@ -299,29 +315,53 @@ __global__ __launch_bounds__(FV_BLOCK) void k_obj(DevMesh M, DevParams P,
       pd = V * g1 * (0.5 * (u * u + v * v + w * w) * D[0] - u * D[1] - v * D[2] - w * D[3] + D[4]);
     }
   }
-  sh[0][threadIdx.x] = pv;
-  sh[1][threadIdx.x] = pd;
+  const int t = threadIdx.x;
+  sh[0][t] = pv;
+  sh[1][t] = 0.0;
+  sh[2][t] = pd;
+  sh[3][t] = 0.0;
   __syncthreads();
   for (int w = FV_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    if (t < w) {
+      dd_add(sh[0][t], sh[1][t], sh[0][t + w], sh[1][t + w]);
+      dd_add(sh[2][t], sh[3][t], sh[2][t + w], sh[3][t + w]);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sh[0][0];
-    part[2 * blockIdx.x + 1] = sh[1][0];
-  }
+  if (t == 0)
+    for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = sh[k][0];
 }
 
-__global__ void k_obj_final(const double *__restrict__ part, int nb, double *__restrict__ J,
-                            double *__restrict__ Jd) {
-  if (threadIdx.x || blockIdx.x) return;
-  double a = 0.0, b = 0.0;
-  for (int k = 0; k < nb; ++k) { a += part[2 * k]; b += part[2 * k + 1]; }
-  *J = a;
-  if (Jd) *Jd = b;
+/* J = sum of the block partials: one block, thread t sums partials t, t+256, ...
+ * in order, then a fixed pairwise tree (the same order on every run); J and Jd
+ * receive (hi, lo) */
+__global__ __launch_bounds__(FV_BLOCK) void k_obj_final(const double *__restrict__ part, int nb,
+                                                        double *__restrict__ J,
+                                                        double *__restrict__ Jd) {
+  __shared__ double sh[4][FV_BLOCK];
+  const int t = threadIdx.x;
+  double a = 0.0, al = 0.0, b = 0.0, bl = 0.0;
+  for (int k = t; k < nb; k += FV_BLOCK) {
+    dd_add(a, al, part[4 * k], part[4 * k + 1]);
+    dd_add(b, bl, part[4 * k + 2], part[4 * k + 3]);
+  }
+  sh[0][t] = a;
+  sh[1][t] = al;
+  sh[2][t] = b;
+  sh[3][t] = bl;
+  __syncthreads();
+  for (int w = FV_BLOCK / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      dd_add(sh[0][t], sh[1][t], sh[0][t + w], sh[1][t + w]);
+      dd_add(sh[2][t], sh[3][t], sh[2][t + w], sh[3][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    J[0] = sh[0][0];
+    J[1] = sh[1][0];
+    if (Jd) { Jd[0] = sh[2][0]; Jd[1] = sh[3][0]; }
+  }
 }
 
 /* lam += dt Vol dp/dS on the probe */
@@ -457,56 +497,86 @@ static int adj_alloc(fvg_ctx *c) {
   const size_t pl = a->plane * 8;
   HIPCHK(c, hipMalloc(&a->mask, (size_t)c->ns));
   HIPCHK(c, hipMemsetAsync(a->mask, 0, (size_t)c->ns, c->stream));
-  HIPCHK(c, hipMalloc(&a->Sfinal, pl));
   HIPCHK(c, hipMalloc(&a->lam, pl));
   HIPCHK(c, hipMalloc(&a->lamSN, pl));
-  HIPCHK(c, hipMalloc(&a->mu, pl));
-  HIPCHK(c, hipMalloc(&a->nu, pl));
   HIPCHK(c, hipMalloc(&a->self, pl));
   HIPCHK(c, hipMalloc(&a->Pp, 3 * pl));
   HIPCHK(c, hipMalloc(&a->Pm, 3 * pl));
   HIPCHK(c, hipMalloc(&a->lamUL, (size_t)5 * c->nfp * c->rsize));
   HIPCHK(c, hipMalloc(&a->lamUR, (size_t)5 * c->nfp * c->rsize));
   a->part_cap = nblk(c->N);
-  HIPCHK(c, hipMalloc(&a->part, (size_t)2 * a->part_cap * 8));
+  HIPCHK(c, hipMalloc(&a->part, (size_t)4 * a->part_cap * 8));
   return FVG_OK;
 }
 
-/* J and (optionally) dJ contributions of the current S / Sd into slot n */
-static void obj_step(fvg_ctx *c, const double *Sd, int n) {
+/* J and (optionally) dJ contributions of the state S / tangent Sd into slot n */
+static void obj_step(fvg_ctx *c, const double *S, const double *Sd, int n) {
   fv_adj_state *a = c->adj;
   DevMesh M = dev_mesh(c);
   const int nb = nblk(c->N);
-  hipLaunchKernelGGL(k_obj, dim3(nb), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, Sd, a->mask,
+  hipLaunchKernelGGL(k_obj, dim3(nb), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S, Sd, a->mask,
                      a->part);
-  hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(64), 0, c->stream, a->part, nb, a->Jsteps + n,
-                     Sd ? a->Jdsteps + n : (double *)nullptr);
+  hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(FV_BLOCK), 0, c->stream, a->part, nb,
+                     a->Jsteps + 2 * n, Sd ? a->Jdsteps + 2 * n : (double *)nullptr);
 }
 
-/* nu = R'(S)^T mu for the current S (reconstructs SF of S first) */
-static void residual_vjp(fvg_ctx *c) {
+/* the reverse of one stage taken at the stage's input state S:
+ * nu = R'(S)^T (cdt lam / Vol), then lam / lamSN updated per `mode` */
+static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double cb, int mode) {
   fv_adj_state *a = c->adj;
   DevMesh M = dev_mesh(c);
-  if (c->prm.ischeme == 2) fv_launch_recon(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblk(n0), b1 = nblk(n1), b2 = nblk(n2);
-  if (c->rsize == 8)
-    hipLaunchKernelGGL((k_adj_face<double, double>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, c->S, (const double *)c->SF, a->mu,
-                       (double *)a->lamUL, (double *)a->lamUR);
-  else
-    hipLaunchKernelGGL((k_adj_face<float, float>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, c->S, (const float *)c->SF, a->mu,
-                       (float *)a->lamUL, (float *)a->lamUR);
   const int nc = nblk(c->N);
-  if (c->rsize == 8)
-    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S,
+  if (c->rsize == 8) {
+    hipLaunchKernelGGL((k_adj_face<double, double>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
+                       M, c->dp, b0, b1, n0, n1, n2, S, a->lam, cdt, (double *)a->lamUL,
+                       (double *)a->lamUR);
+    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
                        (const double *)a->lamUL, (const double *)a->lamUR, a->self, a->Pp, a->Pm);
-  else
-    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S,
+  } else {
+    hipLaunchKernelGGL((k_adj_face<float, float>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
+                       M, c->dp, b0, b1, n0, n1, n2, S, a->lam, cdt, (float *)a->lamUL,
+                       (float *)a->lamUR);
+    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
                        (const float *)a->lamUL, (const float *)a->lamUR, a->self, a->Pp, a->Pm);
-  hipLaunchKernelGGL(k_adj_recv, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->self, a->Pp, a->Pm,
-                     a->nu);
+  }
+  hipLaunchKernelGGL(k_adj_recv_stage, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->self, a->Pp,
+                     a->Pm, a->lam, a->lamSN, ca, cb, mode);
+}
+
+/* Segment store: 3 ckpt + 1 states.  Step m of a segment has its stage inputs at
+ * seg_state(m, 0..2) (S^n, S1, S2) and S^{n+1} at seg_state(m + 1, 0); the first
+ * state of a segment is its checkpoint, read in place. */
+static double *seg_state(fv_adj_state *a, int sg, int m, int k) {
+  if (m == 0 && k == 0) return a->ckpts + (size_t)sg * a->plane;
+  return a->seg + (size_t)(3 * m + k) * a->plane;
+}
+
+/* the SSP-RK3 steps n0 .. n1-1 of segment sg from its checkpoint into the store */
+static void seg_forward(fvg_ctx *c, int sg, int n0, int n1, bool objective) {
+  fv_adj_state *a = c->adj;
+  for (int n = n0; n < n1; ++n) {
+    const int m = n - n0;
+    double *X0 = seg_state(a, sg, m, 0);
+    fv_stage_ptrs(c, X0, X0, seg_state(a, sg, m, 1), 1, a->dt);
+    fv_stage_ptrs(c, seg_state(a, sg, m, 1), X0, seg_state(a, sg, m, 2), 2, a->dt);
+    fv_stage_ptrs(c, seg_state(a, sg, m, 2), X0, seg_state(a, sg, m + 1, 0), 3, a->dt);
+    if (objective) obj_step(c, seg_state(a, sg, m + 1, 0), nullptr, n);
+  }
+}
+
+/* sum_n dt (hi_n + lo_n) in double-double: dt hi_n exactly as a product and its
+ * fma rounding error */
+static void dt_sum(double dt, const std::vector<double> &hl, int nsteps, double *hi, double *lo) {
+  double sh = 0.0, sl = 0.0;
+  for (int n = 0; n < nsteps; ++n) {
+    const double p = dt * hl[2 * n];
+    const double e = std::fma(dt, hl[2 * n], -p) + dt * hl[2 * n + 1];
+    dd_add(sh, sl, p, e);
+  }
+  *hi = sh;
+  *lo = sl;
 }
 
 extern "C" {
@@ -536,27 +606,37 @@ int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   if (a->seg) (void)hipFree(a->seg);
   if (a->Jsteps) (void)hipFree(a->Jsteps);
   a->ckpts = a->seg = a->Jsteps = nullptr;
+  a->fwd_done = false;
+  a->seg_valid = -1;
   HIPCHK(c, hipMalloc(&a->ckpts, (size_t)nck * pl));
   HIPCHK(c, hipMalloc(&a->seg, (size_t)(3 * ckpt + 1) * pl));
-  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)nsteps * 8));
+  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)2 * nsteps * 8));
   a->nsteps = nsteps;
   a->ckpt = ckpt;
   a->nck = nck;
   a->dt = dt;
-  for (int n = 0; n < nsteps; ++n) {
+  /* every segment but the last runs in place on S, storing only its checkpoint;
+   * the last one runs in the segment store, which the backward sweep then reads
+   * without recomputing it */
+  const int nlast = (nck - 1) * ckpt;
+  for (int n = 0; n < nlast; ++n) {
     if (n % ckpt == 0)
       HIPCHK(c, hipMemcpyAsync(a->ckpts + (size_t)(n / ckpt) * a->plane, c->S, pl,
                                hipMemcpyDeviceToDevice, c->stream));
     for (int k = 1; k <= 3; ++k) fv_stage(c, k, dt, k == 1, false);
-    obj_step(c, nullptr, n);
+    obj_step(c, c->S, nullptr, n);
   }
-  HIPCHK(c, hipMemcpyAsync(a->Sfinal, c->S, pl, hipMemcpyDeviceToDevice, c->stream));
-  std::vector<double> js(nsteps);
-  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(a->ckpts + (size_t)(nck - 1) * a->plane, c->S, pl,
+                           hipMemcpyDeviceToDevice, c->stream));
+  seg_forward(c, nck - 1, nlast, nsteps, true);
+  a->seg_valid = nck - 1;
+  HIPCHK(c, hipMemcpyAsync(c->S, seg_state(a, nck - 1, nsteps - nlast, 0), pl,
+                           hipMemcpyDeviceToDevice, c->stream));
+  std::vector<double> js(2 * nsteps);
+  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, js.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double sum = 0.0;
-  for (int n = 0; n < nsteps; ++n) sum += dt * js[n];
-  *J = sum;
+  dt_sum(dt, js, nsteps, &a->J_hi, &a->J_lo);
+  *J = a->J_hi;
   a->fwd_done = true;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
@@ -573,44 +653,27 @@ int fvg_adj_backward(fvg_ctx *c, double *grad_host, void *grad_dev) {
   const size_t pl = a->plane * 8;
   const int nc = nblk(c->N);
   const double dt = a->dt;
+  /* S_k = a SN + b S_{k-1} + c dt/Vol R(S_{k-1}) for stages k = 1, 2, 3 */
+  const double cst[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  const double ast[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
+  const double bst[3] = {0.0, 1.0 / 4.0, 2.0 / 3.0};
   HIPCHK(c, hipMemsetAsync(a->lam, 0, pl, c->stream));
   for (int sg = a->nck - 1; sg >= 0; --sg) {
     const int n0 = sg * a->ckpt, n1 = std::min(n0 + a->ckpt, a->nsteps);
-    HIPCHK(c, hipMemcpyAsync(c->S, a->ckpts + (size_t)sg * a->plane, pl, hipMemcpyDeviceToDevice,
-                             c->stream));
-    /* recompute the segment, keeping S^n, S1, S2 of every step and S^{n1} */
-    for (int n = n0; n < n1; ++n)
-      for (int k = 1; k <= 3; ++k) {
-        HIPCHK(c, hipMemcpyAsync(a->seg + (size_t)(3 * (n - n0) + k - 1) * a->plane, c->S, pl,
-                                 hipMemcpyDeviceToDevice, c->stream));
-        fv_stage(c, k, dt, k == 1, false);
-      }
-    HIPCHK(c, hipMemcpyAsync(a->seg + (size_t)(3 * (n1 - n0)) * a->plane, c->S, pl,
-                             hipMemcpyDeviceToDevice, c->stream));
+    if (a->seg_valid != sg) {
+      seg_forward(c, sg, n0, n1, false);
+      a->seg_valid = sg;
+    }
     for (int n = n1 - 1; n >= n0; --n) {
+      const int m = n - n0;
       /* d J_{n+1} / d S^{n+1} */
-      HIPCHK(c, hipMemcpyAsync(c->S, a->seg + (size_t)(3 * (n + 1 - n0)) * a->plane, pl,
-                               hipMemcpyDeviceToDevice, c->stream));
-      hipLaunchKernelGGL(k_obj_grad, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, a->mask,
-                         dt, a->lam);
-      /* stage 3: S3 = 1/3 SN + 2/3 S2 + 2/3 dt/V R(S2);  stage 2: S2 = 3/4 SN + 1/4 S1 + ...;
-         stage 1: S1 = SN + dt/V R(SN) */
-      const double cst[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-      const double ast[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0};
-      const double bst[3] = {0.0, 1.0 / 4.0, 2.0 / 3.0};
-      for (int k = 3; k >= 1; --k) {
-        HIPCHK(c, hipMemcpyAsync(c->S, a->seg + (size_t)(3 * (n - n0) + k - 1) * a->plane, pl,
-                                 hipMemcpyDeviceToDevice, c->stream));
-        hipLaunchKernelGGL(k_adj_mu, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->lam,
-                           cst[k - 1] * dt, a->mu);
-        residual_vjp(c);
-        const int mode = k == 3 ? 0 : (k == 2 ? 1 : 2);
-        hipLaunchKernelGGL(k_adj_stage, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->lam, a->lamSN,
-                           a->nu, ast[k - 1], bst[k - 1], mode);
-      }
+      hipLaunchKernelGGL(k_obj_grad, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp,
+                         (const double *)seg_state(a, sg, m + 1, 0), a->mask, dt, a->lam);
+      for (int k = 3; k >= 1; --k)
+        stage_vjp(c, seg_state(a, sg, m, k - 1), cst[k - 1] * dt, ast[k - 1], bst[k - 1],
+                  k == 3 ? 0 : (k == 2 ? 1 : 2));
     }
   }
-  HIPCHK(c, hipMemcpyAsync(c->S, a->Sfinal, pl, hipMemcpyDeviceToDevice, c->stream));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
   for (int v = 0; v < 5; ++v) {
@@ -645,8 +708,8 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
   if (a->Jsteps) (void)hipFree(a->Jsteps);
   if (a->Jdsteps) (void)hipFree(a->Jdsteps);
   a->Jsteps = a->Jdsteps = nullptr;
-  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)nsteps * 8));
-  HIPCHK(c, hipMalloc(&a->Jdsteps, (size_t)nsteps * 8));
+  HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)2 * nsteps * 8));
+  HIPCHK(c, hipMalloc(&a->Jdsteps, (size_t)2 * nsteps * 8));
   HIPCHK(c, hipMemsetAsync(a->Sd, 0, pl, c->stream));
   {
     /* the tangent state is cell-major like S */
@@ -663,7 +726,7 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
   for (int n = 0; n < nsteps; ++n) {
     for (int k = 1; k <= 3; ++k) {
       if (c->prm.ischeme == 2) {
-        fv_launch_recon(c);
+        fv_launch_recon(c, c->S);
         hipLaunchKernelGGL(k_tlm_recon, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, a->Sd,
                            a->SFd);
       }
@@ -673,18 +736,27 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
       fv_launch_gather_ptrs(c, a->Gd, a->Sd, a->SNd, k, dt, k == 1);
       fv_stage(c, k, dt, k == 1, false);
     }
-    obj_step(c, a->Sd, n);
+    obj_step(c, c->S, a->Sd, n);
   }
-  std::vector<double> js(nsteps), jd(nsteps);
-  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(jd.data(), a->Jdsteps, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> js(2 * nsteps), jd(2 * nsteps);
+  HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, js.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(jd.data(), a->Jdsteps, jd.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
-  double s0 = 0.0, s1 = 0.0;
-  for (int n = 0; n < nsteps; ++n) { s0 += dt * js[n]; s1 += dt * jd[n]; }
-  *J = s0;
-  *dJ = s1;
+  double jl, dl;
+  dt_sum(dt, js, nsteps, J, &jl);
+  dt_sum(dt, jd, nsteps, dJ, &dl);
+  a->J_hi = *J;
+  a->J_lo = jl;
+  return FVG_OK;
+}
+
+int fvg_adj_objective(const fvg_ctx *c, double *J_hi_lo) {
+  if (!c || !J_hi_lo) return FVG_ERR_ARG;
+  if (!c->adj || !c->adj->Jsteps) return FVG_ERR_STATE;
+  J_hi_lo[0] = c->adj->J_hi;
+  J_hi_lo[1] = c->adj->J_lo;
   return FVG_OK;
 }
 

@@ -178,33 +178,6 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
   if (!ok) atomicMin(err, iL);
 }
 
-/* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face).  The
- * limited slope is computed in T (FP32 in the mixed form, from FP64 deltas) and the
- * face state is the FP64 cell state plus that increment, so the FP64 equation of
- * state sees an unrounded density (the barotropic EOS turns a density error into
- * a pressure error c^2 times larger). */
-template <int NEQ, typename T>
-__device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
-                                           const double *__restrict__ S, int c, int f,
-                                           double *out) {
-  const uint32_t cf = M.cflag[c];
-  const int fo = f ^ (f < 4 ? 2 : 1); /* fcopp: 0<->2, 1<->3, 4<->5 */
-  double U[NEQ], U0[NEQ], U1[NEQ];
-  load_S<NEQ>(S, M.ns, c, U);
-  nb_state<NEQ>(M, P, S, c, cf, f, U, U0);
-  nb_state<NEQ>(M, P, S, c, cf, fo, U, U1);
-  double pcf, pcc, mcf, mcc, pic, mic;
-  face_dist(M, cf, f, pcf, pcc, pic);
-  face_dist(M, cf, fo, mcf, mcc, mic);
-  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) {
-    const T g = limited<T>(P.limiter, P.va_eps, bnd, (T)(U0[v] - U[v]), (T)(U1[v] - U[v]), (T)pic,
-                           (T)mic);
-    out[v] = U[v] + (double)((T)pcf * g);
-  }
-}
-
 /* face flux with the reconstruction fused in: reads S (the 2-ring through the
  * caches) instead of a stored SF, same results as recon + face */
 template <int MODEL, bool NS, int a, typename T, typename R>
@@ -293,13 +266,17 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
  * the RK stage (RK.c:24-47).  irk 1..4; FIRST: also SN = S (RK(0)).
  * A block owns 256 consecutive cells, whose cell-major S / SN / RHS rows form one
  * contiguous span: it is moved between HBM and LDS with fully coalesced accesses
- * and each thread works on its own row in LDS. */
+ * and each thread works on its own row in LDS.  The stage reads S and writes Sout,
+ * which is S itself in the time loop and another buffer when the adjoint
+ * recomputes stage states into its segment store (each block reads its whole span
+ * before any thread writes, so Sout == S is safe). */
 template <int NEQ, typename R, bool WRITE_RHS, bool DO_RK, bool FIRST>
 __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__restrict__ G,
-                                                        double *__restrict__ S,
-                                                        double *__restrict__ SN,
+                                                        const double *S,
+                                                        double *SN,
                                                         double *__restrict__ RHS, int irk,
-                                                        double dt, int *__restrict__ err) {
+                                                        double dt, int *__restrict__ err,
+                                                        double *Sout) {
   __shared__ double shS[FV_BLOCK * NEQ];
   __shared__ double shN[FV_BLOCK * NEQ];
   const int lb = logical_block();
@@ -381,7 +358,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   __syncthreads();
   if constexpr (DO_RK) {
     for (int k = t; k < nval; k += FV_BLOCK) {
-      S[base + k] = shS[k];
+      Sout[base + k] = shS[k];
       if constexpr (FIRST) SN[base + k] = shN[k];
     }
     if constexpr (WRITE_RHS) {
@@ -820,131 +797,145 @@ __global__ void k_transpose_state(const double *__restrict__ src, double *__rest
 
 static inline int nblocks(int n) { return (n + FV_BLOCK - 1) / FV_BLOCK; }
 
+/* Every launcher takes the state it reads (S) explicitly: c->S in the time loop,
+ * a segment slot when the adjoint recomputes stage states. */
 template <int MODEL>
-static void launch_recon(fvg_ctx *c) {
+static void launch_recon(fvg_ctx *c, const double *S) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   if (c->rsize == 8)
-    hipLaunchKernelGGL((k_recon<MODEL, double>), g, b, 0, c->stream, M, c->dp, c->S, (double *)c->SF);
+    hipLaunchKernelGGL((k_recon<MODEL, double>), g, b, 0, c->stream, M, c->dp, S, (double *)c->SF);
   else
-    hipLaunchKernelGGL((k_recon<MODEL, float>), g, b, 0, c->stream, M, c->dp, c->S, (float *)c->SF);
+    hipLaunchKernelGGL((k_recon<MODEL, float>), g, b, 0, c->stream, M, c->dp, S, (float *)c->SF);
 }
 
 template <int MODEL>
-static void launch_grad(fvg_ctx *c, bool full) {
+static void launch_grad(fvg_ctx *c, const double *S, bool full) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   if (full)
-    hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+    hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)nullptr);
   else if (c->rsize == 8)
-    hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+    hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)c->SGm);
   else
-    hipLaunchKernelGGL((k_grad<MODEL, false, float>), g, b, 0, c->stream, M, c->dp, c->S, c->SG,
+    hipLaunchKernelGGL((k_grad<MODEL, false, float>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (float *)c->SGm);
 }
 
 template <int MODEL, bool NS>
-static void launch_face_m(fvg_ctx *c) {
+static void launch_face_m(fvg_ctx *c, const double *S) {
   DevMesh M = dev_mesh(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->rsize == 8)
     hipLaunchKernelGGL((k_face<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
-                       n1, n2, c->S, (const double *)c->SF, (const double *)c->SGm, (double *)c->G,
+                       n1, n2, S, (const double *)c->SF, (const double *)c->SGm, (double *)c->G,
                        c->err);
   else
     hipLaunchKernelGGL((k_face<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
-                       n1, n2, c->S, (const float *)c->SF, (const float *)c->SGm, (float *)c->G,
+                       n1, n2, S, (const float *)c->SF, (const float *)c->SGm, (float *)c->G,
                        c->err);
 }
 
 template <int MODEL, bool NS>
-static void launch_face_fused_m(fvg_ctx *c) {
+static void launch_face_fused_m(fvg_ctx *c, const double *S) {
   DevMesh M = dev_mesh(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->rsize == 8)
     hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0,
-                       b1, n0, n1, n2, c->S, (const double *)c->SGm, (double *)c->G, c->err);
+                       b1, n0, n1, n2, S, (const double *)c->SGm, (double *)c->G, c->err);
   else
     hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1,
-                       n0, n1, n2, c->S, (const float *)c->SGm, (float *)c->G, c->err);
+                       n0, n1, n2, S, (const float *)c->SGm, (float *)c->G, c->err);
 }
 
 template <int MODEL>
-static void launch_face_fused(fvg_ctx *c) {
-  if (c->prm.ns) launch_face_fused_m<MODEL, true>(c);
-  else launch_face_fused_m<MODEL, false>(c);
+static void launch_face_fused(fvg_ctx *c, const double *S) {
+  if (c->prm.ns) launch_face_fused_m<MODEL, true>(c, S);
+  else launch_face_fused_m<MODEL, false>(c, S);
 }
 
 template <int MODEL>
-static void launch_face(fvg_ctx *c) {
-  if (c->prm.ns) launch_face_m<MODEL, true>(c);
-  else launch_face_m<MODEL, false>(c);
+static void launch_face(fvg_ctx *c, const double *S) {
+  if (c->prm.ns) launch_face_m<MODEL, true>(c, S);
+  else launch_face_m<MODEL, false>(c, S);
 }
+
+/* stage state pointers: the gather reads S (and SN), writes Sout */
+struct StagePtrs {
+  const double *S;
+  double *SN, *Sout;
+};
 
 template <int NEQ, typename R>
-static void launch_gather_t(fvg_ctx *c, bool write_rhs, bool do_rk, bool first, int irk, double dt) {
+static void launch_gather_t(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first,
+                            int irk, double dt) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   const R *G = (const R *)c->G;
   if (write_rhs && !do_rk)
-    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, c->S,
-                       c->SN, c->RHS, irk, dt, c->err);
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, sp.S,
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
   else if (write_rhs && first)
-    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, c->S,
-                       c->SN, c->RHS, irk, dt, c->err);
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, sp.S,
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
   else if (write_rhs)
-    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, c->S,
-                       c->SN, c->RHS, irk, dt, c->err);
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, sp.S,
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
   else if (first)
-    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, c->S,
-                       c->SN, c->RHS, irk, dt, c->err);
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, sp.S,
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
   else
-    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, c->S,
-                       c->SN, c->RHS, irk, dt, c->err);
+    hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, sp.S,
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+}
+
+static void launch_gather(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first, int irk,
+                          double dt) {
+  if (c->neq == 5) {
+    if (c->rsize == 8) launch_gather_t<5, double>(c, sp, write_rhs, do_rk, first, irk, dt);
+    else launch_gather_t<5, float>(c, sp, write_rhs, do_rk, first, irk, dt);
+  } else {
+    if (c->rsize == 8) launch_gather_t<4, double>(c, sp, write_rhs, do_rk, first, irk, dt);
+    else launch_gather_t<4, float>(c, sp, write_rhs, do_rk, first, irk, dt);
+  }
 }
 
 static void launch_gather(fvg_ctx *c, bool write_rhs, bool do_rk, bool first, int irk, double dt) {
-  if (c->neq == 5) {
-    if (c->rsize == 8) launch_gather_t<5, double>(c, write_rhs, do_rk, first, irk, dt);
-    else launch_gather_t<5, float>(c, write_rhs, do_rk, first, irk, dt);
-  } else {
-    if (c->rsize == 8) launch_gather_t<4, double>(c, write_rhs, do_rk, first, irk, dt);
-    else launch_gather_t<4, float>(c, write_rhs, do_rk, first, irk, dt);
-  }
+  launch_gather(c, StagePtrs{c->S, c->SN, c->S}, write_rhs, do_rk, first, irk, dt);
 }
 
-static void dispatch_recon(fvg_ctx *c) {
+static void dispatch_recon(fvg_ctx *c, const double *S) {
   switch (c->prm.model) {
-    case 1: launch_recon<1>(c); break;
-    case 2: launch_recon<2>(c); break;
-    default: launch_recon<3>(c); break;
+    case 1: launch_recon<1>(c, S); break;
+    case 2: launch_recon<2>(c, S); break;
+    default: launch_recon<3>(c, S); break;
   }
 }
-static void dispatch_grad(fvg_ctx *c, bool full) {
+static void dispatch_grad(fvg_ctx *c, const double *S, bool full) {
   switch (c->prm.model) {
-    case 1: launch_grad<1>(c, full); break;
-    case 2: launch_grad<2>(c, full); break;
-    default: launch_grad<3>(c, full); break;
+    case 1: launch_grad<1>(c, S, full); break;
+    case 2: launch_grad<2>(c, S, full); break;
+    default: launch_grad<3>(c, S, full); break;
   }
 }
-static void dispatch_face_fused(fvg_ctx *c) {
+static void dispatch_face_fused(fvg_ctx *c, const double *S) {
   switch (c->prm.model) {
-    case 1: launch_face_fused<1>(c); break;
-    case 2: launch_face_fused<2>(c); break;
-    default: launch_face_fused<3>(c); break;
+    case 1: launch_face_fused<1>(c, S); break;
+    case 2: launch_face_fused<2>(c, S); break;
+    default: launch_face_fused<3>(c, S); break;
   }
 }
-static void dispatch_face(fvg_ctx *c) {
+static void dispatch_face(fvg_ctx *c, const double *S) {
   switch (c->prm.model) {
-    case 1: launch_face<1>(c); break;
-    case 2: launch_face<2>(c); break;
-    default: launch_face<3>(c); break;
+    case 1: launch_face<1>(c, S); break;
+    case 2: launch_face<2>(c, S); break;
+    default: launch_face<3>(c, S); break;
   }
 }
 
@@ -971,20 +962,26 @@ static int finish(fvg_ctx *c, const char *what) {
   return FVG_OK;
 }
 
-/* one stage's kernels; kt (optional) receives per-kernel event ids */
-static void stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs, hipEvent_t *ev,
-                  int *nev, int *kid) {
+/* one stage's kernels on the state sp.S -> sp.Sout; ev (optional) receives
+ * per-kernel events and kid their kernel ids */
+static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool write_rhs,
+                  hipEvent_t *ev, int *nev, int *kid) {
   auto mark = [&](int k) {
     if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
   };
-  if (c->prm.ns) { mark(1); dispatch_grad(c, false); }
+  if (c->prm.ns) { mark(1); dispatch_grad(c, sp.S, false); }
   mark(2);
   /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
-  if (c->prm.ischeme == 2) dispatch_face_fused(c);
-  else dispatch_face(c);
+  if (c->prm.ischeme == 2) dispatch_face_fused(c, sp.S);
+  else dispatch_face(c, sp.S);
   mark(3);
-  launch_gather(c, write_rhs, true, first, irk, dt);
+  launch_gather(c, sp, write_rhs, true, first, irk, dt);
   mark(-1);
+}
+
+static void stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs, hipEvent_t *ev,
+                  int *nev, int *kid) {
+  stage(c, StagePtrs{c->S, c->SN, c->S}, irk, dt, first, write_rhs, ev, nev, kid);
 }
 
 static const int *stage_list(int method, int *n) {
@@ -995,10 +992,14 @@ static const int *stage_list(int method, int *n) {
   return st1;
 }
 
-void fv_launch_recon(fvg_ctx *c) { dispatch_recon(c); }
+void fv_launch_recon(fvg_ctx *c, const double *S) { dispatch_recon(c, S); }
 
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs) {
   stage(c, irk, dt, first, write_rhs, nullptr, nullptr, nullptr);
+}
+
+void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int irk, double dt) {
+  stage(c, StagePtrs{S, SN, Sout}, irk, dt, false, false, nullptr, nullptr, nullptr);
 }
 
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
@@ -1007,10 +1008,10 @@ void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, i
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   if (first)
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err);
+                       (double *)nullptr, irk, dt, c->err, S);
   else
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, false>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err);
+                       (double *)nullptr, irk, dt, c->err, S);
 }
 
 extern "C" {
@@ -1053,15 +1054,15 @@ int fvg_device_ptr(fvg_ctx *c, int which, void **ptr) {
 int fvg_faceinterp(fvg_ctx *c) {
   if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
   (void)hipSetDevice(c->device);
-  dispatch_recon(c);
+  dispatch_recon(c, c->S);
   return finish(c, "faceinterp");
 }
 
 int fvg_gradcalc(fvg_ctx *c) {
   if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
   (void)hipSetDevice(c->device);
-  dispatch_grad(c, true);
-  if (c->prm.ns) dispatch_grad(c, false);
+  dispatch_grad(c, c->S, true);
+  if (c->prm.ns) dispatch_grad(c, c->S, false);
   return finish(c, "gradcalc");
 }
 
@@ -1069,7 +1070,7 @@ int fvg_rhs(fvg_ctx *c) {
   if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
   (void)hipSetDevice(c->device);
   reset_err(c);
-  dispatch_face(c);
+  dispatch_face(c, c->S);
   launch_gather(c, true, false, false, 0, 0.0);
   int rc = finish(c, "rhs");
   if (rc) return rc;

@@ -1,0 +1,203 @@
+"""Per-workload accuracy and throughput studies: plain FP64 against the mixed
+form, and the adjoint against the forward rollout.
+
+    python -m fv_140823_amd.studies [sod|blast|cavitation|tgv|vortex|all]
+
+Each study prints one JSON line.  The numbers are relative L1 norms,
+sum |x_mixed - x_plain| Vol / sum |x_plain| Vol, after the workload's step count at
+a fixed dt (the CFL dt of the initial state), plus cell-updates/s of both forms.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import configs, driver
+from .core import Solver
+
+
+def _vol(forest):
+    _, lev = forest.leaf_info()
+    box = forest.box[1] - forest.box[0]
+    return np.prod(box) / 8.0 ** (lev - 1).astype(float)
+
+
+def _l1(a, b, V):
+    return float((np.abs(a - b) * V).sum() / max((np.abs(b) * V).sum(), 1e-300))
+
+
+def _run(forest, p, steps, dt, hook=None, every=1):
+    g = Solver.from_forest(forest, p)
+    g.sync()
+    t0 = time.perf_counter()
+    if hook is None:
+        g.steps(steps, dt, 3)
+    else:
+        for n in range(0, steps, every):
+            g.steps(min(every, steps - n), dt, 3)
+            hook(n + min(every, steps - n), g)
+    g.sync()
+    wall = time.perf_counter() - t0
+    S = g.get_state()
+    g.close()
+    return S, wall
+
+
+def _mixed(p):
+    m = type(p).from_buffer_copy(p)
+    m.precision = 1
+    return m
+
+
+def study_sod(steps=100, level=5):
+    F, p, meta = configs.sod(level)
+    g = Solver.from_forest(F, p)
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    V = _vol(F)
+    Sp, tp = _run(F, p, steps, dt)
+    Sm, tm = _run(F, _mixed(p), steps, dt)
+    rp, up, vp, wp, pp = configs.cons_to_prim(Sp)
+    rm, um, vm, wm, pm = configs.cons_to_prim(Sm)
+    N = F.nleaves
+    return dict(study="sod", leaves=N, steps=steps, dt=dt,
+                l1_density=_l1(rm, rp, V), l1_pressure=_l1(pm, pp, V),
+                plain_cell_updates_per_s=N * 3 * steps / tp, mixed_cell_updates_per_s=N * 3 * steps / tm)
+
+
+def study_blast(steps=200):
+    """AMR run of the plain form with adaptation every 10 steps; the mixed form then
+    repeats the last 100 steps on the same (final) mesh from the same state."""
+    F, p, meta = configs.blast(5, 2)
+    g = Solver.from_forest(F, p)
+    flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, refine=0.01,
+                                                                 coarsen=0.002, base_level=6)
+    t0 = time.perf_counter()
+    st = driver.evolve(F, g, steps - 100, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"])
+    driver.remesh(F, g, flags, meta["max_level"])
+    F.set_state(g.get_state())
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    V = _vol(F)
+    Sp, tp = _run(F, p, 100, dt)
+    Sm, tm = _run(F, _mixed(p), 100, dt)
+    rp, *_, pp = configs.cons_to_prim(Sp)
+    rm, *_, pm = configs.cons_to_prim(Sm)
+    N = F.nleaves
+    return dict(study="blast", leaves=N, steps_amr=steps - 100, steps_compared=100,
+                remeshes=len(st["remesh"]) + 1, l1_density=_l1(rm, rp, V), l1_pressure=_l1(pm, pp, V),
+                plain_cell_updates_per_s=N * 300 / tp, mixed_cell_updates_per_s=N * 300 / tm,
+                amr_wall_s=time.perf_counter() - t0)
+
+
+def study_cavitation(steps=500, level=7):
+    F, p, meta = configs.cavitation(level)
+    g = Solver.from_forest(F, p)
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    V = _vol(F)
+    Sp, tp = _run(F, p, steps, dt)
+    Sm, tm = _run(F, _mixed(p), steps, dt)
+
+    def pres(S):
+        rho = S[0]
+        out = np.empty_like(rho)
+        liq = rho >= p.baro_rsat
+        out[liq] = p.baro_psat + p.baro_beta * ((rho[liq] / p.baro_rsat) ** p.baro_nexp - 1.0)
+        out[~liq] = p.baro_psat + p.baro_cbar * (1.0 / p.baro_rsat - 1.0 / rho[~liq])
+        return out
+
+    rho_v = 0.02
+    alpha = lambda S: np.clip((p.baro_rsat - S[0]) / (p.baro_rsat - rho_v), 0.0, 1.0)
+    N = F.nleaves
+    return dict(study="cavitation", leaves=N, steps=steps, dt=dt,
+                l1_density=_l1(Sm[0], Sp[0], V), l1_pressure=_l1(pres(Sm), pres(Sp), V),
+                l1_volume_fraction=_l1(alpha(Sm), alpha(Sp), V),
+                plain_cell_updates_per_s=N * 3 * steps / tp, mixed_cell_updates_per_s=N * 3 * steps / tm)
+
+
+def study_tgv(steps=1000, level=7, every=10):
+    F, p, meta = configs.tgv(level)
+    g = Solver.from_forest(F, p)
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    V = _vol(F)
+    series = {}
+    for tag, q in (("plain", p), ("mixed", _mixed(p))):
+        ek = []
+
+        def hook(n, g, ek=ek):
+            S = g.get_state()
+            ek.append((n * dt, float((0.5 * (S[1] ** 2 + S[2] ** 2 + S[3] ** 2) / S[0] * V).sum() / V.sum())))
+        t0 = time.perf_counter()
+        _run(F, q, steps, dt, hook, every)
+        series[tag] = (ek, time.perf_counter() - t0)
+    tp_, ep = zip(*series["plain"][0])
+    _, em = zip(*series["mixed"][0])
+    ep, em = np.array(ep), np.array(em)
+    eps_p = -np.diff(ep) / (every * dt)
+    eps_m = -np.diff(em) / (every * dt)
+    N = F.nleaves
+    return dict(study="tgv", leaves=N, steps=steps, dt=dt,
+                ke_dissipation_rate_rel_err_max=float(np.abs(eps_m - eps_p).max() / np.abs(eps_p).max()),
+                ke_dissipation_rate_rel_err_l1=float(np.abs(eps_m - eps_p).sum() / np.abs(eps_p).sum()),
+                final_ke_plain=float(ep[-1]), final_ke_mixed=float(em[-1]),
+                note="wall time includes a state download every 10 steps for the kinetic energy")
+
+
+def study_vortex(steps=50, ckpt=10, base_level=6):
+    F, p, meta = configs.vortex(base_level)
+    xc, _ = F.leaf_info()
+    probe = ((np.abs(xc[:, 0] - 6.5) < 1.0) & (np.abs(xc[:, 1] - 6.5) < 1.0)).astype(np.uint8)
+    out = dict(study="vortex", leaves=F.nleaves, steps=steps, ckpt=ckpt)
+    S0 = F.get_state()
+    grads = {}
+    for tag, q in (("plain", p), ("mixed", _mixed(p))):
+        g = Solver.from_forest(F, q)
+        g.adj_set_probe(probe)
+        dt = g.cfl_dt(meta["cfl"]) if tag == "plain" else out["dt"]
+        out["dt"] = dt
+        g.adj_forward(2, 1, dt)  # warm-up
+        g.set_state(S0)
+        g.sync()
+        t0 = time.perf_counter()
+        g.steps(steps, dt, 3)
+        g.sync()
+        t_fwd = time.perf_counter() - t0
+        g.set_state(S0)
+        t0 = time.perf_counter()
+        J = g.adj_forward(steps, ckpt, dt)
+        t_fwd_ck = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lam = g.adj_backward()
+        t_adj = time.perf_counter() - t0
+        g.close()
+        grads[tag] = lam
+        out[tag] = dict(J=J, forward_s=t_fwd, forward_with_checkpoints_s=t_fwd_ck, adjoint_s=t_adj,
+                        adjoint_over_forward=t_adj / t_fwd)
+    lp, lm = grads["plain"][:4], grads["mixed"][:4]
+    out["mixed_grad_rel_l2"] = float(np.linalg.norm(lm - lp) / np.linalg.norm(lp))
+    return out
+
+
+STUDIES = {"sod": study_sod, "blast": study_blast, "cavitation": study_cavitation, "tgv": study_tgv,
+           "vortex": study_vortex}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", nargs="?", default="all")
+    a = ap.parse_args()
+    names = list(STUDIES) if a.which == "all" else [a.which]
+    for n in names:
+        t0 = time.perf_counter()
+        r = STUDIES[n]()
+        r["wall_s"] = time.perf_counter() - t0
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -181,6 +181,10 @@ int fvg_time_steps(fvg_ctx *ctx, int nsteps, int method, double dt, double *ms_t
 int fvg_adj_set_probe(fvg_ctx *ctx, const uint8_t *mask /* host [ncell], 1 = in the probe box */);
 /* Runs the rollout, keeping S every `ckpt` steps; *J = objective; S ends at the final state. */
 int fvg_adj_forward(fvg_ctx *ctx, int nsteps, int ckpt, double dt, double *J);
+/* The objective of the last fvg_adj_forward / fvg_tlm as an unevaluated double-double sum
+ * J_hi_lo[0] + J_hi_lo[1] (accumulated with error-free transformations), for finite
+ * differences finer than the rounding of J itself. */
+int fvg_adj_objective(const fvg_ctx *ctx, double *J_hi_lo /* [2] */);
 /* Discrete adjoint dJ/dS^0 [neq][ncell] of the last forward rollout, recomputing stage
  * states from the checkpoints; into host memory and/or device memory (either may be NULL). */
 int fvg_adj_backward(fvg_ctx *ctx, double *grad_host, void *grad_dev);
