@@ -49,25 +49,39 @@ def log(msg, quiet=False):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes(kernel, m, neq, rs, ns_visc, first_stage, fused=True):
-    """Algorithmic HBM bytes of one launch: each input read once, each output
-    written once (per-unit figures in the design notes).  With `fused` the face
-    kernel reconstructs its face states from S (MUSCL in the flux kernel)."""
-    N, F, Q, Fb = m["N"], m["F"], m["Q"], m["Fb"]
+def ref_bytes_per_cell(part, neq, f, w, ns_visc):
+    """Algorithmic bytes per cell-update of the reference's data flow, one read and
+    one write of each of its data products per stage (RHS()/RK() split into K1
+    recon: S -> SF, K2 face flux: SF -> face flux, K3 gather: face flux -> RHS,
+    K4 RK: S, SN, RHS, 1/V -> S; K0 gradient S -> SG for Navier-Stokes), with
+    f faces per cell, 8-byte states and w-byte face intermediates (8 plain, 4
+    mixed).  K1..K4 sum to neq (48 + 6w) + 4 w neq f + 12 f + 56; Navier-Stokes
+    adds K0 and the SG reads of K2, 8 neq + 24 + 6 w (neq + 1) in all (56 neq + 72
+    plain, 32 neq + 48 mixed).  The fused kernels of this build are charged the
+    reference kernels they replace: the face kernel K1 + K2, the gather K3 + K4."""
+    k1 = 8 * neq + 24 + 6 * w * neq
+    k2 = 12 * f + 3 * w * neq * f
+    k3 = 24 + w * neq * f + 8 * neq
+    k4 = 32 * neq + 8
+    k0 = 0.0
+    if ns_visc:
+        k0 = 8 * neq + 24 + 3 * w * (neq + 1)       # K0: S + neighbours -> SG
+        k2 += 3 * w * (neq + 1)                     # K2 also reads SG (both sides)
+    return {"grad": k0, "face": k1 + k2, "gather_rk": k3 + k4,
+            "stage": k0 + k1 + k2 + k3 + k4}[part]
+
+
+def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage):
+    """Bytes a launch of this build's fused kernels must move at least: each
+    input read once, each output written once (MUSCL inside the face kernel)."""
+    N, F, Q = m["N"], m["F"], m["Q"]
     meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
-    if kernel == "recon":
-        return 8 * neq * N + meta_c + 6 * neq * rs * N
     if kernel == "grad":
         return 8 * 3 * N + meta_c + 9 * rs * N
     if kernel == "face":
-        if fused:
-            b = 8 * neq * N + meta_c + 12 * F + neq * rs * F
-            if ns_visc:
-                b += 9 * rs * N
-            return b
-        b = 12 * F + neq * rs * (2 * F - Fb) + neq * rs * F
+        b = 8 * neq * N + meta_c + 12 * F + neq * rs * F
         if ns_visc:
-            b += 9 * rs * (2 * F - Fb)
+            b += 9 * rs * N
         return b
     if kernel == "gather_rk":
         b = 4 * N + 24 * N + neq * rs * F + 8 * neq * N * (1 if first_stage else 2) + 8 * neq * N
@@ -230,20 +244,39 @@ def main():
     val_p, val_m = cu / wall_p, cu / wall_m
     log(f"plain {val_p:.4g} cell-upd/s, mixed {val_m:.4g} cell-upd/s", a.quiet)
     # dominant kernel of the plain form (largest time per step)
-    per_step = {KERNELS[i]: k_p[i] * (3 if i < 4 else 1) for i in range(4) if k_p[i] > 0}
+    per_step = {KERNELS[i]: k_p[i] for i in range(4) if k_p[i] > 0}
     dom = max(per_step, key=per_step.get)
     di = KERNELS.index(dom)
-    nb = alg_bytes(dom, mf, neq, 8, p.ns, False, p.ischeme == 2)
+    fpc = mf["F"] / mf["N"]
+    nb = ref_bytes_per_cell(dom, neq, fpc, 8, p.ns) * mf["N"]
     achieved = nb / (k_p[di] * 1e-3) / 1e9
     kname = {"face": "k_face_fused" if p.ischeme == 2 else "k_face"}.get(dom, f"k_{dom}")
+    comp = compulsory_bytes(dom, mf, neq, 8, p.ns, False)
     roof = dict(bound="hbm", kernel=kname, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=measured_traffic(meta["name"], kname),
-                alg_bytes_per_launch=int(nb), avg_launch_ms=round(k_p[di], 5))
-    kern = {KERNELS[i]: dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5),
-                             plain_gbs=round(alg_bytes(KERNELS[i], mf, neq, 8, p.ns, False, p.ischeme == 2) / (k_p[i] * 1e-3) / 1e9, 1) if k_p[i] else None,
-                             mixed_gbs=round(alg_bytes(KERNELS[i], mf, neq, 4, p.ns, False, p.ischeme == 2) / (k_m[i] * 1e-3) / 1e9, 1) if k_m[i] else None)
-            for i in range(4) if k_p[i] > 0}
+                alg_bytes_per_launch=int(nb), avg_launch_ms=round(k_p[di], 5),
+                compulsory_bytes_per_launch=int(comp),
+                compulsory_gbs=round(comp / (k_p[di] * 1e-3) / 1e9, 1),
+                faces_per_cell=round(fpc, 4))
+    # whole stage against the reference data flow: B x cell-updates/s
+    stage_roof = {}
+    for tag, val, w in (("plain", val_p, 8), ("mixed", val_m, 4)):
+        B = ref_bytes_per_cell("stage", neq, fpc, w, p.ns)
+        stage_roof[tag] = dict(bytes_per_cell_update=round(B, 1), achieved_gbs=round(B * val / 1e9, 1),
+                               frac=round(B * val / 1e9 / HBM_PEAK_GBS, 4))
+    kern = {}
+    for i in range(4):
+        if k_p[i] <= 0:
+            continue
+        k = KERNELS[i]
+        e = dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
+        if k != "recon":
+            for tag, ms, w, rs in (("plain", k_p[i], 8, 8), ("mixed", k_m[i], 4, 4)):
+                if ms:
+                    e[f"{tag}_alg_gbs"] = round(ref_bytes_per_cell(k, neq, fpc, w, p.ns) * mf["N"] / (ms * 1e-3) / 1e9, 1)
+                    e[f"{tag}_compulsory_gbs"] = round(compulsory_bytes(k, mf, neq, rs, p.ns, False) / (ms * 1e-3) / 1e9, 1)
+        kern[k] = e
     acc = l1_rel_density(F, p, pm, a.acc_steps, dt)
     log(f"mixed vs plain after {a.acc_steps} steps: {acc}", a.quiet)
     cpu = None
@@ -270,6 +303,7 @@ def main():
                   "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},
         "kernels": kern,
         "roofline": roof,
+        "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

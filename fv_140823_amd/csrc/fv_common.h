@@ -164,6 +164,68 @@ __device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
   }
 }
 
+/* ---- FP32 reconstruction of the mixed form (ideal gas): the gather keeps an
+ * FP32 copy Sf of the cell-major state beside the FP64 one, and the face kernel
+ * reconstructs from it (half the bytes of every gathered state). */
+template <int NEQ>
+__device__ __forceinline__ void load_Sf(const float *__restrict__ Sf, int i, float *U) {
+  const float *p = Sf + (size_t)i * NEQ;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) U[v] = p[v];
+}
+
+template <int NEQ>
+__device__ __forceinline__ void nb_state_f(const DevMesh &M, const DevParams &P,
+                                           const float *__restrict__ Sf, int i, uint32_t cf, int f,
+                                           const float *Uown, float *Un) {
+  int cls = (cf >> (2 * f)) & 3;
+  int nb = M.nbr[(size_t)f * M.ns + i];
+  if (cls == CLS_SAME || cls == CLS_COARSER) {
+    load_Sf<NEQ>(Sf, nb, Un);
+  } else if (cls == CLS_FINER) {
+    int q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = M.quad[4 * nb + k];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc += Sf[(size_t)q[k] * NEQ + v] * 0.25f;
+      Un[v] = acc;
+    }
+  } else {
+    int bc = -1 - nb;
+    if (bc == 1) {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Un[v] = (float)P.inlet[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Un[v] = Uown[v];
+    }
+  }
+}
+
+/* recon_face in FP32 throughout, from the FP32 state copy */
+template <int NEQ>
+__device__ __forceinline__ void recon_face_f(const DevMesh &M, const DevParams &P,
+                                             const float *__restrict__ Sf, int c, int f,
+                                             float *out) {
+  const uint32_t cf = M.cflag[c];
+  const int fo = f ^ (f < 4 ? 2 : 1);
+  float U[NEQ], U0[NEQ], U1[NEQ];
+  load_Sf<NEQ>(Sf, c, U);
+  nb_state_f<NEQ>(M, P, Sf, c, cf, f, U, U0);
+  nb_state_f<NEQ>(M, P, Sf, c, cf, fo, U, U1);
+  double pcf, pcc, mcf, mcc, pic, mic;
+  face_dist(M, cf, f, pcf, pcc, pic);
+  face_dist(M, cf, fo, mcf, mcc, mic);
+  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
+  const float fp = (float)pcf, ip = (float)pic, im = (float)mic;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v)
+    out[v] = U[v] + fp * limited<float>(P.limiter, P.va_eps, bnd, U0[v] - U[v], U1[v] - U[v], ip, im);
+}
+
 struct fvg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -179,6 +241,10 @@ struct fvg_ctx {
   uint32_t *fmeta = nullptr;
   double *tab = nullptr;
   double *S = nullptr, *SN = nullptr, *RHS = nullptr, *SG = nullptr;
+  /* spare state buffer: the time loop runs its stages out of place over S, SN and
+   * Sx and swaps the three pointers at the end of each step (so S and SN keep
+   * their meaning, but device addresses taken from them change) */
+  double *Sx = nullptr;
   void *SF = nullptr, *SGm = nullptr, *G = nullptr;
   int *err = nullptr;
   double *red = nullptr;
@@ -188,6 +254,11 @@ struct fvg_ctx {
   std::string last;
   hipEvent_t ev[2 * 64 + 2];
   bool ev_ok = false;
+  /* FP32 copy of S for the mixed ideal-gas face kernel; valid while nothing but
+   * that path's gather has written S since it was filled */
+  float *Sf = nullptr;
+  bool sf_valid = false;
+
   /* adjoint / tangent buffers (fvadj.hip), released with the mesh */
   struct fv_adj_state *adj = nullptr;
 };

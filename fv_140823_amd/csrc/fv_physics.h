@@ -256,6 +256,22 @@ FV_HD __forceinline__ void make_state_ideal(const DevParams &P, const T *U, FSta
   for (int v = 0; v < 5; ++v) s.q[v] = U[v];
 }
 
+/* Ideal-gas inversion of the mixed form in FP32.  Returns false where FP32 is
+ * not good enough: near vacuum (rho <= 1e-20), where the internal energy is a
+ * small difference of large terms (rho e_int < E / 100, or |p| < gamma p_inf /
+ * 100), or on a non-finite result.  The caller then rebuilds that face state
+ * from the FP64 cell states and inverts it in FP64.  Where FP32 is kept, the
+ * relative error of p is about 1e-7 (E / rho e_int) (1 + gamma p_inf / |p|),
+ * at most about 1e-5. */
+FV_HD __forceinline__ bool make_state_mixed_ideal(const DevParams &P, const float *U,
+                                                  FState<float> &s) {
+  make_state_ideal<float>(P, U, s);
+  /* rho e_int = (p + gamma p_inf) / (gamma - 1) */
+  const float gp = (float)(P.gamma * P.pinf), gm1 = (float)(P.gamma - 1.0);
+  return s.r > 1e-20f && (s.p + gp) >= 0.01f * gm1 * s.e && fabsf(s.p) >= 0.01f * gp &&
+         s.c == s.c && fabsf(s.c) < __builtin_inff();
+}
+
 /* bars of the primitive state (r, u, e, p, c) -> bars of the conserved vector */
 template <typename T>
 FV_HD __forceinline__ void state_ideal_vjp(const DevParams &P, const FState<T> &s, T rb, T ub[3],

@@ -220,7 +220,10 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
 }
 
 template <int MODEL, bool NS, typename T, typename R>
-__global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_face_fused(DevMesh M, DevParams P, int b0, int b1,
+/* occupancy targets: 6 waves per SIMD for the Euler flux (78-80 VGPRs, no
+ * spill); the viscous variants need 96-98 VGPRs and ask for fewer waves instead
+ * of spilling */
+__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k_face_fused(DevMesh M, DevParams P, int b0, int b1,
                                                          int n0, int n1, int n2,
                                                          const double *__restrict__ S,
                                                          const R *__restrict__ SGm,
@@ -237,6 +240,183 @@ __global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) v
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
     if (q < n2) face_fused_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SGm, G, err);
   }
+}
+
+/* Mixed form of the ideal gas (model 1): MUSCL from the FP32 state copy, the
+ * FP32 inversion (make_state_mixed_ideal), FP32 HLLC and viscous flux, FP32
+ * face flux; FP64 stays in the gather, the RK update and the time step.  A face
+ * with a state the FP32 inversion rejects (near vacuum, internal energy a small
+ * difference of large terms) is left to face_rebuild_block, which rebuilds it
+ * from the FP64 cell states with the FP64 inversion, exactly as the FP64-read
+ * mixed kernel computes it.  Returns false for such a face. */
+template <bool NS, int a>
+__device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams &P, int j,
+                                               const float *__restrict__ Sf,
+                                               const float *__restrict__ SGm,
+                                               float *__restrict__ G, int *__restrict__ err) {
+  const int iL = M.fL[j], iR = M.fR[j];
+  const uint32_t meta = M.fmeta[j];
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const float s = (float)c_face_sign[fl];
+  FState<float> L, Rs;
+  float U[5];
+  recon_face_f<5>(M, P, Sf, iL, fl, U);
+  bool fp32 = make_state_mixed_ideal(P, U, L);
+  if (bc == 0) {
+    recon_face_f<5>(M, P, Sf, iR, fr, U);
+    fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
+  } else if (bc == 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) U[v] = (float)P.inlet[v];
+    fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
+  } else {
+    Rs = L;
+  }
+  if (!fp32) return false;
+  if constexpr (NS) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      L.g[k] = SGm[(size_t)k * M.ns + iL];
+      Rs.g[k] = SGm[(size_t)k * M.ns + iR];
+    }
+  }
+  float out[5];
+  const bool ok = hllc_face<1, NS, a, float>(P, L, Rs, s, out);
+  const float A = (float)M.area[3 * lev + a];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) G[(size_t)j * 5 + v] = -(A)*out[v];
+  if (!ok) atomicMin(err, iL);
+  return true;
+}
+
+/* variable v of the MUSCL neighbour state (nb_state for one variable) */
+__device__ __forceinline__ double nb_var(const DevMesh &M, const DevParams &P,
+                                         const double *__restrict__ S, int i, uint32_t cf, int f,
+                                         int v, double own) {
+  const int cls = (cf >> (2 * f)) & 3;
+  const int nb = M.nbr[(size_t)f * M.ns + i];
+  if (cls == CLS_SAME || cls == CLS_COARSER) return S[(size_t)nb * 5 + v];
+  if (cls == CLS_FINER) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += S[(size_t)M.quad[4 * nb + k] * 5 + v] / ((double)4);
+    return acc;
+  }
+  return (-1 - nb) == 1 ? P.inlet[v] : own;
+}
+
+/* variable v of recon_face<5, float> (the same arithmetic, one variable) */
+__device__ __forceinline__ double recon_face_var(const DevMesh &M, const DevParams &P,
+                                                 const double *__restrict__ S, int c, int f, int v) {
+  const uint32_t cf = M.cflag[c];
+  const int fo = f ^ (f < 4 ? 2 : 1);
+  const double u = S[(size_t)c * 5 + v];
+  const double u0 = nb_var(M, P, S, c, cf, f, v, u), u1 = nb_var(M, P, S, c, cf, fo, v, u);
+  double pcf, pcc, mcf, mcc, pic, mic;
+  face_dist(M, cf, f, pcf, pcc, pic);
+  face_dist(M, cf, fo, mcf, mcc, mic);
+  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
+  const float g = limited<float>(P.limiter, P.va_eps, bnd, (float)(u0 - u), (float)(u1 - u),
+                                 (float)pic, (float)mic);
+  return u + (double)((float)pcf * g);
+}
+
+/* The faces of this block that the FP32 inversion rejected (list[0 .. n)),
+ * rebuilt as the FP64-read mixed kernel computes them (MUSCL from the FP64
+ * states with the FP32 limiter, FP64 inversion, FP32 flux).  25 faces at a time:
+ * one thread per (face, side, variable) stages the FP64 face states in LDS, then
+ * one thread per face inverts them and evaluates the flux.  Written lean so the
+ * kernel's register budget stays close to that of its main path. */
+template <bool NS>
+__device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
+                                   const double *__restrict__ S, const float *__restrict__ SGm,
+                                   float *__restrict__ G, int *__restrict__ err, const int *list,
+                                   int n) {
+  constexpr int CH = 25;
+  __shared__ double sX[CH][2][5];
+  const int t = threadIdx.x;
+  for (int base = 0; base < n; base += CH) {
+    if (t < CH * 10) {
+      const int fi = t / 10, side = (t / 5) & 1, v = t % 5;
+      if (base + fi < n) {
+        const int j = list[base + fi];
+        const uint32_t meta = M.fmeta[j];
+        const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3;
+        double x;
+        if (side == 0 || bc == 2) x = recon_face_var(M, P, S, M.fL[j], fl, v);
+        else if (bc == 0) x = recon_face_var(M, P, S, M.fR[j], fr, v);
+        else x = P.inlet[v];
+        sX[fi][side][v] = x;
+      }
+    }
+    __syncthreads();
+    if (t < CH && base + t < n) {
+      const int j = list[base + t];
+      const uint32_t meta = M.fmeta[j];
+      const int fl = meta & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+      const int iL = M.fL[j], iR = M.fR[j];
+      const int a = c_face_axis[fl];
+      FState<float> L, Rs;
+      bool ok = make_state<1, float>(P, sX[t][0], true, L);
+      ok = make_state<1, float>(P, sX[t][1], bc != 1, Rs) && ok;
+      if constexpr (NS) {
+        for (int k = 0; k < 9; ++k) {
+          L.g[k] = SGm[(size_t)k * M.ns + iL];
+          Rs.g[k] = SGm[(size_t)k * M.ns + iR];
+        }
+      }
+      const float s = (float)c_face_sign[fl];
+      float out[5];
+      if (a == 0) ok = hllc_face<1, NS, 0, float>(P, L, Rs, s, out) && ok;
+      else if (a == 1) ok = hllc_face<1, NS, 1, float>(P, L, Rs, s, out) && ok;
+      else ok = hllc_face<1, NS, 2, float>(P, L, Rs, s, out) && ok;
+      const float A = (float)M.area[3 * lev + a];
+      for (int v = 0; v < 5; ++v) G[(size_t)j * 5 + v] = -(A)*out[v];
+      if (!ok) atomicMin(err, iL);
+    }
+    __syncthreads();
+  }
+}
+
+/* The faces of a block whose FP32 inversion was rejected are collected in LDS
+ * and rebuilt by the block itself (no other block, no extra launch); blocks with
+ * no such face skip it after one block-wide vote. */
+template <bool NS>
+__global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, DevParams P, int b0,
+                                                                   int b1, int n0, int n1, int n2,
+                                                                   const double *__restrict__ S,
+                                                                   const float *__restrict__ Sf,
+                                                                   const float *__restrict__ SGm,
+                                                                   float *__restrict__ G,
+                                                                   int *__restrict__ err) {
+  __shared__ int list[FV_BLOCK];
+  __shared__ int nlist;
+  const int lb = logical_block();
+  const int t = threadIdx.x;
+  if (t == 0) nlist = 0;
+  __syncthreads();
+  int j = -1;
+  bool done = true;
+  if (lb < b0) {
+    const int q = lb * FV_BLOCK + t;
+    if (q < n0) { j = q; done = face_fast_body<NS, 0>(M, P, j, Sf, SGm, G, err); }
+  } else if (lb < b0 + b1) {
+    const int q = (lb - b0) * FV_BLOCK + t;
+    if (q < n1) { j = n0 + q; done = face_fast_body<NS, 1>(M, P, j, Sf, SGm, G, err); }
+  } else {
+    const int q = (lb - b0 - b1) * FV_BLOCK + t;
+    if (q < n2) { j = n0 + n1 + q; done = face_fast_body<NS, 2>(M, P, j, Sf, SGm, G, err); }
+  }
+  if (!__syncthreads_or(!done)) return;
+  if (!done) list[atomicAdd(&nlist, 1)] = j;
+  __syncthreads();
+  face_rebuild_block<NS>(M, P, S, SGm, G, err, list, nlist);
+}
+
+/* S (FP64, cell-major) -> its FP32 copy */
+__global__ void k_to_f32(const double *__restrict__ S, float *__restrict__ Sf, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) Sf[k] = (float)S[k];
 }
 
 /* Faces are grouped by axis; blocks [0, b0) take x faces, [b0, b0+b1) y faces,
@@ -269,14 +449,15 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
  * and each thread works on its own row in LDS.  The stage reads S and writes Sout,
  * which is S itself in the time loop and another buffer when the adjoint
  * recomputes stage states into its segment store (each block reads its whole span
- * before any thread writes, so Sout == S is safe). */
+ * before any thread writes, so Sout == S is safe).  Sf (optional) receives the
+ * FP32 copy of the new state for the mixed ideal-gas face kernel. */
 template <int NEQ, typename R, bool WRITE_RHS, bool DO_RK, bool FIRST>
 __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__restrict__ G,
                                                         const double *S,
                                                         double *SN,
                                                         double *__restrict__ RHS, int irk,
                                                         double dt, int *__restrict__ err,
-                                                        double *Sout) {
+                                                        double *Sout, float *Sf) {
   __shared__ double shS[FV_BLOCK * NEQ];
   __shared__ double shN[FV_BLOCK * NEQ];
   const int lb = logical_block();
@@ -359,6 +540,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   if constexpr (DO_RK) {
     for (int k = t; k < nval; k += FV_BLOCK) {
       Sout[base + k] = shS[k];
+      if (Sf) Sf[base + k] = (float)shS[k];
       if constexpr (FIRST) SN[base + k] = shN[k];
     }
     if constexpr (WRITE_RHS) {
@@ -537,13 +719,16 @@ static int check_params(const fvg_params *p) {
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->gslot, c->fmeta, c->tab,
-                  c->S, c->SN, c->RHS, c->SG, c->SF, c->SGm, c->G};
+                  c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->gslot = nullptr;
   c->fmeta = nullptr; c->tab = nullptr;
   c->S = c->SN = c->RHS = c->SG = nullptr;
   c->SF = c->SGm = c->G = nullptr;
+  c->Sf = nullptr;
+  c->Sx = nullptr;
+  c->sf_valid = false;
   c->has_mesh = false;
 }
 
@@ -671,6 +856,9 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   HIPCHK(c, hipMalloc(&c->SF, (size_t)6 * neq * ns * rs));
   HIPCHK(c, hipMalloc(&c->SGm, (size_t)9 * ns * rs));
   HIPCHK(c, hipMalloc(&c->G, (size_t)neq * c->nfp * rs));
+  HIPCHK(c, hipMalloc(&c->Sx, (size_t)neq * ns * 8));
+  if (rs == 4 && c->prm.model == 1) HIPCHK(c, hipMalloc(&c->Sf, (size_t)neq * ns * 4));
+  c->sf_valid = false;
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemcpyAsync(c->cflag, m->cflag, (size_t)N * 4, hipMemcpyHostToDevice, st));
   for (int f = 0; f < 6; ++f) {
@@ -697,6 +885,7 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   /* zero the padding and every state array so no kernel reads uninitialised memory */
   HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SN, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->Sx, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
@@ -740,6 +929,7 @@ static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
   if (!rows) return set_err(c, FVG_ERR_ARG, "unknown field");
   const int n = which == FVG_FIELD_FACEFLUX ? c->F : c->N;
   (void)hipSetDevice(c->device);
+  if (to_dev && which == FVG_FIELD_S) c->sf_valid = false;
   const size_t es = isR ? c->rsize : 8;
   if (cell_major(which)) {
     /* host var-major [rows][n] <-> device [n][rows] */
@@ -870,6 +1060,7 @@ static void launch_face(fvg_ctx *c, const double *S) {
 struct StagePtrs {
   const double *S;
   double *SN, *Sout;
+  float *Sf; /* FP32 copy of Sout to refresh, or null */
 };
 
 template <int NEQ, typename R>
@@ -880,19 +1071,19 @@ static void launch_gather_t(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk
   const R *G = (const R *)c->G;
   if (write_rhs && !do_rk)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
   else if (write_rhs && first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
   else if (write_rhs)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
   else if (first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
   else
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
 }
 
 static void launch_gather(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first, int irk,
@@ -907,7 +1098,7 @@ static void launch_gather(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, 
 }
 
 static void launch_gather(fvg_ctx *c, bool write_rhs, bool do_rk, bool first, int irk, double dt) {
-  launch_gather(c, StagePtrs{c->S, c->SN, c->S}, write_rhs, do_rk, first, irk, dt);
+  launch_gather(c, StagePtrs{c->S, c->SN, c->S, nullptr}, write_rhs, do_rk, first, irk, dt);
 }
 
 static void dispatch_recon(fvg_ctx *c, const double *S) {
@@ -962,26 +1153,28 @@ static int finish(fvg_ctx *c, const char *what) {
   return FVG_OK;
 }
 
-/* one stage's kernels on the state sp.S -> sp.Sout; ev (optional) receives
- * per-kernel events and kid their kernel ids */
-static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool write_rhs,
-                  hipEvent_t *ev, int *nev, int *kid) {
-  auto mark = [&](int k) {
-    if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
-  };
-  if (c->prm.ns) { mark(1); dispatch_grad(c, sp.S, false); }
-  mark(2);
-  /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
-  if (c->prm.ischeme == 2) dispatch_face_fused(c, sp.S);
-  else dispatch_face(c, sp.S);
-  mark(3);
-  launch_gather(c, sp, write_rhs, true, first, irk, dt);
-  mark(-1);
+/* S: the stage's FP64 input (c->Sf holds its FP32 copy) */
+static void launch_face_fast(fvg_ctx *c, const double *S) {
+  DevMesh M = dev_mesh(c);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
+  dim3 g(b0 + b1 + b2), b(FV_BLOCK);
+  if (c->prm.ns)
+    hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2, S, c->Sf,
+                       (const float *)c->SGm, (float *)c->G, c->err);
+  else
+    hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2, S, c->Sf,
+                       (const float *)c->SGm, (float *)c->G, c->err);
 }
 
-static void stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs, hipEvent_t *ev,
-                  int *nev, int *kid) {
-  stage(c, StagePtrs{c->S, c->SN, c->S}, irk, dt, first, write_rhs, ev, nev, kid);
+/* the FP32 state copy of the mixed ideal-gas path, refreshed if anything else
+ * wrote S since the last stage of that path */
+static void ensure_sf(fvg_ctx *c) {
+  if (c->sf_valid) return;
+  const int n = c->N * c->neq;
+  hipLaunchKernelGGL(k_to_f32, dim3((n + FV_BLOCK - 1) / FV_BLOCK), dim3(FV_BLOCK), 0, c->stream,
+                     (const double *)c->S, c->Sf, n);
+  c->sf_valid = true;
 }
 
 static const int *stage_list(int method, int *n) {
@@ -992,14 +1185,70 @@ static const int *stage_list(int method, int *n) {
   return st1;
 }
 
+static bool fast_mixed(const fvg_ctx *c) {
+  return c->Sf && c->prm.model == 1 && c->prm.ischeme == 2;
+}
+
+/* one stage's kernels on the state sp.S -> sp.Sout; ev (optional) receives
+ * per-kernel events and kid their kernel ids.  `fast` selects the mixed
+ * ideal-gas path (FP32 reconstruction from Sf, which the caller keeps equal to
+ * the FP32 copy of sp.S); it needs an out-of-place stage. */
+static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool write_rhs,
+                  hipEvent_t *ev, int *nev, int *kid, bool fast) {
+  auto mark = [&](int k) {
+    if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
+  };
+  fast = fast && fast_mixed(c) && !first && sp.Sout != sp.S;
+  if (c->prm.ns) { mark(1); dispatch_grad(c, sp.S, false); }
+  mark(2);
+  /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
+  if (fast) launch_face_fast(c, sp.S);
+  else if (c->prm.ischeme == 2) dispatch_face_fused(c, sp.S);
+  else dispatch_face(c, sp.S);
+  mark(3);
+  sp.Sf = fast ? c->Sf : nullptr;
+  launch_gather(c, sp, write_rhs, true, first, irk, dt);
+  mark(-1);
+}
+
+/* One time step (RK(0) + the method's stages), out of place: S^n stays in its
+ * buffer and is the SN of every stage (no RK(0) copy is made), each stage writes
+ * a buffer that is neither its input nor S^n, and the three state pointers are
+ * then rotated so that c->S holds S^{n+1} and c->SN holds S^n, as after the
+ * reference's RK(0) .. RK(3). */
+static void time_step(fvg_ctx *c, int method, double dt, bool write_rhs_last, hipEvent_t *ev,
+                      int *nev, int *kid) {
+  int n;
+  const int *st = stage_list(method, &n);
+  const bool fast = fast_mixed(c);
+  if (fast) ensure_sf(c);
+  double *X0 = c->S, *buf[2] = {c->Sx, c->SN};
+  const double *cur = X0;
+  for (int k = 0; k < n; ++k) {
+    double *out = buf[k & 1];
+    stage(c, StagePtrs{cur, X0, out, nullptr}, st[k], dt, false, write_rhs_last && k == n - 1, ev,
+          nev, kid, true);
+    cur = out;
+  }
+  c->SN = X0;
+  c->S = buf[(n - 1) & 1];
+  c->Sx = buf[n & 1];
+  c->sf_valid = fast; /* the last gather wrote the FP32 copy of the new S */
+}
+
 void fv_launch_recon(fvg_ctx *c, const double *S) { dispatch_recon(c, S); }
 
+/* the adjoint's forward stages: always the FP64-read path, whose face states
+ * the adjoint face kernel rebuilds (recon_face) */
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs) {
-  stage(c, irk, dt, first, write_rhs, nullptr, nullptr, nullptr);
+  stage(c, StagePtrs{c->S, c->SN, c->S, nullptr}, irk, dt, first, write_rhs, nullptr, nullptr,
+        nullptr, false);
+  c->sf_valid = false;
 }
 
 void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int irk, double dt) {
-  stage(c, StagePtrs{S, SN, Sout}, irk, dt, false, false, nullptr, nullptr, nullptr);
+  stage(c, StagePtrs{S, SN, Sout, nullptr}, irk, dt, false, false, nullptr, nullptr, nullptr, false);
+  if (Sout == c->S) c->sf_valid = false;
 }
 
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
@@ -1008,10 +1257,10 @@ void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, i
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   if (first)
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err, S);
+                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr);
   else
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, false>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err, S);
+                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr);
 }
 
 extern "C" {
@@ -1027,6 +1276,7 @@ static int state_device_copy(fvg_ctx *c, void *d, bool to_ctx) {
   if (!c || !d) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
   (void)hipSetDevice(c->device);
+  if (to_ctx) c->sf_valid = false;
   if (to_ctx)
     hipLaunchKernelGGL(k_transpose_state, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
                        (const double *)d, c->S, c->N, c->neq, 1);
@@ -1091,6 +1341,7 @@ int fvg_rk(fvg_ctx *c, int irk, double dt) {
   (void)hipSetDevice(c->device);
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  c->sf_valid = false;
   if (c->neq == 5)
     hipLaunchKernelGGL(k_rk<5>, g, b, 0, c->stream, M, c->S, c->SN, c->RHS, irk, dt);
   else
@@ -1102,13 +1353,9 @@ int fvg_steps(fvg_ctx *c, int nsteps, int method, double dt) {
   if (!c || !c->has_mesh) return c ? set_err(c, FVG_ERR_STATE, "no mesh") : FVG_ERR_ARG;
   if (method < 1 || method > 3 || nsteps < 0) return set_err(c, FVG_ERR_ARG, "bad method/nsteps");
   (void)hipSetDevice(c->device);
-  int n;
-  const int *st = stage_list(method, &n);
   reset_err(c);
   /* RHS is written on the last stage only: afterwards it holds that stage's residual */
-  for (int s = 0; s < nsteps; ++s)
-    for (int k = 0; k < n; ++k)
-      stage(c, st[k], dt, k == 0, s == nsteps - 1 && k == n - 1, nullptr, nullptr, nullptr);
+  for (int s = 0; s < nsteps; ++s) time_step(c, method, dt, s == nsteps - 1, nullptr, nullptr, nullptr);
   int rc = finish(c, "steps");
   if (rc) return rc;
   int32_t bad;
@@ -1133,8 +1380,6 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
     c->ev_ok = true;
   }
-  int n;
-  const int *st = stage_list(method, &n);
   double sum[FV_NKT] = {0};
   int cnt[FV_NKT] = {0};
   reset_err(c);
@@ -1143,7 +1388,7 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
   HIPCHK(c, hipEventRecord(t0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
     int nev = 0, kid[64];
-    for (int k = 0; k < n; ++k) stage(c, st[k], dt, k == 0, false, c->ev, &nev, kid);
+    time_step(c, method, dt, false, c->ev, &nev, kid);
     HIPCHK(c, hipEventSynchronize(c->ev[nev - 1]));
     for (int q = 0; q + 1 < nev; ++q) {
       if (kid[q] < 0) continue;
