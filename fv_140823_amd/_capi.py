@@ -81,6 +81,9 @@ class FvgMesh(C.Structure):
         ("dist", C.POINTER(C.c_double)),
         ("area", C.POINTER(C.c_double)),
         ("vol", C.POINTER(C.c_double)),
+        ("nface_reg", C.c_int32 * 3),
+        ("face_LL", C.POINTER(C.c_int32)),
+        ("face_RR", C.POINTER(C.c_int32)),
     ]
 
 

@@ -177,7 +177,9 @@ class FlatMesh:
             h=arr(m.h, 3 * nl, np.float64).reshape(nl, 3),
             dist=arr(m.dist, 18 * nl, np.float64).reshape(nl, 3, 3, 2),
             area=arr(m.area, 3 * nl, np.float64).reshape(nl, 3),
-            vol=arr(m.vol, nl, np.float64), nface_axis=tuple(m.nface_axis))
+            vol=arr(m.vol, nl, np.float64), nface_axis=tuple(m.nface_axis),
+            nface_reg=tuple(m.nface_reg), face_LL=arr(m.face_LL, F, np.int32),
+            face_RR=arr(m.face_RR, F, np.int32))
 
 
 class Solver:

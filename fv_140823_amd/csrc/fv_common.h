@@ -30,6 +30,8 @@ static __constant__ int c_face_sign[6] = {-1, 1, 1, -1, -1, 1};
 enum { CLS_SAME = 0, CLS_FINER = 1, CLS_COARSER = 2, CLS_BOUNDARY = 3 };
 
 static inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+/* host copy of c_face_axis */
+static inline int c_face_axis_host(int f) { return f == 1 || f == 3 ? 0 : (f == 0 || f == 2 ? 1 : 2); }
 
 /* ------------------------------------------------------------------ mesh views */
 
@@ -39,6 +41,7 @@ struct DevMesh {
   const int32_t *nbr;      /* [6][ns] */
   const int32_t *quad;     /* [Q][4] */
   const int32_t *fL, *fR;
+  const int32_t *fLL, *fRR; /* far-side neighbours of regular faces */
   const uint32_t *fmeta;
   const int32_t *gslot;    /* [6][ns] */
   const double *h, *dist, *area, *vol, *idist;
@@ -235,9 +238,11 @@ struct fvg_ctx {
   bool has_mesh = false;
   int N = 0, ns = 0, F = 0, nfp = 0, Q = 0, nlevel = 0;
   int nfa[3] = {0, 0, 0};
+  int nreg[3] = {0, 0, 0}; /* regular faces lead each axis group */
   /* device buffers */
   uint32_t *cflag = nullptr;
   int32_t *nbr = nullptr, *quad = nullptr, *fL = nullptr, *fR = nullptr, *gslot = nullptr;
+  int32_t *fLL = nullptr, *fRR = nullptr;
   uint32_t *fmeta = nullptr;
   double *tab = nullptr;
   double *S = nullptr, *SN = nullptr, *RHS = nullptr, *SG = nullptr;
@@ -301,6 +306,8 @@ static inline DevMesh dev_mesh(const fvg_ctx *c) {
   m.quad = c->quad;
   m.fL = c->fL;
   m.fR = c->fR;
+  m.fLL = c->fLL;
+  m.fRR = c->fRR;
   m.fmeta = c->fmeta;
   m.gslot = c->gslot;
   const int nl = c->nlevel;

@@ -179,8 +179,11 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
 }
 
 /* face flux with the reconstruction fused in: reads S (the 2-ring through the
- * caches) instead of a stored SF, same results as recon + face */
-template <int MODEL, bool NS, int a, typename T, typename R>
+ * caches) instead of a stored SF, same results as recon + face.  REG: a regular
+ * face (interior, stencil face_LL, face_L, face_R, face_RR of one level along
+ * the axis): the same arithmetic as recon_face with the neighbours and the
+ * same-level distances known, without the class and neighbour lookups. */
+template <int MODEL, bool NS, int a, typename T, typename R, bool REG>
 __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParams &P, int j,
                                                 const double *__restrict__ S,
                                                 const R *__restrict__ SGm, R *__restrict__ G,
@@ -188,18 +191,38 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
-  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = REG ? 0 : (meta >> 6) & 3,
+            lev = (meta >> 8) & 0xff;
   const T s = (T)c_face_sign[fl];
   double UL[NEQ], UR[NEQ];
-  recon_face<NEQ, T>(M, P, S, iL, fl, UL);
-  if (bc == 0) {
-    recon_face<NEQ, T>(M, P, S, iR, fr, UR);
-  } else if (bc == 1) {
+  if constexpr (REG) {
+    const int k = (lev * 3 + a) * 3; /* same-level distances of this level and axis */
+    const double pcf = M.dist[2 * k], ic = M.idist[k];
+    double A0[NEQ], A1[NEQ], B0[NEQ], B1[NEQ];
+    load_S<NEQ>(S, M.ns, M.fLL[j], A0);
+    load_S<NEQ>(S, M.ns, iL, A1);
+    load_S<NEQ>(S, M.ns, iR, B0);
+    load_S<NEQ>(S, M.ns, M.fRR[j], B1);
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
+    for (int v = 0; v < NEQ; ++v) {
+      const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0[v] - A1[v]), (T)(A0[v] - A1[v]),
+                              (T)ic, (T)ic);
+      const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1[v] - B0[v]), (T)(B1[v] - B0[v]),
+                              (T)ic, (T)ic);
+      UL[v] = A1[v] + (double)((T)pcf * gL);
+      UR[v] = B0[v] + (double)((T)pcf * gR);
+    }
   } else {
+    recon_face<NEQ, T>(M, P, S, iL, fl, UL);
+    if (bc == 0) {
+      recon_face<NEQ, T>(M, P, S, iR, fr, UR);
+    } else if (bc == 1) {
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) UR[v] = UL[v];
+      for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) UR[v] = UL[v];
+    }
   }
   FState<T> L, Rs;
   bool ok = make_state<MODEL, T>(P, UL, true, L);
@@ -223,22 +246,26 @@ template <int MODEL, bool NS, typename T, typename R>
 /* occupancy targets: 6 waves per SIMD for the Euler flux (78-80 VGPRs, no
  * spill); the viscous variants need 96-98 VGPRs and ask for fewer waves instead
  * of spilling */
-__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k_face_fused(DevMesh M, DevParams P, int b0, int b1,
-                                                         int n0, int n1, int n2,
-                                                         const double *__restrict__ S,
-                                                         const R *__restrict__ SGm,
-                                                         R *__restrict__ G, int *__restrict__ err) {
+__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k_face_fused(
+    DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
+    const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
+    int *__restrict__ err) {
   const int lb = logical_block();
   const int t = threadIdx.x;
+  /* axis a: faces [0, r_a) of the group are regular (a block straddles the end of
+   * that range at most once per axis) */
   if (lb < b0) {
-    const int j = lb * FV_BLOCK + t;
-    if (j < n0) face_fused_body<MODEL, NS, 0, T, R>(M, P, j, S, SGm, G, err);
+    const int q = lb * FV_BLOCK + t;
+    if (q < r0) face_fused_body<MODEL, NS, 0, T, R, true>(M, P, q, S, SGm, G, err);
+    else if (q < n0) face_fused_body<MODEL, NS, 0, T, R, false>(M, P, q, S, SGm, G, err);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) face_fused_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SGm, G, err);
+    if (q < r1) face_fused_body<MODEL, NS, 1, T, R, true>(M, P, n0 + q, S, SGm, G, err);
+    else if (q < n1) face_fused_body<MODEL, NS, 1, T, R, false>(M, P, n0 + q, S, SGm, G, err);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) face_fused_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SGm, G, err);
+    if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true>(M, P, n0 + n1 + q, S, SGm, G, err);
+    else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false>(M, P, n0 + n1 + q, S, SGm, G, err);
   }
 }
 
@@ -249,28 +276,49 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
  * difference of large terms) is left to face_rebuild_block, which rebuilds it
  * from the FP64 cell states with the FP64 inversion, exactly as the FP64-read
  * mixed kernel computes it.  Returns false for such a face. */
-template <bool NS, int a>
+template <bool NS, int a, bool REG>
 __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams &P, int j,
                                                const float *__restrict__ Sf,
                                                const float *__restrict__ SGm,
                                                float *__restrict__ G, int *__restrict__ err) {
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
-  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = REG ? 0 : (meta >> 6) & 3,
+            lev = (meta >> 8) & 0xff;
   const float s = (float)c_face_sign[fl];
   FState<float> L, Rs;
   float U[5];
-  recon_face_f<5>(M, P, Sf, iL, fl, U);
-  bool fp32 = make_state_mixed_ideal(P, U, L);
-  if (bc == 0) {
-    recon_face_f<5>(M, P, Sf, iR, fr, U);
-    fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
-  } else if (bc == 1) {
+  bool fp32;
+  if constexpr (REG) {
+    /* regular face: recon_face_f with the four-cell stencil and the same-level
+     * distances known (same arithmetic) */
+    const int k = (lev * 3 + a) * 3;
+    const float fp = (float)M.dist[2 * k], ic = (float)M.idist[k];
+    float A0[5], A1[5], B0[5], B1[5], V[5];
+    load_Sf<5>(Sf, M.fLL[j], A0);
+    load_Sf<5>(Sf, iL, A1);
+    load_Sf<5>(Sf, iR, B0);
+    load_Sf<5>(Sf, M.fRR[j], B1);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) U[v] = (float)P.inlet[v];
-    fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
+    for (int v = 0; v < 5; ++v) {
+      U[v] = A1[v] + fp * limited<float>(P.limiter, P.va_eps, false, B0[v] - A1[v], A0[v] - A1[v], ic, ic);
+      V[v] = B0[v] + fp * limited<float>(P.limiter, P.va_eps, false, A1[v] - B0[v], B1[v] - B0[v], ic, ic);
+    }
+    fp32 = make_state_mixed_ideal(P, U, L);
+    fp32 = make_state_mixed_ideal(P, V, Rs) && fp32;
   } else {
-    Rs = L;
+    recon_face_f<5>(M, P, Sf, iL, fl, U);
+    fp32 = make_state_mixed_ideal(P, U, L);
+    if (bc == 0) {
+      recon_face_f<5>(M, P, Sf, iR, fr, U);
+      fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
+    } else if (bc == 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) U[v] = (float)P.inlet[v];
+      fp32 = make_state_mixed_ideal(P, U, Rs) && fp32;
+    } else {
+      Rs = L;
+    }
   }
   if (!fp32) return false;
   if constexpr (NS) {
@@ -384,6 +432,7 @@ __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
 template <bool NS>
 __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, DevParams P, int b0,
                                                                    int b1, int n0, int n1, int n2,
+                                                                   int r0, int r1, int r2,
                                                                    const double *__restrict__ S,
                                                                    const float *__restrict__ Sf,
                                                                    const float *__restrict__ SGm,
@@ -399,13 +448,19 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
   bool done = true;
   if (lb < b0) {
     const int q = lb * FV_BLOCK + t;
-    if (q < n0) { j = q; done = face_fast_body<NS, 0>(M, P, j, Sf, SGm, G, err); }
+    j = q;
+    if (q < r0) done = face_fast_body<NS, 0, true>(M, P, j, Sf, SGm, G, err);
+    else if (q < n0) done = face_fast_body<NS, 0, false>(M, P, j, Sf, SGm, G, err);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) { j = n0 + q; done = face_fast_body<NS, 1>(M, P, j, Sf, SGm, G, err); }
+    j = n0 + q;
+    if (q < r1) done = face_fast_body<NS, 1, true>(M, P, j, Sf, SGm, G, err);
+    else if (q < n1) done = face_fast_body<NS, 1, false>(M, P, j, Sf, SGm, G, err);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) { j = n0 + n1 + q; done = face_fast_body<NS, 2>(M, P, j, Sf, SGm, G, err); }
+    j = n0 + n1 + q;
+    if (q < r2) done = face_fast_body<NS, 2, true>(M, P, j, Sf, SGm, G, err);
+    else if (q < n2) done = face_fast_body<NS, 2, false>(M, P, j, Sf, SGm, G, err);
   }
   if (!__syncthreads_or(!done)) return;
   if (!done) list[atomicAdd(&nlist, 1)] = j;
@@ -718,11 +773,11 @@ static int check_params(const fvg_params *p) {
 
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
-  void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->gslot, c->fmeta, c->tab,
+  void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, c->tab,
                   c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->gslot = nullptr;
+  c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->fLL = c->fRR = c->gslot = nullptr;
   c->fmeta = nullptr; c->tab = nullptr;
   c->S = c->SN = c->RHS = c->SG = nullptr;
   c->SF = c->SGm = c->G = nullptr;
@@ -821,6 +876,33 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   }
   for (int64_t q = 0; q < 4LL * Q; ++q)
     if (m->quad_nbr[q] < 0 || m->quad_nbr[q] >= N) return set_err(c, FVG_ERR_ARG, "bad quad neighbour");
+  /* regular faces: each axis group starts with nface_reg[a] faces whose stencil
+   * face_LL, face_L, face_R, face_RR is four same-level cells along the axis */
+  {
+    static const int opp[6] = {2, 3, 0, 1, 5, 4};
+    int64_t start = 0;
+    for (int a = 0; a < 3; ++a) {
+      if (m->nface_reg[a] < 0 || m->nface_reg[a] > m->nface_axis[a])
+        return set_err(c, FVG_ERR_ARG, "bad regular face count");
+      if (m->nface_reg[a] && (!m->face_LL || !m->face_RR))
+        return set_err(c, FVG_ERR_ARG, "regular faces without face_LL / face_RR");
+      for (int64_t q = start; q < start + m->nface_reg[a]; ++q) {
+        const int iL = m->face_L[q], iR = m->face_R[q], iLL = m->face_LL[q], iRR = m->face_RR[q];
+        const uint32_t mt = m->face_meta[q];
+        const int f = mt & 7, fo = (mt >> 3) & 7;
+        if (iL < 0 || iL >= N || iR < 0 || iR >= N || iLL < 0 || iLL >= N || iRR < 0 || iRR >= N ||
+            f > 5 || fo > 5 || ((mt >> 6) & 3) != 0)
+          return set_err(c, FVG_ERR_ARG, "bad regular face");
+        const uint32_t cL = m->cflag[iL], cR = m->cflag[iR];
+        if (((cL >> (2 * f)) & 3) != CLS_SAME || ((cL >> (2 * opp[f])) & 3) != CLS_SAME ||
+            ((cR >> (2 * opp[fo])) & 3) != CLS_SAME || m->nbr[(size_t)f * N + iL] != iR ||
+            m->nbr[(size_t)opp[f] * N + iL] != iLL || m->nbr[(size_t)opp[fo] * N + iR] != iRR ||
+            c_face_axis_host(f) != a)
+          return set_err(c, FVG_ERR_ARG, "face listed as regular does not have a regular stencil");
+      }
+      start += m->nface_axis[a];
+    }
+  }
   for (int j = 0; j < F; ++j) {
     if (m->face_L[j] < 0 || m->face_L[j] >= N || m->face_R[j] < 0 || m->face_R[j] >= N)
       return set_err(c, FVG_ERR_ARG, "face cell out of range");
@@ -838,7 +920,10 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   c->nfp = (int)round_up(F, 64);
   c->Q = Q;
   c->nlevel = nl;
-  for (int a = 0; a < 3; ++a) c->nfa[a] = m->nface_axis[a];
+  for (int a = 0; a < 3; ++a) {
+    c->nfa[a] = m->nface_axis[a];
+    c->nreg[a] = m->nface_reg[a];
+  }
   const int ns = c->ns, neq = c->neq;
   const size_t rs = c->rsize;
   HIPCHK(c, hipMalloc(&c->cflag, (size_t)ns * 4));
@@ -847,6 +932,8 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   HIPCHK(c, hipMalloc(&c->quad, (size_t)4 * (Q ? Q : 1) * 4));
   HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
   HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fLL, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fRR, (size_t)c->nfp * 4));
   HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
   HIPCHK(c, hipMalloc(&c->tab, (size_t)34 * nl * 8));
   HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
@@ -870,6 +957,13 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   if (Q) HIPCHK(c, hipMemcpyAsync(c->quad, m->quad_nbr, (size_t)4 * Q * 4, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->fL, m->face_L, (size_t)F * 4, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->fR, m->face_R, (size_t)F * 4, hipMemcpyHostToDevice, st));
+  if (m->face_LL && m->face_RR) {
+    HIPCHK(c, hipMemcpyAsync(c->fLL, m->face_LL, (size_t)F * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->fRR, m->face_RR, (size_t)F * 4, hipMemcpyHostToDevice, st));
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->fLL, 0xff, (size_t)F * 4, st));
+    HIPCHK(c, hipMemsetAsync(c->fRR, 0xff, (size_t)F * 4, st));
+  }
   HIPCHK(c, hipMemcpyAsync(c->fmeta, m->face_meta, (size_t)F * 4, hipMemcpyHostToDevice, st));
   std::vector<double> tab((size_t)34 * nl, 0.0);
   memcpy(tab.data(), m->h, (size_t)3 * nl * 8);
@@ -1038,10 +1132,10 @@ static void launch_face_fused_m(fvg_ctx *c, const double *S) {
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->rsize == 8)
     hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0,
-                       b1, n0, n1, n2, S, (const double *)c->SGm, (double *)c->G, c->err);
+                       b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const double *)c->SGm, (double *)c->G, c->err);
   else
     hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1,
-                       n0, n1, n2, S, (const float *)c->SGm, (float *)c->G, c->err);
+                       n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const float *)c->SGm, (float *)c->G, c->err);
 }
 
 template <int MODEL>
@@ -1160,10 +1254,12 @@ static void launch_face_fast(fvg_ctx *c, const double *S) {
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->prm.ns)
-    hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2, S, c->Sf,
+    hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2,
+                       c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
                        (const float *)c->SGm, (float *)c->G, c->err);
   else
-    hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2, S, c->Sf,
+    hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2,
+                       c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
                        (const float *)c->SGm, (float *)c->G, c->err);
 }
 

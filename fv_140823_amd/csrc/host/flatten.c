@@ -7,10 +7,14 @@
  *          coarser, domain boundary);
  *   nbr    the neighbour index per face (quad index for finer faces, -1-bc on
  *          the boundary).
- * Unique faces, grouped by axis (x, y, z) and within an axis ordered by the
- * owner leaf: a same-level face is owned by the lower leaf index, a hanging
- * face by the coarse side (4 consecutive sub-faces k = 0..3 in the order of
- * neigtr[f][k], elconnfin.c:41-45), a boundary face by its leaf.
+ * Unique faces, grouped by axis (x, y, z); a same-level face is owned by the
+ * lower leaf index, a hanging face by the coarse side (4 consecutive sub-faces
+ * k = 0..3 in the order of neigtr[f][k], elconnfin.c:41-45), a boundary face by
+ * its leaf.  Within an axis the regular faces come first, then the others, each
+ * part in owner-leaf order.  A regular face is an interior same-level face whose
+ * two cells also have same-level neighbours on their far sides along the axis:
+ * its MUSCL stencil is four cells of one level (face_LL, face_L, face_R,
+ * face_RR), and the kernels evaluate it without the class and neighbour lookups.
  * gslot[f][i] names the face (and side) whose flux enters leaf i through its
  * face f, in the reference's accumulation order f = 0..5, k = 0..3
  * (residual_faces_AMR.c:37-175).
@@ -32,7 +36,7 @@
 struct fvh_flat {
   fvg_mesh m;
   uint32_t *cflag;
-  int32_t *nbr, *quad_nbr, *face_L, *face_R, *gslot;
+  int32_t *nbr, *quad_nbr, *face_L, *face_R, *face_LL, *face_RR, *gslot;
   uint32_t *face_meta;
   double h[FV_MAXLEV * 3];
   double dist[FV_MAXLEV * 3 * 3 * 2];
@@ -60,7 +64,7 @@ static int set_table(double *slot, double v, unsigned char *have) {
 void fvh_flat_destroy(fvh_flat *fl) {
   if (!fl) return;
   free(fl->cflag); free(fl->nbr); free(fl->quad_nbr); free(fl->face_L); free(fl->face_R);
-  free(fl->gslot); free(fl->face_meta);
+  free(fl->gslot); free(fl->face_meta); free(fl->face_LL); free(fl->face_RR);
   free(fl);
 }
 
@@ -251,6 +255,55 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
       if (!other && fl->face_L[jf] != i) { rc = FVG_ERR_UNSUPPORTED; break; }
     }
   if (rc) goto fail;
+  /* ---- regular faces first within each axis (stable), and their outer cells */
+  {
+    int32_t *perm = (int32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(int32_t));
+    int32_t *nL = (int32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(int32_t));
+    int32_t *nR = (int32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(int32_t));
+    uint32_t *nM = (uint32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(uint32_t));
+    fl->face_LL = (int32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(int32_t));
+    fl->face_RR = (int32_t *)malloc((size_t)(nface ? nface : 1) * sizeof(int32_t));
+    if (!perm || !nL || !nR || !nM || !fl->face_LL || !fl->face_RR) {
+      free(perm); free(nL); free(nR); free(nM);
+      rc = FVG_ERR_NOMEM;
+      goto fail;
+    }
+    int64_t start = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int64_t end = start + nfa[a];
+      int64_t nreg = 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        int64_t pos = pass == 0 ? start : start + nreg;
+        for (int64_t q = start; q < end; ++q) {
+          const int iL = fl->face_L[q], iR = fl->face_R[q];
+          const uint32_t mt = fl->face_meta[q];
+          const int f = mt & 7, fo = (mt >> 3) & 7, bc = (mt >> 6) & 3;
+          const uint32_t cL = fl->cflag[iL], cR = fl->cflag[iR];
+          const int fL2 = FV_FACE_OPP[f], fR2 = FV_FACE_OPP[fo];
+          const int reg = bc == 0 && ((cL >> (2 * f)) & 3) == FV_CLS_SAME &&
+                          ((cL >> (2 * fL2)) & 3) == FV_CLS_SAME &&
+                          ((cR >> (2 * fR2)) & 3) == FV_CLS_SAME;
+          if (reg != (pass == 0)) continue;
+          perm[q] = (int32_t)pos;
+          nL[pos] = iL;
+          nR[pos] = iR;
+          nM[pos] = mt;
+          fl->face_LL[pos] = reg ? fl->nbr[(size_t)fL2 * N + iL] : -1;
+          fl->face_RR[pos] = reg ? fl->nbr[(size_t)fR2 * N + iR] : -1;
+          ++pos;
+        }
+        if (pass == 0) nreg = pos - start;
+      }
+      fl->m.nface_reg[a] = (int32_t)nreg;
+      start = end;
+    }
+    for (size_t q = 0; q < (size_t)6 * N; ++q)
+      if (fl->gslot[q] >= 0) fl->gslot[q] = (perm[fl->gslot[q] >> 1] << 1) | (fl->gslot[q] & 1);
+    memcpy(fl->face_L, nL, (size_t)nface * sizeof(int32_t));
+    memcpy(fl->face_R, nR, (size_t)nface * sizeof(int32_t));
+    memcpy(fl->face_meta, nM, (size_t)nface * sizeof(uint32_t));
+    free(perm); free(nL); free(nR); free(nM);
+  }
   free(lv);
   lv = NULL;
   fl->m.ncell = N;
@@ -267,6 +320,8 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
   fl->m.face_L = fl->face_L;
   fl->m.face_R = fl->face_R;
   fl->m.face_meta = fl->face_meta;
+  fl->m.face_LL = fl->face_LL;
+  fl->m.face_RR = fl->face_RR;
   fl->m.gslot = fl->gslot;
   fl->m.h = fl->h;
   fl->m.dist = fl->dist;

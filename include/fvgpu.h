@@ -96,6 +96,13 @@ typedef struct fvg_mesh {
   const double *dist;     /* [nlevel][3][3][2] (d_cf, d_cc) per level, axis, class (distance.c) */
   const double *area;     /* [nlevel][3] face area per level of the finer side and axis */
   const double *vol;      /* [nlevel] cell volume */
+  /* Within each axis the first nface_reg[a] faces are regular: interior, same level, and
+   * both cells have same-level neighbours on their far sides along the axis (a MUSCL
+   * stencil of four cells of one level).  For them face_LL / face_RR hold the far-side
+   * neighbours of face_L / face_R; for the others, -1. */
+  int32_t nface_reg[3];
+  const int32_t *face_LL; /* [nface] */
+  const int32_t *face_RR; /* [nface] */
 } fvg_mesh;
 
 typedef struct fvg_ctx fvg_ctx;

@@ -58,6 +58,16 @@ def test_periodic_counts():
     F = uniform(3, 5, periodic=(1, 1, 1), bc=(0,) * 6)
     fl = fv.FlatMesh(F.run)
     assert fl.nface == 3 * 8 ** 3
+    # every periodic face has a same-level four-cell stencil
+    assert tuple(fl.arrays()["nface_reg"]) == (8 ** 3,) * 3
+
+
+def test_regular_face_counts_uniform():
+    """8^3 with outflow walls: a face along an axis is regular when its lower cell
+    has index 1..5 along that axis (both far neighbours exist): 5 * 64 per axis."""
+    F = uniform(3, 5)
+    fl, A = _check_flat(F)
+    assert tuple(A["nface_reg"]) == (5 * 64,) * 3
 
 
 def _check_flat(F):
@@ -91,6 +101,24 @@ def _check_flat(F):
     bc = (A["face_meta"] >> 6) & 3
     assert np.all(cnt[bc == 0] == 0)
     assert np.all(cnt[bc != 0] == 1)
+    # regular faces first within each axis, and exactly the faces whose MUSCL
+    # stencil is four cells of one level (face_LL, face_L, face_R, face_RR)
+    opp = np.array([2, 3, 0, 1, 5, 4])
+    start = 0
+    for ax in range(3):
+        n_a, r_a = A["nface_axis"][ax], A["nface_reg"][ax]
+        for q in range(start, start + n_a):
+            iL, iR, mt = A["face_L"][q], A["face_R"][q], int(A["face_meta"][q])
+            f, fo, bcq = mt & 7, (mt >> 3) & 7, (mt >> 6) & 3
+            reg = (bcq == 0 and cls[f][iL] == 0 and cls[opp[f]][iL] == 0 and cls[opp[fo]][iR] == 0)
+            assert reg == (q < start + r_a), (ax, q)
+            if reg:
+                assert A["face_LL"][q] == A["nbr"][opp[f]][iL]
+                assert A["face_RR"][q] == A["nbr"][opp[fo]][iR]
+                assert L[A["face_LL"][q]] == L[iL] == L[iR] == L[A["face_RR"][q]]
+            else:
+                assert A["face_LL"][q] == -1 and A["face_RR"][q] == -1
+        start += n_a
     # hanging faces: neighbour level = own level + 1, areas halve per level
     a = A["area"][:, 0]
     lv = np.nonzero(a)[0]
