@@ -167,6 +167,32 @@ __device__ __forceinline__ void recon_face(const DevMesh &M, const DevParams &P,
   }
 }
 
+/* MUSCL face states of a regular face j (interior; stencil M.fLL[j], iL, iR,
+ * M.fRR[j] of one level along axis a): recon_face for both sides with the
+ * neighbours and the same-level distances known, the same arithmetic without
+ * the class and neighbour lookups */
+template <int NEQ, typename T>
+__device__ __forceinline__ void recon_regular(const DevMesh &M, const DevParams &P,
+                                              const double *__restrict__ S, int j, int iL, int iR,
+                                              int lev, int a, double *UL, double *UR) {
+  const int k = (lev * 3 + a) * 3; /* same-level distances of this level and axis */
+  const double pcf = M.dist[2 * k], ic = M.idist[k];
+  double A0[NEQ], A1[NEQ], B0[NEQ], B1[NEQ];
+  load_S<NEQ>(S, M.ns, M.fLL[j], A0);
+  load_S<NEQ>(S, M.ns, iL, A1);
+  load_S<NEQ>(S, M.ns, iR, B0);
+  load_S<NEQ>(S, M.ns, M.fRR[j], B1);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0[v] - A1[v]), (T)(A0[v] - A1[v]),
+                            (T)ic, (T)ic);
+    const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1[v] - B0[v]), (T)(B1[v] - B0[v]),
+                            (T)ic, (T)ic);
+    UL[v] = A1[v] + (double)((T)pcf * gL);
+    UR[v] = B0[v] + (double)((T)pcf * gR);
+  }
+}
+
 /* ---- FP32 reconstruction of the mixed form (ideal gas): the gather keeps an
  * FP32 copy Sf of the cell-major state beside the FP64 one, and the face kernel
  * reconstructs from it (half the bytes of every gathered state). */

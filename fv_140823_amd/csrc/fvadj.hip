@@ -61,24 +61,28 @@ __device__ __forceinline__ double cell_vol(const DevMesh &M, int i) {
 
 /* face^T: lamUL/lamUR[j][v] = d(sum_i mu_i . R_i)/d(face states of face j), with
  * mu_i = cdt lam_i / Vol_i.  The face states are rebuilt exactly as the forward
- * face kernel builds them (recon_face), so the VJP is taken at the forward's
- * own face states in either precision. */
-template <int a, typename T, typename R>
+ * face kernel builds them (recon_face, recon_regular for regular faces), so the
+ * VJP is taken at the forward's own face states in either precision. */
+template <int a, typename T, typename R, bool REG>
 __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams &P, int j,
                                               const double *__restrict__ S,
                                               const double *__restrict__ lam, double cdt,
                                               R *__restrict__ lUL, R *__restrict__ lUR) {
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
-  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = (meta >> 6) & 3, lev = (meta >> 8) & 0xff;
+  const int fl = meta & 7, fr = (meta >> 3) & 7, bc = REG ? 0 : (meta >> 6) & 3,
+            lev = (meta >> 8) & 0xff;
   const T s = (T)c_face_sign[fl];
   const bool first = P.ischeme == 1;
   double XL[5], XR[5];
-  if (first) load_S<5>(S, M.ns, iL, XL);
-  else recon_face<5, T>(M, P, S, iL, fl, XL);
-  if (bc == 0) {
-    if (first) load_S<5>(S, M.ns, iR, XR);
-    else recon_face<5, T>(M, P, S, iR, fr, XR);
+  if (first) {
+    load_S<5>(S, M.ns, iL, XL);
+    if (bc == 0) load_S<5>(S, M.ns, iR, XR);
+  } else if constexpr (REG) {
+    recon_regular<5, T>(M, P, S, j, iL, iR, lev, a, XL, XR);
+  } else {
+    recon_face<5, T>(M, P, S, iL, fl, XL);
+    if (bc == 0) recon_face<5, T>(M, P, S, iR, fr, XR);
   }
   const double VL = cell_vol(M, iL), VR = cell_vol(M, iR);
   T UL[5], UR[5], ob[5], ULb[5], URb[5];
@@ -103,22 +107,24 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
 }
 
 template <typename T, typename R>
-__global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_face(DevMesh M, DevParams P, int b0, int b1,
-                                                       int n0, int n1, int n2,
-                                                       const double *__restrict__ S,
-                                                       const double *__restrict__ lam, double cdt,
-                                                       R *__restrict__ lUL, R *__restrict__ lUR) {
+__global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_face(
+    DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
+    const double *__restrict__ S, const double *__restrict__ lam, double cdt, R *__restrict__ lUL,
+    R *__restrict__ lUR) {
   const int lb = logical_block();
   const int t = threadIdx.x;
   if (lb < b0) {
-    const int j = lb * FV_BLOCK + t;
-    if (j < n0) adj_face_body<0, T, R>(M, P, j, S, lam, cdt, lUL, lUR);
+    const int q = lb * FV_BLOCK + t;
+    if (q < r0) adj_face_body<0, T, R, true>(M, P, q, S, lam, cdt, lUL, lUR);
+    else if (q < n0) adj_face_body<0, T, R, false>(M, P, q, S, lam, cdt, lUL, lUR);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) adj_face_body<1, T, R>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
+    if (q < r1) adj_face_body<1, T, R, true>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
+    else if (q < n1) adj_face_body<1, T, R, false>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) adj_face_body<2, T, R>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
+    if (q < r2) adj_face_body<2, T, R, true>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
+    else if (q < n2) adj_face_body<2, T, R, false>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
   }
 }
 
@@ -530,13 +536,15 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
   const int nc = nblk(c->N);
   if (c->rsize == 8) {
     hipLaunchKernelGGL((k_adj_face<double, double>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, S, a->lam, cdt, (double *)a->lamUL,
+                       M, c->dp, b0, b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, a->lam,
+                       cdt, (double *)a->lamUL,
                        (double *)a->lamUR);
     hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
                        (const double *)a->lamUL, (const double *)a->lamUR, a->self, a->Pp, a->Pm);
   } else {
     hipLaunchKernelGGL((k_adj_face<float, float>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, S, a->lam, cdt, (float *)a->lamUL,
+                       M, c->dp, b0, b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, a->lam,
+                       cdt, (float *)a->lamUL,
                        (float *)a->lamUR);
     hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
                        (const float *)a->lamUL, (const float *)a->lamUR, a->self, a->Pp, a->Pm);

@@ -196,22 +196,7 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
   const T s = (T)c_face_sign[fl];
   double UL[NEQ], UR[NEQ];
   if constexpr (REG) {
-    const int k = (lev * 3 + a) * 3; /* same-level distances of this level and axis */
-    const double pcf = M.dist[2 * k], ic = M.idist[k];
-    double A0[NEQ], A1[NEQ], B0[NEQ], B1[NEQ];
-    load_S<NEQ>(S, M.ns, M.fLL[j], A0);
-    load_S<NEQ>(S, M.ns, iL, A1);
-    load_S<NEQ>(S, M.ns, iR, B0);
-    load_S<NEQ>(S, M.ns, M.fRR[j], B1);
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0[v] - A1[v]), (T)(A0[v] - A1[v]),
-                              (T)ic, (T)ic);
-      const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1[v] - B0[v]), (T)(B1[v] - B0[v]),
-                              (T)ic, (T)ic);
-      UL[v] = A1[v] + (double)((T)pcf * gL);
-      UR[v] = B0[v] + (double)((T)pcf * gR);
-    }
+    recon_regular<NEQ, T>(M, P, S, j, iL, iR, lev, a, UL, UR);
   } else {
     recon_face<NEQ, T>(M, P, S, iL, fl, UL);
     if (bc == 0) {
@@ -522,10 +507,19 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   const int nrow = min(FV_BLOCK, M.N - i0);
   const int nval = nrow * NEQ;
   const size_t base = (size_t)i0 * NEQ;
+  /* the first stage of the out-of-place time step has SN == S: read it once */
   if constexpr (DO_RK) {
-    for (int k = t; k < nval; k += FV_BLOCK) {
-      shS[k] = S[base + k];
-      if constexpr (!FIRST) shN[k] = SN[base + k];
+    if (FIRST || SN == S) {
+      for (int k = t; k < nval; k += FV_BLOCK) {
+        const double x = S[base + k];
+        shS[k] = x;
+        if constexpr (!FIRST) shN[k] = x;
+      }
+    } else {
+      for (int k = t; k < nval; k += FV_BLOCK) {
+        shS[k] = S[base + k];
+        shN[k] = SN[base + k];
+      }
     }
   }
   double rhs[NEQ];
