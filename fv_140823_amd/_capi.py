@@ -137,6 +137,9 @@ _DECLS = [
     ("fvg_tlm", C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp, _dp]),
     ("fvg_face_flux_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp]),
     ("fvg_face_vjp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    ("fvg_face_flux_ns_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    ("fvg_face_vjp_ns_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _dp]),
     ("fvg_face_jvp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     ("fvg_limiter_host", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_double, _dp, _dp, _dp]),

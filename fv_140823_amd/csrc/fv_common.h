@@ -299,6 +299,11 @@ void fv_adj_release(fvg_ctx *c);
 /* forward pieces shared with fvadj.hip (defined in fvgpu.hip) */
 void fv_launch_recon(fvg_ctx *c, const double *S);
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs);
+/* momentum gradients (gradcalc, rows 1..3) of S into c->SGm (precision of the context) */
+void fv_launch_grad(fvg_ctx *c, const double *S);
+/* the same gradients in FP64 of a tangent state Sd (the inlet ghost, a constant, has
+ * tangent zero) into SGmd [9][ns] */
+void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd);
 /* one stage Sout = RK(irk; SN, S, R(S)) without touching c->S / c->SN */
 void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int irk, double dt);
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,

@@ -272,6 +272,73 @@ FV_HD __forceinline__ bool make_state_mixed_ideal(const DevParams &P, const floa
          s.c == s.c && fabsf(s.c) < __builtin_inff();
 }
 
+/* Reverse of the viscous part of hllc_face<1, true, a> (flux_viscous.c:58-73,
+ * 199-232): F[1+i] -= tau[a][i] and, with visc_work, F[4] -= uf . tau[a], with
+ * uf = (uL + uR) / 2 and tau from gr = (gL + gR) / 2 (gradients of the conserved
+ * momenta, gr[3 i + j] = d(rho u_i)/dx_j).  Given ob = bar of out (out = F s +
+ * corr), adds the velocity part to ULb / URb and returns in gb[9] the bar of
+ * each side's gradients (the same for both sides: d gr / d gL = 1/2). */
+template <int a, typename T>
+FV_HD __forceinline__ void visc_vjp_ideal(const DevParams &P, const T *UL, const T *UR,
+                                          const T *gL, const T *gR, T s, const T *ob, T *ULb,
+                                          T *URb, T *gb) {
+  const T mu = (T)P.mu;
+  T gr[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) gr[k] = (T)0.5 * (gL[k] + gR[k]);
+  const T div = gr[0] + gr[4] + gr[8];
+  T ta[3]; /* tau[a][i] */
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    ta[i] = i == a ? mu * ((T)2.0 * gr[4 * a] - ((T)2.0 / (T)3.0) * div)
+                   : mu * (gr[3 * a + i] + gr[3 * i + a]);
+  const T irL = (T)1 / UL[0], irR = (T)1 / UR[0];
+  T uL[3], uR[3], uf[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    uL[i] = UL[1 + i] * irL;
+    uR[i] = UR[1 + i] * irR;
+    uf[i] = (T)0.5 * (uL[i] + uR[i]);
+  }
+  T Fb[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Fb[v] = ob[v] * s;
+  const bool work = P.visc_work != 0;
+  T tab[3], ufb[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    tab[i] = -Fb[1 + i] - (work ? Fb[4] * uf[i] : (T)0);
+    ufb[i] = work ? -Fb[4] * ta[i] : (T)0;
+  }
+  T grb[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) grb[k] = (T)0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i == a) {
+      grb[4 * a] += (T)2.0 * mu * tab[i];
+      const T d = -((T)2.0 / (T)3.0) * mu * tab[i];
+      grb[0] += d;
+      grb[4] += d;
+      grb[8] += d;
+    } else {
+      grb[3 * a + i] += mu * tab[i];
+      grb[3 * i + a] += mu * tab[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) gb[k] = (T)0.5 * grb[k];
+  /* uf = (uL + uR) / 2, u_i = q_{1+i} / rho */
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T h = (T)0.5 * ufb[i];
+    ULb[1 + i] += h * irL;
+    ULb[0] -= h * uL[i] * irL;
+    URb[1 + i] += h * irR;
+    URb[0] -= h * uR[i] * irR;
+  }
+}
+
 /* bars of the primitive state (r, u, e, p, c) -> bars of the conserved vector */
 template <typename T>
 FV_HD __forceinline__ void state_ideal_vjp(const DevParams &P, const FState<T> &s, T rb, T ub[3],

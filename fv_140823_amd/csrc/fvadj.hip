@@ -18,8 +18,12 @@
  * straight into a segment store that the reverse sweep reads in place.
  * Tangent (forward, for the dot-product check): the same stage kernels on
  * forward-mode dual numbers for the face flux and analytic limiter partials.
- * Scope: model 1 (ideal/stiffened gas), inviscid, MUSCL (either limiter) or
- * first order; bc outflow/periodic (an inlet ghost is a constant).
+ * Navier-Stokes: the face VJP also returns the bars of both cells' momentum
+ * gradients (visc_vjp_ideal), k_adj_sg gathers them per cell, and
+ * k_adj_recv_stage adds the transpose of the Green-Gauss gradient (gradcalc.c:
+ * 23-42): the cell's own term and its face neighbours' terms.
+ * Scope: model 1 (ideal/stiffened gas), Euler or Navier-Stokes, MUSCL (either
+ * limiter) or first order; bc outflow/periodic (an inlet ghost is a constant).
  */
 #include <cmath>
 
@@ -36,6 +40,9 @@ struct fv_adj_state {
   double *lam = nullptr, *lamSN = nullptr, *self = nullptr;
   double *Pp = nullptr, *Pm = nullptr;
   void *lamUL = nullptr, *lamUR = nullptr;
+  void *lamG = nullptr;      /* [nfp][9] gradient bars per face side (Navier-Stokes) */
+  double *SGb = nullptr;     /* [9][ns] gradient bars per cell */
+  double *SGmd = nullptr;    /* [9][ns] tangent of the momentum gradients */
   double *Jsteps = nullptr, *Jdsteps = nullptr, *part = nullptr; /* (hi, lo) pairs */
   double J_hi = 0.0, J_lo = 0.0; /* objective of the last rollout, unevaluated sum */
   int part_cap = 0;
@@ -46,7 +53,8 @@ void fv_adj_release(fvg_ctx *c) {
   if (!c || !c->adj) return;
   fv_adj_state *a = c->adj;
   void *ptrs[] = {a->mask, a->ckpts, a->seg, a->lam, a->lamSN, a->self, a->Pp, a->Pm, a->lamUL,
-                  a->lamUR, a->Jsteps, a->Jdsteps, a->part, a->Sd, a->SNd, a->SFd, a->Gd};
+                  a->lamUR, a->lamG, a->SGb, a->SGmd, a->Jsteps, a->Jdsteps, a->part, a->Sd, a->SNd,
+                  a->SFd, a->Gd};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete a;
@@ -63,11 +71,12 @@ __device__ __forceinline__ double cell_vol(const DevMesh &M, int i) {
  * mu_i = cdt lam_i / Vol_i.  The face states are rebuilt exactly as the forward
  * face kernel builds them (recon_face, recon_regular for regular faces), so the
  * VJP is taken at the forward's own face states in either precision. */
-template <int a, typename T, typename R, bool REG>
+template <int a, typename T, typename R, bool REG, bool NS>
 __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams &P, int j,
                                               const double *__restrict__ S,
                                               const double *__restrict__ lam, double cdt,
-                                              R *__restrict__ lUL, R *__restrict__ lUR) {
+                                              const R *__restrict__ SGm, R *__restrict__ lUL,
+                                              R *__restrict__ lUR, R *__restrict__ lG) {
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
   const int fl = meta & 7, fr = (meta >> 3) & 7, bc = REG ? 0 : (meta >> 6) & 3,
@@ -97,6 +106,19 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
     ULb[v] = URb[v] = (T)0;
   }
   hllc_vjp_ideal<a, T>(P, UL, UR, s, ob, ULb, URb);
+  if constexpr (NS) {
+    /* a boundary face's other side is its own cell (face_R = face_L): both
+     * gradient bars belong to it, which the per-cell gather counts twice */
+    T gL[9], gR[9], gb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      gL[k] = (T)SGm[(size_t)k * M.ns + iL];
+      gR[k] = (T)SGm[(size_t)k * M.ns + iR];
+    }
+    visc_vjp_ideal<a, T>(P, UL, UR, gL, gR, s, ob, ULb, URb, gb);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lG[(size_t)j * 9 + k] = (R)gb[k];
+  }
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     if (bc == 2) { ULb[v] += URb[v]; URb[v] = (T)0; }
@@ -106,26 +128,55 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
   }
 }
 
-template <typename T, typename R>
+template <typename T, typename R, bool NS>
 __global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_face(
     DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
-    const double *__restrict__ S, const double *__restrict__ lam, double cdt, R *__restrict__ lUL,
-    R *__restrict__ lUR) {
+    const double *__restrict__ S, const double *__restrict__ lam, double cdt,
+    const R *__restrict__ SGm, R *__restrict__ lUL, R *__restrict__ lUR, R *__restrict__ lG) {
   const int lb = logical_block();
   const int t = threadIdx.x;
   if (lb < b0) {
     const int q = lb * FV_BLOCK + t;
-    if (q < r0) adj_face_body<0, T, R, true>(M, P, q, S, lam, cdt, lUL, lUR);
-    else if (q < n0) adj_face_body<0, T, R, false>(M, P, q, S, lam, cdt, lUL, lUR);
+    if (q < r0) adj_face_body<0, T, R, true, NS>(M, P, q, S, lam, cdt, SGm, lUL, lUR, lG);
+    else if (q < n0) adj_face_body<0, T, R, false, NS>(M, P, q, S, lam, cdt, SGm, lUL, lUR, lG);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < r1) adj_face_body<1, T, R, true>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
-    else if (q < n1) adj_face_body<1, T, R, false>(M, P, n0 + q, S, lam, cdt, lUL, lUR);
+    if (q < r1) adj_face_body<1, T, R, true, NS>(M, P, n0 + q, S, lam, cdt, SGm, lUL, lUR, lG);
+    else if (q < n1) adj_face_body<1, T, R, false, NS>(M, P, n0 + q, S, lam, cdt, SGm, lUL, lUR, lG);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < r2) adj_face_body<2, T, R, true>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
-    else if (q < n2) adj_face_body<2, T, R, false>(M, P, n0 + n1 + q, S, lam, cdt, lUL, lUR);
+    if (q < r2)
+      adj_face_body<2, T, R, true, NS>(M, P, n0 + n1 + q, S, lam, cdt, SGm, lUL, lUR, lG);
+    else if (q < n2)
+      adj_face_body<2, T, R, false, NS>(M, P, n0 + n1 + q, S, lam, cdt, SGm, lUL, lUR, lG);
   }
+}
+
+/* bar of each cell's momentum gradients: the sum over its face slots of the
+ * faces' gradient bars (twice on a boundary face, whose two sides are the cell) */
+template <typename R>
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_sg(DevMesh M, const R *__restrict__ lG,
+                                                     double *__restrict__ SGb) {
+  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  if (i >= M.N) return;
+  const uint32_t cf = M.cflag[i];
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int g = M.gslot[(size_t)f * M.ns + i];
+    if (g < 0) continue;
+    const int cls = (cf >> (2 * f)) & 3;
+    const int jf = g >> 1;
+    const int nk = cls == CLS_FINER ? 4 : 1;
+    const double w = cls == CLS_BOUNDARY ? 2.0 : 1.0;
+    for (int k = 0; k < nk; ++k)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] += w * (double)lG[(size_t)(jf + k) * 9 + q];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) SGb[(size_t)k * M.ns + i] = acc[k];
 }
 
 /* bar of the face state SF[f] of cell i, gathered from the face(s) of slot f */
@@ -233,10 +284,12 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
  *   mode 0 (last stage):  lamSN  = a lam;  lam = b lam + nu
  *   mode 1:               lamSN += a lam;  lam = b lam + nu
  *   mode 2 (first stage): lam = lamSN + a lam + nu */
+template <bool NS>
 __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M,
                                                              const double *__restrict__ self,
                                                              const double *__restrict__ Pp,
                                                              const double *__restrict__ Pm,
+                                                             const double *__restrict__ SGb,
                                                              double *__restrict__ lam,
                                                              double *__restrict__ lamSN, double a,
                                                              double b, int mode) {
@@ -272,6 +325,44 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M,
           for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + q];
         }
       }
+    }
+  }
+  if constexpr (NS) {
+    /* transpose of SG_i[c][d] = sum_f sum_k (U_i + U_nb)_{1+c} n_d A / (2 Vol_i) (gradcalc.c:
+     * 33-37): the cell's own term over its faces and sub-faces, plus, from each face
+     * neighbour nb, the coefficient nb's gradient gives this cell (normal -n, the
+     * sub-face area of the finer side, nb's volume).  A transmissive ghost is the cell
+     * itself (its term counts twice); an inlet ghost is a constant. */
+    const int L = (cf >> 16) & 0xff;
+    const double Vi = M.vol[L];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int d = c_face_axis[f];
+      const double sf = (double)c_face_sign[f];
+      const int cls = (cf >> (2 * f)) & 3;
+      const int nb = M.nbr[(size_t)f * M.ns + i];
+      double own;
+      if (cls == CLS_SAME || cls == CLS_COARSER) {
+        const double A = M.area[3 * L + d];
+        own = 0.5 * sf * A / Vi;
+        const double cn = 0.5 * (-sf) * A / M.vol[(M.cflag[nb] >> 16) & 0xff];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[1 + c] += cn * SGb[(size_t)(3 * c + d) * M.ns + nb];
+      } else if (cls == CLS_FINER) {
+        const double A = M.area[3 * (L + 1) + d];
+        own = 4.0 * (0.5 * sf * A / Vi);
+        const double cn = 0.5 * (-sf) * A / M.vol[L + 1];
+        for (int k = 0; k < 4; ++k) {
+          const int q = M.quad[4 * nb + k];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[1 + c] += cn * SGb[(size_t)(3 * c + d) * M.ns + q];
+        }
+      } else {
+        const double A = M.area[3 * L + d];
+        own = ((-1 - nb) == 1 ? 0.5 : 1.0) * sf * A / Vi;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[1 + c] += own * SGb[(size_t)(3 * c + d) * M.ns + i];
     }
   }
 #pragma unroll
@@ -427,12 +518,15 @@ __global__ __launch_bounds__(FV_BLOCK) void k_tlm_recon(DevMesh M, DevParams P,
   }
 }
 
-/* tangent of the face flux by forward-mode dual numbers */
-template <int a>
+/* tangent of the face flux by forward-mode dual numbers (NS: the momentum
+ * gradients SGm and their tangents SGmd enter as dual numbers too) */
+template <int a, bool NS>
 __device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams &P, int j,
                                               const double *__restrict__ S,
                                               const double *__restrict__ Sd, const double *SF,
-                                              const double *SFd, double *__restrict__ Gd) {
+                                              const double *SFd, const double *__restrict__ SGm,
+                                              const double *__restrict__ SGmd,
+                                              double *__restrict__ Gd) {
   typedef Dual<double> D;
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
@@ -455,31 +549,41 @@ __device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams 
   FState<D> L, R;
   make_state_ideal<D>(P, UL, L);
   make_state_ideal<D>(P, UR, R);
+  if constexpr (NS) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      L.g[k] = D(SGm[(size_t)k * M.ns + iL], SGmd[(size_t)k * M.ns + iL]);
+      R.g[k] = D(SGm[(size_t)k * M.ns + iR], SGmd[(size_t)k * M.ns + iR]);
+    }
+  }
   D out[5];
-  hllc_face<1, false, a, D>(P, L, R, D((double)c_face_sign[fl]), out);
+  hllc_face<1, NS, a, D>(P, L, R, D((double)c_face_sign[fl]), out);
   const double A = M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < 5; ++v) Gd[(size_t)j * 5 + v] = -(A)*out[v].d;
 }
 
+template <bool NS>
 __global__ __launch_bounds__(FV_BLOCK) void k_tlm_face(DevMesh M, DevParams P, int b0, int b1,
                                                        int n0, int n1, int n2,
                                                        const double *__restrict__ S,
                                                        const double *__restrict__ Sd,
                                                        const double *__restrict__ SF,
                                                        const double *__restrict__ SFd,
+                                                       const double *__restrict__ SGm,
+                                                       const double *__restrict__ SGmd,
                                                        double *__restrict__ Gd) {
   const int lb = logical_block();
   const int t = threadIdx.x;
   if (lb < b0) {
     const int j = lb * FV_BLOCK + t;
-    if (j < n0) tlm_face_body<0>(M, P, j, S, Sd, SF, SFd, Gd);
+    if (j < n0) tlm_face_body<0, NS>(M, P, j, S, Sd, SF, SFd, SGm, SGmd, Gd);
   } else if (lb < b0 + b1) {
     const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) tlm_face_body<1>(M, P, n0 + q, S, Sd, SF, SFd, Gd);
+    if (q < n1) tlm_face_body<1, NS>(M, P, n0 + q, S, Sd, SF, SFd, SGm, SGmd, Gd);
   } else {
     const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) tlm_face_body<2>(M, P, n0 + n1 + q, S, Sd, SF, SFd, Gd);
+    if (q < n2) tlm_face_body<2, NS>(M, P, n0 + n1 + q, S, Sd, SF, SFd, SGm, SGmd, Gd);
   }
 }
 
@@ -490,8 +594,8 @@ static inline int nblk(int n) { return (n + FV_BLOCK - 1) / FV_BLOCK; }
 static int adj_check(fvg_ctx *c) {
   if (!c) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
-  if (c->prm.model != 1 || c->prm.ns)
-    return set_err(c, FVG_ERR_UNSUPPORTED, "adjoint/tangent: model 1, inviscid only");
+  if (c->prm.model != 1)
+    return set_err(c, FVG_ERR_UNSUPPORTED, "adjoint/tangent: model 1 (ideal/stiffened gas) only");
   return FVG_OK;
 }
 
@@ -510,6 +614,9 @@ static int adj_alloc(fvg_ctx *c) {
   HIPCHK(c, hipMalloc(&a->Pm, 3 * pl));
   HIPCHK(c, hipMalloc(&a->lamUL, (size_t)5 * c->nfp * c->rsize));
   HIPCHK(c, hipMalloc(&a->lamUR, (size_t)5 * c->nfp * c->rsize));
+  HIPCHK(c, hipMalloc(&a->lamG, (size_t)9 * c->nfp * c->rsize));
+  HIPCHK(c, hipMalloc(&a->SGb, (size_t)9 * c->ns * 8));
+  HIPCHK(c, hipMemsetAsync(a->SGb, 0, (size_t)9 * c->ns * 8, c->stream));
   a->part_cap = nblk(c->N);
   HIPCHK(c, hipMalloc(&a->part, (size_t)4 * a->part_cap * 8));
   return FVG_OK;
@@ -533,24 +640,44 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
   DevMesh M = dev_mesh(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblk(n0), b1 = nblk(n1), b2 = nblk(n2);
+  const int r0 = c->nreg[0], r1 = c->nreg[1], r2 = c->nreg[2];
   const int nc = nblk(c->N);
+  const bool ns = c->prm.ns != 0;
+  if (ns) fv_launch_grad(c, S); /* the momentum gradients of the stage's input, into c->SGm */
+  dim3 gf(b0 + b1 + b2), b(FV_BLOCK);
   if (c->rsize == 8) {
-    hipLaunchKernelGGL((k_adj_face<double, double>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, a->lam,
-                       cdt, (double *)a->lamUL,
-                       (double *)a->lamUR);
-    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
-                       (const double *)a->lamUL, (const double *)a->lamUR, a->self, a->Pp, a->Pm);
+    const double *SGm = (const double *)c->SGm;
+    double *lUL = (double *)a->lamUL, *lUR = (double *)a->lamUR, *lG = (double *)a->lamG;
+    if (ns)
+      hipLaunchKernelGGL((k_adj_face<double, double, true>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+                         n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
+    else
+      hipLaunchKernelGGL((k_adj_face<double, double, false>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+                         n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
+    if (ns)
+      hipLaunchKernelGGL(k_adj_sg<double>, dim3(nc), b, 0, c->stream, M, (const double *)lG, a->SGb);
+    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), b, 0, c->stream, M, c->dp, S,
+                       (const double *)lUL, (const double *)lUR, a->self, a->Pp, a->Pm);
   } else {
-    hipLaunchKernelGGL((k_adj_face<float, float>), dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream,
-                       M, c->dp, b0, b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, a->lam,
-                       cdt, (float *)a->lamUL,
-                       (float *)a->lamUR);
-    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, S,
-                       (const float *)a->lamUL, (const float *)a->lamUR, a->self, a->Pp, a->Pm);
+    const float *SGm = (const float *)c->SGm;
+    float *lUL = (float *)a->lamUL, *lUR = (float *)a->lamUR, *lG = (float *)a->lamG;
+    if (ns)
+      hipLaunchKernelGGL((k_adj_face<float, float, true>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+                         n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
+    else
+      hipLaunchKernelGGL((k_adj_face<float, float, false>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+                         n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
+    if (ns)
+      hipLaunchKernelGGL(k_adj_sg<float>, dim3(nc), b, 0, c->stream, M, (const float *)lG, a->SGb);
+    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), b, 0, c->stream, M, c->dp, S,
+                       (const float *)lUL, (const float *)lUR, a->self, a->Pp, a->Pm);
   }
-  hipLaunchKernelGGL(k_adj_recv_stage, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, a->self, a->Pp,
-                     a->Pm, a->lam, a->lamSN, ca, cb, mode);
+  if (ns)
+    hipLaunchKernelGGL(k_adj_recv_stage<true>, dim3(nc), b, 0, c->stream, M, a->self, a->Pp, a->Pm,
+                       (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode);
+  else
+    hipLaunchKernelGGL(k_adj_recv_stage<false>, dim3(nc), b, 0, c->stream, M, a->self, a->Pp, a->Pm,
+                       (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode);
 }
 
 /* Segment store: 3 ckpt + 1 states.  Step m of a segment has its stage inputs at
@@ -712,6 +839,7 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
     HIPCHK(c, hipMalloc(&a->SNd, pl));
     HIPCHK(c, hipMalloc(&a->SFd, 6 * pl));
     HIPCHK(c, hipMalloc(&a->Gd, (size_t)5 * c->nfp * 8));
+    HIPCHK(c, hipMalloc(&a->SGmd, (size_t)9 * c->ns * 8));
     HIPCHK(c, hipMemsetAsync(a->SFd, 0, 6 * pl, c->stream));
   }
   if (a->Jsteps) (void)hipFree(a->Jsteps);
@@ -739,8 +867,17 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
         hipLaunchKernelGGL(k_tlm_recon, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp, c->S, a->Sd,
                            a->SFd);
       }
-      hipLaunchKernelGGL(k_tlm_face, dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream, M, c->dp, b0,
-                         b1, n0, n1, n2, c->S, a->Sd, (const double *)c->SF, a->SFd, a->Gd);
+      if (c->prm.ns) {
+        fv_launch_grad(c, c->S);
+        fv_launch_grad_tangent(c, a->Sd, a->SGmd);
+        hipLaunchKernelGGL(k_tlm_face<true>, dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream, M,
+                           c->dp, b0, b1, n0, n1, n2, c->S, a->Sd, (const double *)c->SF, a->SFd,
+                           (const double *)c->SGm, (const double *)a->SGmd, a->Gd);
+      } else {
+        hipLaunchKernelGGL(k_tlm_face<false>, dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream, M,
+                           c->dp, b0, b1, n0, n1, n2, c->S, a->Sd, (const double *)c->SF, a->SFd,
+                           (const double *)nullptr, (const double *)nullptr, a->Gd);
+      }
       /* tangent stage update is the same linear combination (k_gather_rk on tangents) */
       fv_launch_gather_ptrs(c, a->Gd, a->Sd, a->SNd, k, dt, k == 1);
       fv_stage(c, k, dt, k == 1, false);
@@ -822,6 +959,56 @@ int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL,
   else if (axis == 1) hllc_face<1, false, 1, D>(P, L, R, s, o);
   else hllc_face<1, false, 2, D>(P, L, R, s, o);
   for (int v = 0; v < 5; ++v) outd[v] = o[v].d;
+  return FVG_OK;
+}
+
+/* the Navier-Stokes face flux (HLLC + viscous flux; gL, gR the 9 momentum gradients
+ * of each side) and its VJP, including the bars of the gradients (gb, per side) */
+int fvg_face_flux_ns_host(const fvg_params *p, int axis, int sign, const double *UL,
+                          const double *UR, const double *gL, const double *gR, double *out) {
+  if (!p || axis < 0 || axis > 2 || !UL || !UR || !gL || !gR || !out || p->model != 1)
+    return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  P.gamma = p->gamma;
+  P.pinf = p->pinf;
+  P.mu = p->mu;
+  P.visc_work = p->visc_work;
+  FState<double> L, R;
+  make_state_ideal<double>(P, UL, L);
+  make_state_ideal<double>(P, UR, R);
+  for (int k = 0; k < 9; ++k) { L.g[k] = gL[k]; R.g[k] = gR[k]; }
+  const double s = sign < 0 ? -1.0 : 1.0;
+  if (axis == 0) hllc_face<1, true, 0, double>(P, L, R, s, out);
+  else if (axis == 1) hllc_face<1, true, 1, double>(P, L, R, s, out);
+  else hllc_face<1, true, 2, double>(P, L, R, s, out);
+  return FVG_OK;
+}
+
+int fvg_face_vjp_ns_host(const fvg_params *p, int axis, int sign, const double *UL,
+                         const double *UR, const double *gL, const double *gR, const double *ob,
+                         double *ULb, double *URb, double *gb) {
+  if (!p || axis < 0 || axis > 2 || !UL || !UR || !gL || !gR || !ob || !ULb || !URb || !gb ||
+      p->model != 1)
+    return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  P.gamma = p->gamma;
+  P.pinf = p->pinf;
+  P.mu = p->mu;
+  P.visc_work = p->visc_work;
+  for (int v = 0; v < 5; ++v) ULb[v] = URb[v] = 0.0;
+  const double s = sign < 0 ? -1.0 : 1.0;
+  if (axis == 0) {
+    hllc_vjp_ideal<0, double>(P, UL, UR, s, ob, ULb, URb);
+    visc_vjp_ideal<0, double>(P, UL, UR, gL, gR, s, ob, ULb, URb, gb);
+  } else if (axis == 1) {
+    hllc_vjp_ideal<1, double>(P, UL, UR, s, ob, ULb, URb);
+    visc_vjp_ideal<1, double>(P, UL, UR, gL, gR, s, ob, ULb, URb, gb);
+  } else {
+    hllc_vjp_ideal<2, double>(P, UL, UR, s, ob, ULb, URb);
+    visc_vjp_ideal<2, double>(P, UL, UR, gL, gR, s, ob, ULb, URb, gb);
+  }
   return FVG_OK;
 }
 

@@ -1328,6 +1328,16 @@ static void time_step(fvg_ctx *c, int method, double dt, bool write_rhs_last, hi
 
 void fv_launch_recon(fvg_ctx *c, const double *S) { dispatch_recon(c, S); }
 
+void fv_launch_grad(fvg_ctx *c, const double *S) { dispatch_grad(c, S, false); }
+
+void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd) {
+  DevMesh M = dev_mesh(c);
+  DevParams Pz = c->dp;
+  for (int v = 0; v < 5; ++v) Pz.inlet[v] = 0.0;
+  hipLaunchKernelGGL((k_grad<1, false, double>), dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
+                     M, Pz, Sd, (double *)nullptr, SGmd);
+}
+
 /* the adjoint's forward stages: always the FP64-read path, whose face states
  * the adjoint face kernel rebuilds (recon_face) */
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs) {

@@ -206,6 +206,14 @@ int fvg_face_flux_host(const fvg_params *p, int axis, int sign, const double *UL
                        double *out);
 int fvg_face_vjp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
                       const double *ob, double *ULb, double *URb);
+/* The same with the viscous flux (model 1, ns): gL / gR are each side's 9 momentum gradients
+ * (d(rho u_i)/dx_j at [3 i + j]); the VJP also returns gb, the bar of either side's gradients
+ * (the face uses their mean, so both sides get the same bar). */
+int fvg_face_flux_ns_host(const fvg_params *p, int axis, int sign, const double *UL,
+                          const double *UR, const double *gL, const double *gR, double *out);
+int fvg_face_vjp_ns_host(const fvg_params *p, int axis, int sign, const double *UL,
+                         const double *UR, const double *gL, const double *gR, const double *ob,
+                         double *ULb, double *URb, double *gb);
 int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
                       const double *ULd, const double *URd, double *outd);
 int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc, double dm1cc,

@@ -126,3 +126,65 @@ def test_limiter_partials(limiter):
             continue
         assert abs(g0.value - f0) <= 1e-6 * max(1.0, abs(f0))
         assert abs(g1.value - f1) <= 1e-6 * max(1.0, abs(f1))
+
+
+def flux_ns(p, axis, sign, UL, UR, gL, gR):
+    out = np.zeros(5)
+    assert L.fvg_face_flux_ns_host(C.byref(p), axis, sign, _d(UL), _d(UR), _d(gL), _d(gR), _d(out)) == 0
+    return out
+
+
+def vjp_ns(p, axis, sign, UL, UR, gL, gR, ob):
+    a, b, g = np.zeros(5), np.zeros(5), np.zeros(9)
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (UL, UR, gL, gR, ob)]
+    assert L.fvg_face_vjp_ns_host(C.byref(p), axis, sign, *[_d(x) for x in arrs], a.ctypes.data_as(D),
+                                  b.ctypes.data_as(D), g.ctypes.data_as(D)) == 0
+    return a, b, g
+
+
+@pytest.mark.parametrize("work", [0, 1])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_viscous_flux_closed_form(work, axis):
+    """The viscous part of the device NS face flux is -s (0, tau[a][0..2], uf . tau[a])
+    (flux_viscous.c:199-232), tau = mu (G + G^T - 2/3 tr(G) I) of the mean gradient."""
+    rng = np.random.default_rng(11 + axis + 3 * work)
+    p = fv.params(1, ns=1, mu=0.05, visc_work=work)
+    UL, UR = _states("subsonic_left_star", rng)
+    gL, gR = rng.standard_normal(9), rng.standard_normal(9)
+    for sign in (1, -1):
+        diff = flux_ns(p, axis, sign, UL, UR, gL, gR) - flux(p, axis, sign, UL, UR)
+        G = 0.5 * (gL + gR).reshape(3, 3)
+        tau = 0.05 * (G + G.T - 2.0 / 3.0 * np.trace(G) * np.eye(3))
+        uf = 0.5 * (UL[1:4] / UL[0] + UR[1:4] / UR[0])
+        want = -sign * np.concatenate([[0.0], tau[axis], [work * uf @ tau[axis]]])
+        assert np.allclose(diff, want, rtol=1e-12, atol=1e-14), (diff, want)
+
+
+@pytest.mark.parametrize("regime", ["subsonic_left_star", "supersonic_right"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("sign", [1, -1])
+@pytest.mark.parametrize("work", [0, 1])
+def test_ns_face_vjp_matches_finite_differences(regime, axis, sign, work):
+    """VJP of the Navier-Stokes face flux (HLLC + viscous flux) against central
+    differences in all 28 inputs: both states and both sides' 9 gradients."""
+    rng = np.random.default_rng(hash((regime, axis, sign, work)) % 2**32)
+    p = fv.params(1, ns=1, mu=0.05, visc_work=work)
+    UL, UR = _states(regime, rng)
+    UL, UR = _permute(UL, axis), _permute(UR, axis)
+    if sign < 0:
+        UL, UR = UR, UL
+    gL, gR = rng.standard_normal(9), rng.standard_normal(9)
+    ob = rng.standard_normal(5)
+    bL, bR, bg = vjp_ns(p, axis, sign, UL, UR, gL, gR, ob)
+    X = [UL, UR, gL, gR]
+    bars = [bL, bR, bg, bg]
+    for side in range(4):
+        for m in range(len(X[side])):
+            h = 1e-6 * max(1.0, abs(X[side][m]))
+            Xp = [x.copy() for x in X]
+            Xm = [x.copy() for x in X]
+            Xp[side][m] += h
+            Xm[side][m] -= h
+            fd = ob @ (flux_ns(p, axis, sign, *Xp) - flux_ns(p, axis, sign, *Xm)) / (2 * h)
+            g = bars[side][m]
+            assert abs(g - fd) <= 1e-6 * max(1.0, abs(fd)), (side, m, g, fd)

@@ -1,4 +1,5 @@
-"""GPU checks of the discrete adjoint of RHS + RK rollouts (fvg_adj_*):
+"""GPU checks of the discrete adjoint of RHS + RK rollouts (fvg_adj_*), Euler and
+Navier-Stokes:
   - <dJ/dS0, v> against the forward-mode tangent (fvg_tlm) of the same rollout,
     two independent derivative codes (hand-written reverse vs dual numbers);
   - against central finite differences of J (FP64, eps = 1e-6 relative);
@@ -179,3 +180,76 @@ def test_mixed_adjoint_close_to_plain():
     e = np.abs(lm - lp).max() / np.abs(lp).max()
     print("mixed adjoint rel Linf", e, flush=True)
     assert e < 1e-4
+
+
+NS_CASES = [
+    # (mesh, limiter, ischeme, visc_work)
+    ("vortex_patch", 1, 2, 1),   # periodic, 2:1 hanging faces, van Albada
+    ("amr_outflow", 0, 2, 1),    # outflow sides: ghost terms of the gradient transpose
+    ("vortex_patch", 1, 1, 0),   # first order, no viscous work
+]
+
+
+def _ns_problem(mesh):
+    if mesh == "vortex_patch":
+        F, probe = _vortex(4, True)
+    else:
+        F = amr(3, 5)
+        xc, _ = F.leaf_info()
+        F.set_state(state(1, xc, seed=23, jump=False))
+        probe = (xc[:, 0] > 0.6).astype(np.uint8)
+    return F, probe
+
+
+@pytest.mark.parametrize("mesh,limiter,ischeme,work", NS_CASES)
+def test_ns_adjoint_matches_tangent(mesh, limiter, ischeme, work):
+    """Navier-Stokes rollouts: the adjoint (HLLC + viscous VJP, gradient transpose)
+    against the dual-number tangent of the same rollout, three random directions.
+    mu is raised so the viscous terms carry a sizeable share of the flux."""
+    print(f"case NS adjoint vs tangent mesh={mesh} limiter={limiter} ischeme={ischeme} work={work}",
+          flush=True)
+    F, probe = _ns_problem(mesh)
+    mu = 0.05 if mesh == "vortex_patch" else 2e-3
+    g = _solver(F, probe, limiter=limiter, ischeme=ischeme, ns=1, mu=mu, visc_work=work)
+    S0 = F.get_state()
+    dt = 0.3 * g.cfl_dt(0.4)
+    J = g.adj_forward(5, 2, dt)
+    lam = g.adj_backward()
+    rng = np.random.default_rng(2)
+    for k in range(3):
+        v = rng.standard_normal(S0.shape) * np.abs(S0).max(axis=1, keepdims=True) * 1e-2
+        g.set_state(S0)
+        J2, dJ = g.tlm(v, 5, dt)
+        assert abs(J2 - J) <= 1e-13 * abs(J)
+        adj = float((lam * v).sum())
+        print(f"  dir {k}: adjoint {adj:.15e} tangent {dJ:.15e}", flush=True)
+        assert abs(adj - dJ) <= 1e-10 * max(abs(dJ), 1e-300), (adj, dJ)
+
+
+def test_ns_adjoint_matches_central_fd():
+    """Navier-Stokes rollout on the vortex patch mesh, 8 seeded smooth directions,
+    central differences with eps = 1e-6 on the double-double objective."""
+    print("case NS adjoint vs central FD", flush=True)
+    F, probe = _vortex(4, True)
+    g = _solver(F, probe, limiter=1, ns=1, mu=0.05, visc_work=1)
+    S0 = F.get_state()
+    xc, _ = F.leaf_info()
+    dt = 0.3 * g.cfl_dt(0.4)
+    g.set_state(S0)
+    g.adj_forward(6, 3, dt)
+    lam = g.adj_backward()
+    eps = 1e-6
+
+    def J(S):
+        g.set_state(S)
+        g.adj_forward(6, 3, dt)
+        return g.adj_objective()
+
+    for k in range(8):
+        v = _smooth_direction(100 + k, xc, S0.shape)
+        (ph, pl), (mh, ml) = J(S0 + eps * v), J(S0 - eps * v)
+        fd = ((ph - mh) + (pl - ml)) / (2 * eps)
+        adj = float((lam * v).sum())
+        rel = abs(adj - fd) / abs(fd)
+        print(f"  dir {k}: adjoint {adj:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-6, (adj, fd)
