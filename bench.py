@@ -115,18 +115,21 @@ def build_workload(name, evolve, quiet, level=None, refine=0.01):
     return F, p, meta
 
 
-def measured_traffic(workload, kernel):
+def measured_traffic(workload, kernel, grid=None):
     """HBM bytes per launch of `kernel` on `workload` (the workload name, e.g.
     "blast_32^3+2") from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     profiles/traffic.json, already scaled by the copy-kernel calibration; None if
-    that workload was not profiled."""
+    that workload was not profiled or was profiled on a mesh of another size
+    (launch grid `grid`, threads)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     k = d.get(workload, {}).get("kernels", {}).get(kernel)
-    return None if k is None else int(k["read_bytes"] + k["write_bytes"])
+    if k is None or (grid is not None and k.get("grid") != grid):
+        return None
+    return int(k["read_bytes"] + k["write_bytes"])
 
 
 def mesh_facts(F):
@@ -135,8 +138,10 @@ def mesh_facts(F):
     A = fl.arrays()
     Fb = int(((A["face_meta"] >> 6) & 3 != 0).sum())
     _, lev = F.leaf_info()
+    rnd = lambda n: (n + 255) // 256 * 256
     return dict(N=fl.ncell, F=fl.nface, Q=int(fl.mesh.nquad), Fb=Fb,
-                levels={int(l): int(c) for l, c in zip(*np.unique(lev, return_counts=True))})
+                levels={int(l): int(c) for l, c in zip(*np.unique(lev, return_counts=True))},
+                grid_face=sum(rnd(int(n)) for n in fl.mesh.nface_axis), grid_cell=rnd(fl.ncell))
 
 
 def time_form(F, p, steps, warmup, dt):
@@ -243,38 +248,49 @@ def main():
     cu = mf["N"] * stages * a.steps
     val_p, val_m = cu / wall_p, cu / wall_m
     log(f"plain {val_p:.4g} cell-upd/s, mixed {val_m:.4g} cell-upd/s", a.quiet)
-    # dominant kernel of the plain form (largest time per step)
-    per_step = {KERNELS[i]: k_p[i] for i in range(4) if k_p[i] > 0}
-    dom = max(per_step, key=per_step.get)
-    di = KERNELS.index(dom)
+    # Roofline of one RK stage of the plain form (its two launches, face + gather)
+    # against the reference data flow's bytes per cell-update (B, ref_bytes_per_cell):
+    # achieved = B x leaves / (mean face + mean gather duration, HIP events);
+    # traffic = the two launches' HBM bytes from the committed rocprof counters.
     fpc = mf["F"] / mf["N"]
-    nb = ref_bytes_per_cell(dom, neq, fpc, 8, p.ns) * mf["N"]
-    achieved = nb / (k_p[di] * 1e-3) / 1e9
-    kname = {"face": "k_face_fused" if p.ischeme == 2 else "k_face"}.get(dom, f"k_{dom}")
-    comp = compulsory_bytes(dom, mf, neq, 8, p.ns, False)
-    roof = dict(bound="hbm", kernel=kname, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                traffic=measured_traffic(meta["name"], kname),
-                alg_bytes_per_launch=int(nb), avg_launch_ms=round(k_p[di], 5),
-                compulsory_bytes_per_launch=int(comp),
-                compulsory_gbs=round(comp / (k_p[di] * 1e-3) / 1e9, 1),
+    stage_kernels = ["grad", "face", "gather_rk"] if p.ns else ["face", "gather_rk"]
+    t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)
+    B = ref_bytes_per_cell("stage", neq, fpc, 8, p.ns)
+    achieved = B * mf["N"] / (t_stage * 1e-3) / 1e9
+    fname = "k_face_fused" if p.ischeme == 2 else "k_face"
+    tf = measured_traffic(meta["name"], fname, mf["grid_face"])
+    tg = measured_traffic(meta["name"], "k_gather_rk", mf["grid_cell"])
+    traffic = tf + tg if (tf is not None and tg is not None and not p.ns) else None
+    comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False) for k in stage_kernels)
+    roof = dict(bound="hbm", kernel=f"RK stage: {fname} + k_gather_rk" + (" + k_grad" if p.ns else ""),
+                achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_gbs=round(traffic / (t_stage * 1e-3) / 1e9, 1) if traffic else None,
+                alg_bytes_per_launch=int(B * mf["N"]), alg_bytes_per_cell_update=round(B, 1),
+                avg_launch_ms=round(t_stage, 5), compulsory_bytes_per_launch=int(comp),
                 faces_per_cell=round(fpc, 4))
-    # whole stage against the reference data flow: B x cell-updates/s
+    # the same against the wall clock of the timed steps, plain and mixed
     stage_roof = {}
     for tag, val, w in (("plain", val_p, 8), ("mixed", val_m, 4)):
-        B = ref_bytes_per_cell("stage", neq, fpc, w, p.ns)
-        stage_roof[tag] = dict(bytes_per_cell_update=round(B, 1), achieved_gbs=round(B * val / 1e9, 1),
-                               frac=round(B * val / 1e9 / HBM_PEAK_GBS, 4))
+        Bw = ref_bytes_per_cell("stage", neq, fpc, w, p.ns)
+        stage_roof[tag] = dict(bytes_per_cell_update=round(Bw, 1), achieved_gbs=round(Bw * val / 1e9, 1),
+                               frac=round(Bw * val / 1e9 / HBM_PEAK_GBS, 4))
+    # per kernel: mean durations, the actual HBM rate (rocprof bytes / duration) and
+    # the rate on the kernel's compulsory bytes
     kern = {}
     for i in range(4):
         if k_p[i] <= 0:
             continue
         k = KERNELS[i]
         e = dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
+        kn = {"face": (fname, mf["grid_face"]), "gather_rk": ("k_gather_rk", mf["grid_cell"])}.get(k)
+        tb = measured_traffic(meta["name"], *kn) if kn else None
+        if tb is not None:
+            e["plain_traffic_bytes"] = tb
+            e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
         if k != "recon":
-            for tag, ms, w, rs in (("plain", k_p[i], 8, 8), ("mixed", k_m[i], 4, 4)):
+            for tag, ms, rs in (("plain", k_p[i], 8), ("mixed", k_m[i], 4)):
                 if ms:
-                    e[f"{tag}_alg_gbs"] = round(ref_bytes_per_cell(k, neq, fpc, w, p.ns) * mf["N"] / (ms * 1e-3) / 1e9, 1)
                     e[f"{tag}_compulsory_gbs"] = round(compulsory_bytes(k, mf, neq, rs, p.ns, False) / (ms * 1e-3) / 1e9, 1)
         kern[k] = e
     acc = l1_rel_density(F, p, pm, a.acc_steps, dt)
@@ -296,8 +312,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"],
-                   "hanging_quads": mf["Q"], "levels": mf["levels"], "model": "ideal gas gamma 1.4",
-                   "scheme": "MUSCL-minmod + HLLC(Davis), SSP-RK3", "dt": dt,
+                   "hanging_quads": mf["Q"], "levels": mf["levels"],
+                   "model": {1: f"ideal gas gamma {p.gamma:g}", 2: "barotropic (model 2)",
+                             3: "two-phase barotropic (model 3)"}[p.model] + (" Navier-Stokes" if p.ns else ""),
+                   "scheme": ("MUSCL-" + ("minmod" if p.limiter == 0 else "van Albada") if p.ischeme == 2
+                              else "first order") + " + HLLC(Davis), SSP-RK3", "dt": dt,
                    **{k: v for k, v in meta.items() if k != "name"}},
         "mixed": {"value": round(val_m, 1), "ms_per_step": round(wall_m / a.steps * 1e3, 4),
                   "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},
