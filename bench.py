@@ -165,7 +165,10 @@ def time_form(F, p, steps, warmup, dt):
 
 
 def l1_rel_density(F, p_plain, p_mixed, steps, dt):
+    """Relative L1 (volume-weighted) of density and pressure, mixed vs plain, after
+    `steps` steps at the fixed dt."""
     import fv_140823_amd as fv
+    from fv_140823_amd.configs import pressure
     out = {}
     for tag, p in (("plain", p_plain), ("mixed", p_mixed)):
         g = fv.Solver.from_forest(F, p)
@@ -175,11 +178,12 @@ def l1_rel_density(F, p_plain, p_mixed, steps, dt):
     _, lev = F.leaf_info()
     V = 8.0 ** (-(lev - 1).astype(float))
     a, b = out["mixed"], out["plain"]
-    rho = float((np.abs(a[0] - b[0]) * V).sum() / (np.abs(b[0]) * V).sum())
-    from fv_140823_amd.configs import cons_to_prim
-    pa, pb = cons_to_prim(a)[4], cons_to_prim(b)[4]
-    pres = float((np.abs(pa - pb) * V).sum() / (np.abs(pb) * V).sum())
-    return dict(density=rho, pressure=pres, steps=steps)
+    l1 = lambda x, y: float((np.abs(x - y) * V).sum() / (np.abs(y) * V).sum())
+    res = dict(density=l1(a[0], b[0]), steps=steps)
+    pa, pb = pressure(a, p_plain), pressure(b, p_plain)
+    if pa is not None:
+        res["pressure"] = l1(pa, pb)
+    return res
 
 
 def cpu_baseline(F, p, dt, budget_s, quiet):

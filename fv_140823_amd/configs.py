@@ -35,6 +35,24 @@ def cons_to_prim(S, gamma=1.4, pinf=0.0):
     return rho, u, v, w, p
 
 
+def pressure(S, p):
+    """Cell pressure of a conserved state [neq][N] for the EOS of params p: the
+    ideal/stiffened gas (model 1) or the barotropic law (model 2, barotropic_eos.c:
+    13-22); None for the two-phase model 3 (its pressure needs the mixture root)."""
+    if p.model == 1:
+        rho = S[0]
+        ke = 0.5 * (S[1] ** 2 + S[2] ** 2 + S[3] ** 2) / rho
+        return (p.gamma - 1.0) * (S[4] - ke) - p.gamma * p.pinf
+    if p.model == 2:
+        rho = S[0]
+        out = np.empty_like(rho)
+        liq = rho >= p.baro_rsat
+        out[liq] = p.baro_psat + p.baro_beta * ((rho[liq] / p.baro_rsat) ** p.baro_nexp - 1.0)
+        out[~liq] = p.baro_psat + p.baro_cbar * (1.0 / p.baro_rsat - 1.0 / rho[~liq])
+        return out
+    return None
+
+
 def sod(level: int = 5, nroot=(1, 1, 1)):
     F = Forest(nroot, (0, 0, 0), (1, 1, 1), level, (2, 2, 2, 2, 2, 2), (0, 0, 0), 5)
     xc, _ = F.leaf_info()

@@ -70,8 +70,12 @@ FV_HD __forceinline__ bool eos64(const DevParams &P, const double *U, bool clip,
   } else if constexpr (MODEL == 2) {
     const double rsat = P.baro_rsat;
     if (r >= rsat) {
-      p = P.baro_psat + P.baro_beta * (pow((r / rsat), P.baro_nexp) - 1.0);
-      c = sqrt(P.baro_beta * P.baro_nexp * (pow((r / rsat), (P.baro_nexp - 1))) / rsat);
+      /* one pow: (r/rsat)^n = (r/rsat)^(n-1) (r/rsat) (the reference calls pow twice;
+       * the results agree to an ulp of the power) */
+      const double x = r / rsat;
+      const double xn1 = pow(x, (P.baro_nexp - 1));
+      p = P.baro_psat + P.baro_beta * (xn1 * x - 1.0);
+      c = sqrt(P.baro_beta * P.baro_nexp * xn1 / rsat);
     } else {
       p = P.baro_psat + P.baro_cbar * ((1.0 / rsat) - (1.0 / r));
       c = sqrt(P.baro_cbar / (r * r));

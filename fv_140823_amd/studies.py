@@ -102,14 +102,7 @@ def study_cavitation(steps=500, level=7):
     Sp, tp = _run(F, p, steps, dt)
     Sm, tm = _run(F, _mixed(p), steps, dt)
 
-    def pres(S):
-        rho = S[0]
-        out = np.empty_like(rho)
-        liq = rho >= p.baro_rsat
-        out[liq] = p.baro_psat + p.baro_beta * ((rho[liq] / p.baro_rsat) ** p.baro_nexp - 1.0)
-        out[~liq] = p.baro_psat + p.baro_cbar * (1.0 / p.baro_rsat - 1.0 / rho[~liq])
-        return out
-
+    pres = lambda S: configs.pressure(S, p)
     rho_v = 0.02
     alpha = lambda S: np.clip((p.baro_rsat - S[0]) / (p.baro_rsat - rho_v), 0.0, 1.0)
     N = F.nleaves
