@@ -52,12 +52,20 @@ def lib() -> C.CDLL:
         L.orc_limited_slope.restype = C.c_double
         L.orc_last_bad_cell.argtypes = []
         L.orc_last_bad_cell.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         _lib = L
     return _lib
 
 
 class OracleError(RuntimeError):
     pass
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-leaf loops (OpenMP; 1 = serial).  Results do not
+    depend on it."""
+    return lib().orc_set_threads(int(n))
 
 
 def _ck(rc: int, what: str) -> None:

@@ -74,6 +74,34 @@ typedef struct OCTX {
 static int g_last_bad = -1;
 int orc_last_bad_cell(void) { return g_last_bad; }
 
+/* Threads for the per-leaf loops (OpenMP; 1 = serial).  Every leaf writes only
+ * its own fields and reads its neighbours' inputs, so the results do not depend
+ * on the thread count; an error reports the lowest failing leaf index. */
+static int g_threads = 1;
+int orc_set_threads(int n) {
+  g_threads = n > 0 ? n : 1;
+  return g_threads;
+}
+
+static BRANCH **leaf_array(RUN *run, int *n) {
+  int cnt = 0;
+  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) ++cnt;
+  BRANCH **L = (BRANCH **)malloc((size_t)(cnt ? cnt : 1) * sizeof(BRANCH *));
+  if (!L) { *n = -1; return NULL; }
+  int q = 0;
+  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) L[q++] = c;
+  *n = cnt;
+  return L;
+}
+
+/* first (lowest-index) failure of a parallel leaf loop */
+static void note_failure(int rc, int leaf, int *err, int *bad) {
+#pragma omp critical(orc_fail)
+  {
+    if (!*err || leaf < *bad) { *err = rc; *bad = leaf; }
+  }
+}
+
 static double sq(double x) { return x * x; }
 
 /* inputset.c:130-222 */
@@ -309,23 +337,33 @@ static const int OFCOPP[6] = {2, 3, 0, 1, 5, 4};
 
 /* faceinterp.c:21-61 */
 int orc_faceinterp(RUN *run, const fvg_params *p) {
-  OCTX o;
-  memset(&o, 0, sizeof(o));
-  o.p = p;
-  o.neq = run->con->neq;
-  o.period = run->topo->fv_period;
-  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) {
-    for (int f = 0; f < c->nlfc; ++f) {
-      int fo = OFCOPP[f];
-      double p1cf, p1cc, m1cf, m1cc;
-      orc_distance(&o, c, f, &p1cf, &p1cc);
-      orc_distance(&o, c, fo, &m1cf, &m1cc);
-      int rc = orc_deltavalues(&o, c, o.delta1, fo);
-      if (!rc) rc = orc_deltavalues(&o, c, o.delta0, f);
-      if (rc) { g_last_bad = c->fv_index; return rc; }
-      orc_reconstruct(&o, c, f, p1cf, p1cc, m1cf, m1cc);
+  int n, err = 0, bad = -1;
+  BRANCH **L = leaf_array(run, &n);
+  if (!L) return ORC_ERR_ARG;
+#pragma omp parallel num_threads(g_threads)
+  {
+    OCTX o;
+    memset(&o, 0, sizeof(o));
+    o.p = p;
+    o.neq = run->con->neq;
+    o.period = run->topo->fv_period;
+#pragma omp for schedule(static)
+    for (int q = 0; q < n; ++q) {
+      BRANCH *c = L[q];
+      for (int f = 0; f < c->nlfc; ++f) {
+        int fo = OFCOPP[f];
+        double p1cf, p1cc, m1cf, m1cc;
+        orc_distance(&o, c, f, &p1cf, &p1cc);
+        orc_distance(&o, c, fo, &m1cf, &m1cc);
+        int rc = orc_deltavalues(&o, c, o.delta1, fo);
+        if (!rc) rc = orc_deltavalues(&o, c, o.delta0, f);
+        if (rc) { note_failure(rc, c->fv_index, &err, &bad); break; }
+        orc_reconstruct(&o, c, f, p1cf, p1cc, m1cf, m1cc);
+      }
     }
   }
+  free(L);
+  if (err) { g_last_bad = bad; return err; }
   return ORC_OK;
 }
 
@@ -333,14 +371,22 @@ int orc_faceinterp(RUN *run, const fvg_params *p) {
 
 /* gradcalc.c:9-47 */
 int orc_gradcalc(RUN *run, const fvg_params *p) {
-  OCTX o;
-  memset(&o, 0, sizeof(o));
   fvg_params q = *p;
   q.ns = 0; /* face states here need no gradients */
+  const int neq = run->con->neq;
+  int n, err = 0, bad = -1;
+  BRANCH **Lv = leaf_array(run, &n);
+  if (!Lv) return ORC_ERR_ARG;
+#pragma omp parallel num_threads(g_threads)
+  {
+  OCTX o;
+  memset(&o, 0, sizeof(o));
   o.p = &q;
-  o.neq = run->con->neq;
-  const int neq = o.neq;
-  for (BRANCH *b = run->topo->locl; b; b = b->lnxt) {
+  o.neq = neq;
+#pragma omp for schedule(static)
+  for (int iq = 0; iq < n; ++iq) {
+    BRANCH *b = Lv[iq];
+    int failed = 0;
     for (int v = 0; v <= neq; ++v) b->el->SG[v][0] = b->el->SG[v][1] = b->el->SG[v][2] = 0.0;
     for (int f = 0; f < b->nlfc; ++f) {
       for (int k = 0; k < b->nsfc[f]; ++k) {
@@ -352,7 +398,7 @@ int orc_gradcalc(RUN *run, const fvg_params *p) {
             if (!rc) rc = orc_bc(&o, b->cl->fc[f].bc, &o.sol1);
           }
         }
-        if (rc) { g_last_bad = b->fv_index; return rc; }
+        if (rc) { note_failure(rc, b->fv_index, &err, &bad); failed = 1; break; }
         CELL *cl = b->cl;
         for (int v = 0; v < neq; ++v) {
           double m = 0.5 * (o.sol1.vec[v] + o.sol.vec[v]);
@@ -365,8 +411,12 @@ int orc_gradcalc(RUN *run, const fvg_params *p) {
         b->el->SG[neq][1] += mp * cl->ny[f] * cl->Area[f][k] / cl->Vol;
         b->el->SG[neq][2] += mp * cl->nz[f] * cl->Area[f][k] / cl->Vol;
       }
+      if (failed) break;
     }
   }
+  }
+  free(Lv);
+  if (err) { g_last_bad = bad; return err; }
   return ORC_OK;
 }
 
@@ -567,21 +617,35 @@ static int check_params(RUN *run, const fvg_params *p) {
 int orc_rhs(RUN *run, const fvg_params *p) {
   int rc = check_params(run, p);
   if (rc) return rc;
-  OCTX o;
-  memset(&o, 0, sizeof(o));
-  o.p = p;
-  o.neq = run->con->neq;
-  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) {
-    rc = orc_residual(&o, c);
-    if (rc) { g_last_bad = c->fv_index; return rc; }
+  int n, err = 0, bad = -1;
+  BRANCH **L = leaf_array(run, &n);
+  if (!L) return ORC_ERR_ARG;
+#pragma omp parallel num_threads(g_threads)
+  {
+    OCTX o;
+    memset(&o, 0, sizeof(o));
+    o.p = p;
+    o.neq = run->con->neq;
+#pragma omp for schedule(static)
+    for (int q = 0; q < n; ++q) {
+      int r = orc_residual(&o, L[q]);
+      if (r) note_failure(r, L[q]->fv_index, &err, &bad);
+    }
   }
+  free(L);
+  if (err) { g_last_bad = bad; return err; }
   return ORC_OK;
 }
 
 /* RK.c:3-56 */
 int orc_rk(RUN *run, int irk, double dt) {
   const int neq = run->con->neq;
-  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) {
+  int n;
+  BRANCH **Lv = leaf_array(run, &n);
+  if (!Lv) return ORC_ERR_ARG;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int q = 0; q < n; ++q) {
+    BRANCH *c = Lv[q];
     LEAF *el = c->el;
     double V = c->cl->Vol;
     for (int f = 0; f < c->nlfc; ++f) el->flux_flag[f] = 0;
@@ -608,6 +672,7 @@ int orc_rk(RUN *run, int irk, double dt) {
         break;
     }
   }
+  free(Lv);
   return ORC_OK;
 }
 

@@ -322,3 +322,19 @@ def test_ssp_rk3_stability_polynomial():
         oracle.rk(F, irk, dt)
     z = lam * dt
     assert np.allclose(F.get_state(), S0 * (1 + z + z * z / 2 + z ** 3 / 6), rtol=1e-14)
+
+
+def test_oracle_threads_do_not_change_results():
+    """The per-leaf loops give the same bits on 1 and 4 threads (MUSCL, NS, AMR)."""
+    F = amr(3, 5)
+    xc, _ = F.leaf_info()
+    S0 = state(1, xc, seed=31)
+    p = fv.params(1, ischeme=2, ns=1, mu=1e-3, visc_work=1)
+    out = []
+    for t in (1, 4):
+        oracle.set_threads(t)
+        F.set_state(S0)
+        oracle.step(F, p, 3, 1e-3)
+        out.append(F.get_state())
+    oracle.set_threads(1)
+    assert np.array_equal(out[0], out[1])
