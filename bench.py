@@ -99,8 +99,9 @@ def build_workload(name, evolve, quiet, level=None, refine=0.01):
         g = fv.Solver.from_forest(F, p)
         t0 = time.perf_counter()
         base = (level or 5) + 1
-        flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, refine=refine,
-                                                                     coarsen=refine / 5, base_level=base)
+        # the density-gradient criterion on the device (the same flags as
+        # configs.density_gradient_flags on the downloaded gradients)
+        flags = dict(refine=refine, coarsen=refine / 5, base_level=base, kind=0)
         st = driver.evolve(F, g, evolve, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"],
                            log=lambda s: log(s, quiet))
         # final adaptation so the timed mesh matches the state

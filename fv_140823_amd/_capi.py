@@ -131,6 +131,8 @@ _DECLS = [
     ("fvg_last_error", C.c_char_p, [_vp]),
     ("fvg_time_steps", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp]),
     ("fvg_adj_set_probe", C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    ("fvg_adapt_flags", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                  C.POINTER(C.c_int8)]),
     ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
     ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),

@@ -271,6 +271,15 @@ class Solver:
     def steps(self, n: int, dt: float, method: int = 3) -> None:
         self._ck(lib().fvg_steps(self._h, int(n), int(method), float(dt)), "fvg_steps")
 
+    def adapt_flags(self, refine: float, coarsen: float, base_level: int, max_level: int,
+                    kind: int = 0) -> np.ndarray:
+        """Split/merge flags (+1/-1/0 per cell) from the density gradient, on the device."""
+        f = np.empty(self.ncell, np.int8)
+        self._ck(lib().fvg_adapt_flags(self._h, int(kind), float(refine), float(coarsen), int(base_level),
+                                       int(max_level), f.ctypes.data_as(C.POINTER(C.c_int8))),
+                 "fvg_adapt_flags")
+        return f
+
     def cfl_dt(self, cfl: float) -> float:
         d = C.c_double()
         self._ck(lib().fvg_cfl_dt(self._h, float(cfl), C.byref(d)), "fvg_cfl_dt")

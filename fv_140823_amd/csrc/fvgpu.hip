@@ -453,6 +453,27 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
   face_rebuild_block<NS>(M, P, S, SGm, G, err, list, nlist);
 }
 
+/* Refinement flags from the density gradient SG[0][.] (gradcalc rows, [neq+1][3][ns]):
+ * kind 0: g = h |grad rho| / rho (scale-free), kind 1: g = |grad rho|, the quantity of
+ * criterionsplt.c:60-69.  +1 (split) where g > refine and the level is below
+ * max_level, -1 (merge) where g < coarsen and the level is above base_level. */
+__global__ void k_adapt_flags(DevMesh M, const double *__restrict__ S, int neq,
+                              const double *__restrict__ SG, int kind, double refine,
+                              double coarsen, int base_level, int max_level,
+                              int8_t *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.N) return;
+  const int L = (M.cflag[i] >> 16) & 0xff;
+  const double gx = SG[(size_t)0 * M.ns + i], gy = SG[(size_t)1 * M.ns + i],
+               gz = SG[(size_t)2 * M.ns + i];
+  double g = sqrt(gx * gx + gy * gy + gz * gz);
+  if (kind == 0) g = g * M.h[3 * L] / S[(size_t)i * neq];
+  int8_t f = 0;
+  if (g > refine && L < max_level) f = 1;
+  else if (g < coarsen && L > base_level) f = -1;
+  flags[i] = f;
+}
+
 /* S (FP64, cell-major) -> its FP32 copy */
 __global__ void k_to_f32(const double *__restrict__ S, float *__restrict__ Sf, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1414,6 +1435,26 @@ int fvg_gradcalc(fvg_ctx *c) {
   dispatch_grad(c, c->S, true);
   if (c->prm.ns) dispatch_grad(c, c->S, false);
   return finish(c, "gradcalc");
+}
+
+int fvg_adapt_flags(fvg_ctx *c, int kind, double refine, double coarsen, int base_level,
+                    int max_level, int8_t *flags) {
+  if (!c || !flags) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  if (kind != 0 && kind != 1) return set_err(c, FVG_ERR_ARG, "kind must be 0 or 1");
+  (void)hipSetDevice(c->device);
+  dispatch_grad(c, c->S, true);
+  int8_t *d = nullptr;
+  HIPCHK(c, hipMalloc((void **)&d, (size_t)c->ns));
+  DevMesh M = dev_mesh(c);
+  hipLaunchKernelGGL(k_adapt_flags, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream, M,
+                     (const double *)c->S, c->neq, (const double *)c->SG, kind, refine, coarsen,
+                     base_level, max_level, d);
+  hipError_t e = hipMemcpyAsync(flags, d, (size_t)c->N, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  return finish(c, "adapt flags");
 }
 
 int fvg_rhs(fvg_ctx *c) {

@@ -16,15 +16,25 @@ from ._capi import FIELD_SG
 from .core import FlatMesh, Forest, Solver
 
 
-def remesh(forest: Forest, solver: Solver, flags_fn: Callable, max_level: int) -> dict:
-    """Adapt the forest on the device state and re-upload.  Returns timings."""
+def remesh(forest: Forest, solver: Solver, flags_fn, max_level: int) -> dict:
+    """Adapt the forest on the device state and re-upload.  Returns timings.
+
+    flags_fn is either a host function (forest, SG, S, max_level) -> int8 flags, for
+    which the gradients are downloaded, or a dict dict(refine=, coarsen=,
+    base_level=, kind=0) naming the density-gradient criterion, which then runs on
+    the device (fvg_adapt_flags) so that only the flags cross to the host."""
     t0 = time.perf_counter()
     S = solver.get_state()
     forest.set_state(S)
-    solver.gradcalc()
-    SG = solver.get_field(FIELD_SG)
-    t1 = time.perf_counter()
-    flags = flags_fn(forest, SG, S, max_level)
+    if isinstance(flags_fn, dict):
+        c = flags_fn
+        flags = solver.adapt_flags(c["refine"], c["coarsen"], c["base_level"], max_level, c.get("kind", 0))
+        t1 = time.perf_counter()
+    else:
+        solver.gradcalc()
+        SG = solver.get_field(FIELD_SG)
+        t1 = time.perf_counter()
+        flags = flags_fn(forest, SG, S, max_level)
     forest.adapt(flags, max_level)
     t2 = time.perf_counter()
     flat = FlatMesh(forest.run)

@@ -73,8 +73,7 @@ def study_blast(steps=200):
     repeats the last 100 steps on the same (final) mesh from the same state."""
     F, p, meta = configs.blast(5, 2)
     g = Solver.from_forest(F, p)
-    flags = lambda fr, SG, S, ml: configs.density_gradient_flags(fr, SG, S, ml, refine=0.01,
-                                                                 coarsen=0.002, base_level=6)
+    flags = dict(refine=0.01, coarsen=0.002, base_level=6, kind=0)  # criterion on the device
     t0 = time.perf_counter()
     st = driver.evolve(F, g, steps - 100, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"])
     driver.remesh(F, g, flags, meta["max_level"])

@@ -160,6 +160,13 @@ void *fvg_stream(fvg_ctx *ctx);                       /* the context's hipStream
 /* The reference's phases, each one stream-ordered kernel sequence. */
 int fvg_faceinterp(fvg_ctx *ctx);            /* SF from S (faceinterp.c:21-61) */
 int fvg_gradcalc(fvg_ctx *ctx);              /* SG[neq+1][3] from S (gradcalc.c:9-47) */
+/* Refinement flags per cell (host int8 [ncell]: +1 split, -1 merge, 0 keep) from the density
+ * gradient of S, computed on the device so that only the flags cross to the host (the
+ * criterion of criterionsplt.c:60-69 before the host's meshadaptation): kind 0 uses
+ * h |grad rho| / rho, kind 1 |grad rho|; split where it exceeds `refine` below max_level,
+ * merge where it is under `coarsen` above base_level.  Leaves SG = gradcalc(S). */
+int fvg_adapt_flags(fvg_ctx *ctx, int kind, double refine, double coarsen, int base_level,
+                    int max_level, int8_t *flags);
 int fvg_rhs(fvg_ctx *ctx);                   /* RHS from S / SF / SG (RHS.c:3-13) */
 int fvg_rk(fvg_ctx *ctx, int irk, double dt);/* stage formula irk = 0..4 (RK.c:16-51) */
 /* One time step: RK(0), then per stage [faceinterp] [gradcalc] RHS RK(k) with the RHS
