@@ -134,13 +134,11 @@ static int ht_reserve(struct fvh_forest *fr, size_t need) {
   fr->ht = ht; fr->hk = hk; fr->hcap = cap;
   return 0;
 }
-static void ht_clear(struct fvh_forest *fr) {
-  if (fr->ht) memset(fr->ht, 0, fr->hcap * sizeof(BRANCH *));
-  fr->hcount = 0;
-}
 static int ht_insert(struct fvh_forest *fr, BRANCH *b) {
-  int rc = ht_reserve(fr, fr->hcount + 1);
-  if (rc) return rc;
+  if (fr->hcap < 2 * (fr->hcount + 1) + 16) {
+    int rc = ht_reserve(fr, fr->hcount + 1);
+    if (rc) return rc;
+  }
   uint64_t k = hkey(b->level, b->fv_ijk);
   size_t p = hmix(k) & (fr->hcap - 1);
   while (fr->ht[p]) {
@@ -149,6 +147,26 @@ static int ht_insert(struct fvh_forest *fr, BRANCH *b) {
   }
   fr->ht[p] = b; fr->hk[p] = k; fr->hcount++;
   return 0;
+}
+/* removal with backward shift: an entry after the hole moves into it unless its
+ * home slot lies cyclically between the hole and itself, so probe chains stay whole */
+static void ht_remove(struct fvh_forest *fr, const BRANCH *b) {
+  if (!fr->hcap) return;
+  const size_t mask = fr->hcap - 1;
+  uint64_t k = hkey(b->level, b->fv_ijk);
+  size_t p = hmix(k) & mask;
+  while (fr->ht[p] && fr->hk[p] != k) p = (p + 1) & mask;
+  if (!fr->ht[p]) return;
+  for (size_t q = (p + 1) & mask; fr->ht[q]; q = (q + 1) & mask) {
+    size_t h = hmix(fr->hk[q]) & mask;
+    if (((q - h) & mask) >= ((q - p) & mask)) {
+      fr->ht[p] = fr->ht[q];
+      fr->hk[p] = fr->hk[q];
+      p = q;
+    }
+  }
+  fr->ht[p] = NULL;
+  fr->hcount--;
 }
 static BRANCH *ht_find(const struct fvh_forest *fr, int level, const int ijk[3]) {
   if (!fr->hcap) return NULL;
@@ -232,16 +250,10 @@ static double face_area(const BRANCH *o, int fopp) {
   return A;
 }
 
-/* meshproperties.c:15-111 */
-static void set_props(BRANCH *b) {
+/* volume as six tetrahedra and centre as the node mean (meshproperties.c:64-111);
+ * depends on the nodes only */
+static void set_volume(BRANCH *b) {
   CELL *cl = b->cl;
-  for (int f = 0; f < 6; ++f) {
-    for (int k = 0; k < 4; ++k) cl->Area[f][k] = 0.0;
-    for (int k = 0; k < b->nsfc[f]; ++k) {
-      if (b->nsfc[f] == 1) cl->Area[f][k] = face_area(b, f);
-      else cl->Area[f][k] = face_area(b->neigtr[f][k], b->neigfc[f]);
-    }
-  }
   double V = 0.0;
   for (int it = 0; it < 6; ++it) {
     double dx[3], dy[3], dz[3];
@@ -263,6 +275,20 @@ static void set_props(BRANCH *b) {
     cl->yc += cl->nd[n].y / 8;
     cl->zc += cl->nd[n].z / 8;
   }
+}
+
+/* meshproperties.c:15-111: face areas (a hanging sub-face from the finer
+ * neighbour's face), then volume and centre */
+static void set_props(BRANCH *b) {
+  CELL *cl = b->cl;
+  for (int f = 0; f < 6; ++f) {
+    for (int k = 0; k < 4; ++k) cl->Area[f][k] = 0.0;
+    for (int k = 0; k < b->nsfc[f]; ++k) {
+      if (b->nsfc[f] == 1) cl->Area[f][k] = face_area(b, f);
+      else cl->Area[f][k] = face_area(b->neigtr[f][k], b->neigfc[f]);
+    }
+  }
+  set_volume(b);
 }
 
 /* ------------------------------------------------------------------ topology */
@@ -312,7 +338,12 @@ static void free_subtree(BRANCH *b) {
   free(b);
 }
 
-static int merge_branch(BRANCH *b, int neq) {
+/* merge.c:85-107: the parent's volume is computed before the kids' gradients are
+ * volume-weighted into it (a parent created and split in one pass has none yet);
+ * the kids leave the hash */
+static int merge_branch(struct fvh_forest *fr, BRANCH *b) {
+  const int neq = fr->neq;
+  set_volume(b);
   for (int v = 0; v < neq; ++v) {
     b->el->S[v] = 0.0;
     b->el->SG[v][0] = b->el->SG[v][1] = b->el->SG[v][2] = 0.0;
@@ -322,7 +353,11 @@ static int merge_branch(BRANCH *b, int neq) {
       for (int d = 0; d < 3; ++d) b->el->SG[v][d] += k->el->SG[v][d] * k->cl->Vol / b->cl->Vol;
     }
   }
-  for (int c = 0; c < 8; ++c) { free_subtree(b->kids[c]); b->kids[c] = NULL; }
+  for (int c = 0; c < 8; ++c) {
+    ht_remove(fr, b->kids[c]);
+    free_subtree(b->kids[c]);
+    b->kids[c] = NULL;
+  }
   b->nkids = 0;
   b->nsplit = 0;
   return 0;
@@ -338,12 +373,6 @@ static void collect_leaves(BRANCH *b, BRANCH ***out, int *n, int *cap) {
     return;
   }
   for (int c = 0; c < 8; ++c) collect_leaves(b->kids[c], out, n, cap);
-}
-
-static void hash_subtree(struct fvh_forest *fr, BRANCH *b, int *rc) {
-  if (*rc) return;
-  *rc = ht_insert(fr, b);
-  for (int c = 0; c < b->nkids; ++c) hash_subtree(fr, b->kids[c], rc);
 }
 
 /* neighbour across face f of leaf b: fills neigtr/lfc_neigs/neigfc/nsfc and fc[f].bc */
@@ -393,12 +422,26 @@ static int resolve_face(struct fvh_forest *fr, BRANCH *b, int f) {
   return 0;
 }
 
-/* Leaf list, hash, neighbours and geometry after any topology change. */
-static int rebuild(struct fvh_forest *fr) {
-  int rc = 0;
-  ht_clear(fr);
-  for (int r = 0; r < fr->nroots; ++r) hash_subtree(fr, fr->roots[r], &rc);
-  if (rc) return rc;
+/* growable list of branch pointers */
+typedef struct {
+  BRANCH **p;
+  size_t n, cap;
+} bvec;
+
+static int bvec_push(bvec *v, BRANCH *b) {
+  if (v->n == v->cap) {
+    size_t cap = v->cap ? 2 * v->cap : 256;
+    BRANCH **q = (BRANCH **)realloc(v->p, cap * sizeof(BRANCH *));
+    if (!q) return FVG_ERR_NOMEM;
+    v->p = q;
+    v->cap = cap;
+  }
+  v->p[v->n++] = b;
+  return 0;
+}
+
+/* leaf list in depth-first order with list links and indices */
+static void relist(struct fvh_forest *fr) {
   int n = 0, cap = fr->nleaves > 0 ? fr->nleaves : 0;
   BRANCH **lv = fr->leaves;
   for (int r = 0; r < fr->nroots; ++r) collect_leaves(fr->roots[r], &lv, &n, &cap);
@@ -409,19 +452,90 @@ static int rebuild(struct fvh_forest *fr) {
     b->fv_index = i;
     b->lprv = i ? lv[i - 1] : fr->glob;
     b->lnxt = (i + 1 < n) ? lv[i + 1] : NULL;
-    for (int f = 0; f < 6; ++f) {
-      rc = resolve_face(fr, b, f);
-      if (rc) return rc;
-    }
-  }
-  for (int i = 0; i < n; ++i) {
-    set_normals(lv[i]);
-    set_props(lv[i]);
   }
   fr->topo.locl = n ? lv[0] : NULL;
   fr->topo.nleaves = n;
   fr->topo.tleaves = n;
+}
+
+/* neighbours across all six faces, normals, areas, volume and centre of leaf b */
+static int refresh_leaf(struct fvh_forest *fr, BRANCH *b) {
+  for (int f = 0; f < 6; ++f) {
+    int rc = resolve_face(fr, b, f);
+    if (rc) return rc;
+  }
+  set_normals(b);
+  set_props(b);
   return 0;
+}
+
+/* every leaf refreshed: after the forest is created */
+static int rebuild(struct fvh_forest *fr) {
+  relist(fr);
+  for (int i = 0; i < fr->nleaves; ++i) {
+    int rc = refresh_leaf(fr, fr->leaves[i]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* position of the same-level cell across face f of node b, wrapped on periodic
+ * axes; 0 when f lies on a non-periodic domain boundary */
+static int across(const struct fvh_forest *fr, const BRANCH *b, int f, int n[3]) {
+  int a = FV_FACE_AXIS[f];
+  int Na = ncell_axis(fr, b->level, a);
+  for (int d = 0; d < 3; ++d) n[d] = b->fv_ijk[d];
+  n[a] += FV_FACE_SIGN[f];
+  if (n[a] < 0 || n[a] >= Na) {
+    if (!fr->periodic[a]) return 0;
+    n[a] = (n[a] + Na) % Na;
+  }
+  return 1;
+}
+
+static void mark_subtree(const BRANCH *b, uint8_t *mark) {
+  if (!b->nkids) { mark[b->fv_index] = 1; return; }
+  for (int c = 0; c < 8; ++c) mark_subtree(b->kids[c], mark);
+}
+
+/* leaves of b's subtree that touch the cell across face f of b's same-level
+ * neighbour: the kids on the side of b facing that cell, recursively */
+static void mark_face(const BRANCH *b, int f, uint8_t *mark) {
+  if (!b->nkids) { mark[b->fv_index] = 1; return; }
+  int a = FV_FACE_AXIS[f];
+  for (int k = 0; k < 4; ++k) mark_face(b->kids[FV_FACE_NODES[f][k] ^ (1 << a)], f, mark);
+}
+
+/* Leaf list, neighbours and geometry after an adaptation that split or merged the
+ * nodes in chg.  Only leaves inside a changed node or face-adjacent to one can see
+ * different neighbours (a leaf's neighbours are the same-level cell across each
+ * face, its kids or the coarser cell), so only those are refreshed; the nodes of
+ * every other leaf and of its neighbours are as before. */
+static int rebuild_changed(struct fvh_forest *fr, const bvec *chg) {
+  relist(fr);
+  uint8_t *mark = (uint8_t *)calloc((size_t)fr->nleaves + 1, 1);
+  if (!mark) return FVG_ERR_NOMEM;
+  for (size_t q = 0; q < chg->n; ++q) {
+    const BRANCH *p = chg->p[q];
+    mark_subtree(p, mark);
+    for (int f = 0; f < 6; ++f) {
+      int n[3];
+      if (!across(fr, p, f, n)) continue;
+      const BRANCH *nb = ht_find(fr, p->level, n);
+      if (nb) {
+        mark_face(nb, f, mark);
+      } else if (p->level > 1) {
+        int m[3] = {n[0] >> 1, n[1] >> 1, n[2] >> 1};
+        const BRANCH *cb = ht_find(fr, p->level - 1, m);
+        if (cb && !cb->nkids) mark[cb->fv_index] = 1;
+      }
+    }
+  }
+  int rc = 0;
+  for (int i = 0; i < fr->nleaves && !rc; ++i)
+    if (mark[i]) rc = refresh_leaf(fr, fr->leaves[i]);
+  free(mark);
+  return rc;
 }
 
 /* ------------------------------------------------------------------ public */
@@ -572,43 +686,36 @@ int fvh_forest_get_field(fvh_forest *fr, int which, double *dst) {
   return field_io(fr, which, dst, 0);
 }
 
-/* true if some leaf adjacent to leaf b across a face is two or more levels finer */
-static int needs_balance(const struct fvh_forest *fr, const BRANCH *b) {
-  for (int f = 0; f < 6; ++f) {
-    int a = FV_FACE_AXIS[f];
-    int n[3] = {b->fv_ijk[0], b->fv_ijk[1], b->fv_ijk[2]};
-    int Na = ncell_axis(fr, b->level, a);
-    n[a] += FV_FACE_SIGN[f];
-    if (n[a] < 0 || n[a] >= Na) {
-      if (!fr->periodic[a]) continue;
-      n[a] = (n[a] + Na) % Na;
-    }
-    const BRANCH *nb = ht_find(fr, b->level, n);
-    if (!nb || !nb->nkids) continue;
-    for (int k = 0; k < 4; ++k)
-      if (nb->kids[FV_FACE_NODES[f][k] ^ (1 << a)]->nkids) return 1;
-  }
-  return 0;
-}
-
 /* a merge of parent p keeps 2:1 only if no kid has a finer face neighbour */
 static int merge_allowed(const struct fvh_forest *fr, const BRANCH *p) {
   for (int c = 0; c < 8; ++c) {
     const BRANCH *k = p->kids[c];
     for (int f = 0; f < 6; ++f) {
-      int a = FV_FACE_AXIS[f];
-      int n[3] = {k->fv_ijk[0], k->fv_ijk[1], k->fv_ijk[2]};
-      int Na = ncell_axis(fr, k->level, a);
-      n[a] += FV_FACE_SIGN[f];
-      if (n[a] < 0 || n[a] >= Na) {
-        if (!fr->periodic[a]) continue;
-        n[a] = (n[a] + Na) % Na;
-      }
+      int n[3];
+      if (!across(fr, k, f, n)) continue;
       const BRANCH *nb = ht_find(fr, k->level, n);
       if (nb && nb->nkids) return 0;
     }
   }
   return 1;
+}
+
+/* Split leaf b and queue the leaves that the split leaves two levels coarser than
+ * a face neighbour: the leaves one level coarser than b across its faces.  In a
+ * 2:1-balanced forest those are the only new violations, and each of them must be
+ * split in any balanced refinement, so working the queue empty gives the minimal
+ * balanced forest whatever the order. */
+static int split_queue(struct fvh_forest *fr, BRANCH *b, bvec *chg, bvec *work) {
+  int rc = split_branch(fr, b);
+  if (!rc) rc = bvec_push(chg, b);
+  for (int f = 0; f < 6 && !rc && b->level > 1; ++f) {
+    int n[3];
+    if (!across(fr, b, f, n) || ht_find(fr, b->level, n)) continue;
+    int m[3] = {n[0] >> 1, n[1] >> 1, n[2] >> 1};
+    BRANCH *c = ht_find(fr, b->level - 1, m);
+    if (c && !c->nkids) rc = bvec_push(work, c);
+  }
+  return rc;
 }
 
 int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
@@ -631,20 +738,20 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
     for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0 && p->kids[c]->merge);
     if (ok) cand[ncand++] = p;
   }
-  int rc = ht_reserve(fr, fr->hcount + (size_t)8 * N);
+  /* room for the flagged splits; balance splits grow the hash as they go */
+  size_t nflag = 0;
+  for (int i = 0; i < N; ++i) nflag += lv[i]->split;
+  bvec chg = {0}, work = {0};
+  int rc = ht_reserve(fr, fr->hcount + 8 * nflag);
   for (int i = 0; i < N && !rc; ++i)
-    if (lv[i]->split) rc = split_branch(fr, lv[i]);
+    if (lv[i]->split) rc = split_queue(fr, lv[i], &chg, &work);
   /* 2:1 face balance by further splits */
-  while (!rc) {
-    int n = 0, cap = 0, changed = 0;
-    BRANCH **cur = NULL;
-    for (int r = 0; r < fr->nroots; ++r) collect_leaves(fr->roots[r], &cur, &n, &cap);
-    for (int i = 0; i < n && !rc; ++i)
-      if (needs_balance(fr, cur[i])) { rc = split_branch(fr, cur[i]); ++changed; }
-    free(cur);
-    if (!changed) break;
+  while (!rc && work.n) {
+    BRANCH *c = work.p[--work.n];
+    if (!c->nkids) rc = split_queue(fr, c, &chg, &work);
   }
-  if (rc) { free(cand); return rc; }
+  free(work.p);
+  if (rc) { free(cand); free(chg.p); return rc; }
   /* decide every merge on the same (post-split) tree, then apply */
   int nm = 0;
   for (int q = 0; q < ncand; ++q) {
@@ -653,10 +760,14 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
     for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0);
     if (ok && merge_allowed(fr, p)) cand[nm++] = p;
   }
-  for (int q = 0; q < nm; ++q) merge_branch(cand[q], fr->neq);
+  for (int q = 0; q < nm && !rc; ++q) {
+    merge_branch(fr, cand[q]);
+    rc = bvec_push(&chg, cand[q]);
+  }
   free(cand);
   /* fr->leaves still lists freed kids of merged parents: rebuild before any use */
-  rc = rebuild(fr);
+  if (!rc) rc = rebuild_changed(fr, &chg);
+  free(chg.p);
   if (rc) return rc;
   for (int i = 0; i < fr->nleaves; ++i) fr->leaves[i]->split = fr->leaves[i]->merge = 0;
   return fr->nleaves;
