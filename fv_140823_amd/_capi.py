@@ -19,6 +19,7 @@ FVG_ERR_STATE = -3
 FVG_ERR_NONFINITE = -4
 FVG_ERR_UNSUPPORTED = -5
 FVG_ERR_NOMEM = -6
+FVG_ERR_IO = -7
 
 FVG_PLAIN, FVG_MIXED = 0, 1
 FVG_MINMOD, FVG_VANALBADA = 0, 1
@@ -31,6 +32,7 @@ ERR_NAMES = {
     FVG_ERR_NONFINITE: "non-finite or non-physical state",
     FVG_ERR_UNSUPPORTED: "unsupported option",
     FVG_ERR_NOMEM: "out of memory",
+    FVG_ERR_IO: "file error",
 }
 
 
@@ -100,10 +102,16 @@ _DECLS = [
     ("fvh_forest_run", _vp, [_vp]),
     ("fvh_forest_nleaves", C.c_int, [_vp]),
     ("fvh_forest_max_level", C.c_int, [_vp]),
+    ("fvh_forest_geometry", C.c_int, [_vp, _ip, _dp, _dp, _ip, _ip, _ip]),
     ("fvh_forest_leaf_info", C.c_int, [_vp, _dp, C.POINTER(C.c_int32)]),
     ("fvh_forest_set_field", C.c_int, [_vp, C.c_int, _dp]),
     ("fvh_forest_get_field", C.c_int, [_vp, C.c_int, _dp]),
     ("fvh_forest_adapt", C.c_int, [_vp, C.POINTER(C.c_int8), C.c_int]),
+    ("fvh_forest_save", C.c_int, [_vp, C.c_char_p, C.c_int32, C.c_double]),
+    ("fvh_forest_load", C.c_int, [C.c_char_p, C.POINTER(_vp), C.POINTER(C.c_int32), _dp]),
+    ("fvh_forest_save_text", C.c_int, [_vp, C.c_char_p, C.c_int32, C.c_double]),
+    ("fvh_forest_load_text", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int32), _dp]),
+    ("fvh_export_tecplot", C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
     ("fvh_flatten_run", C.c_int, [_vp, C.POINTER(_vp)]),
     ("fvh_flat_mesh", C.POINTER(FvgMesh), [_vp]),
     ("fvh_flat_destroy", None, [_vp]),
@@ -133,6 +141,7 @@ _DECLS = [
     ("fvg_adj_set_probe", C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     ("fvg_adapt_flags", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.POINTER(C.c_int8)]),
+    ("fvg_pressure", C.c_int, [_vp, _dp]),
     ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
     ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),

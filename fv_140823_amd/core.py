@@ -69,6 +69,59 @@ class Forest:
         self.box = (np.array(x0, float), np.array(x1, float))
         self.periodic = tuple(int(p) for p in periodic)
 
+    @classmethod
+    def _adopt(cls, h: C.c_void_p) -> "Forest":
+        self = cls.__new__(cls)
+        self._h = h
+        nr, bc, pe, neq = (C.c_int * 3)(), (C.c_int * 6)(), (C.c_int * 3)(), C.c_int()
+        a0, a1 = (C.c_double * 3)(), (C.c_double * 3)()
+        check(lib().fvh_forest_geometry(h, nr, a0, a1, bc, pe, C.byref(neq)), "fvh_forest_geometry")
+        self.neq = neq.value
+        self.box = (np.array(a0[:], float), np.array(a1[:], float))
+        self.periodic = tuple(pe[:])
+        return self
+
+    def geometry(self) -> dict:
+        nr, bc, pe, neq = (C.c_int * 3)(), (C.c_int * 6)(), (C.c_int * 3)(), C.c_int()
+        a0, a1 = (C.c_double * 3)(), (C.c_double * 3)()
+        check(lib().fvh_forest_geometry(self._h, nr, a0, a1, bc, pe, C.byref(neq)), "fvh_forest_geometry")
+        return dict(nroot=tuple(nr[:]), x0=tuple(a0[:]), x1=tuple(a1[:]), bc=tuple(bc[:]),
+                    periodic=tuple(pe[:]), neq=neq.value)
+
+    # ---- restart / export
+    def save(self, path: str, tstep: int = 0, time: float = 0.0) -> None:
+        """Binary restart (exact): geometry, leaf levels in list order, S in f64."""
+        check(lib().fvh_forest_save(self._h, str(path).encode(), int(tstep), float(time)), "fvh_forest_save")
+
+    @classmethod
+    def load(cls, path: str):
+        """Forest from a binary restart; returns (forest, tstep, time)."""
+        h, ts, tm = C.c_void_p(), C.c_int32(), C.c_double()
+        check(lib().fvh_forest_load(str(path).encode(), C.byref(h), C.byref(ts), C.byref(tm)), "fvh_forest_load")
+        return cls._adopt(h), ts.value, tm.value
+
+    def save_text(self, path: str, tstep: int = 0, time: float = 0.0) -> None:
+        """The reference's text restart layout (restart_save.c)."""
+        check(lib().fvh_forest_save_text(self._h, str(path).encode(), int(tstep), float(time)),
+              "fvh_forest_save_text")
+
+    def load_text(self, path: str):
+        """Spawn the leaves of a reference text restart into this forest of bare roots
+        (restart_load.c); returns (tstep, time)."""
+        ts, tm = C.c_int32(), C.c_double()
+        check(lib().fvh_forest_load_text(self._h, str(path).encode(), C.byref(ts), C.byref(tm)),
+              "fvh_forest_load_text")
+        return ts.value, tm.value
+
+    def export_tecplot(self, path: str, model: int, p: np.ndarray) -> None:
+        """Tecplot FEPOINT brick zone (exportfield_linked_V3.c) from the leaves' S and SG and
+        the cell pressure p."""
+        p = np.ascontiguousarray(p, np.float64)
+        if p.shape != (self.nleaves,):
+            raise ValueError("one pressure per leaf")
+        check(lib().fvh_export_tecplot(self.run, str(path).encode(), int(model), _dptr(p)),
+              "fvh_export_tecplot")
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().fvh_forest_destroy(self._h)
@@ -279,6 +332,12 @@ class Solver:
                                        int(max_level), f.ctypes.data_as(C.POINTER(C.c_int8))),
                  "fvg_adapt_flags")
         return f
+
+    def pressure(self) -> np.ndarray:
+        """Cell pressure from the device state (FP64 EOS)."""
+        p = np.empty(self.ncell, np.float64)
+        self._ck(lib().fvg_pressure(self._h, _dptr(p)), "fvg_pressure")
+        return p
 
     def cfl_dt(self, cfl: float) -> float:
         d = C.c_double()

@@ -474,6 +474,20 @@ __global__ void k_adapt_flags(DevMesh M, const double *__restrict__ S, int neq,
   flags[i] = f;
 }
 
+/* cell pressure with the gradient kernel's FP64 EOS (the p row of gradcalc) */
+template <int MODEL>
+__global__ void k_pressure(DevMesh M, DevParams P, const double *__restrict__ S,
+                           double *__restrict__ pout) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.N) return;
+  double U[NEQ];
+  load_S<NEQ>(S, M.ns, i, U);
+  double r, u[3], e, p, c, y;
+  eos64<MODEL>(P, U, false, r, u, e, p, c, y);
+  pout[i] = p;
+}
+
 /* S (FP64, cell-major) -> its FP32 copy */
 __global__ void k_to_f32(const double *__restrict__ S, float *__restrict__ Sf, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1455,6 +1469,26 @@ int fvg_adapt_flags(fvg_ctx *c, int kind, double refine, double coarsen, int bas
   (void)hipFree(d);
   if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
   return finish(c, "adapt flags");
+}
+
+int fvg_pressure(fvg_ctx *c, double *p) {
+  if (!c || !p) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  (void)hipSetDevice(c->device);
+  double *d = nullptr;
+  HIPCHK(c, hipMalloc((void **)&d, (size_t)c->N * sizeof(double)));
+  DevMesh M = dev_mesh(c);
+  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  switch (c->prm.model) {
+    case 1: hipLaunchKernelGGL(k_pressure<1>, g, b, 0, c->stream, M, c->dp, (const double *)c->S, d); break;
+    case 2: hipLaunchKernelGGL(k_pressure<2>, g, b, 0, c->stream, M, c->dp, (const double *)c->S, d); break;
+    default: hipLaunchKernelGGL(k_pressure<3>, g, b, 0, c->stream, M, c->dp, (const double *)c->S, d); break;
+  }
+  hipError_t e = hipMemcpyAsync(p, d, (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  return finish(c, "pressure");
 }
 
 int fvg_rhs(fvg_ctx *c) {

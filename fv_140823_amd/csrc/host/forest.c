@@ -16,6 +16,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -293,6 +294,12 @@ static void set_props(BRANCH *b) {
 
 /* ------------------------------------------------------------------ topology */
 
+static int64_t pow10_i64(int e) {
+  int64_t r = 1;
+  while (e-- > 0) r *= 10;
+  return r;
+}
+
 static BRANCH *new_branch(struct fvh_forest *fr, BRANCH *prnt, int level, const int ijk[3],
                           int root) {
   BRANCH *b = branch_alloc(fr->neq);
@@ -324,6 +331,10 @@ static int split_branch(struct fvh_forest *fr, BRANCH *b) {
       k->el->SG[v][1] = b->el->SG[v][1];
       k->el->SG[v][2] = b->el->SG[v][2];
     }
+    /* tagaddress.c:19-27: decimal digit level-2 of the tag is 1 + the kid index at
+       that level (split.c:73); past 10 levels it no longer fits an int */
+    int64_t t = b->tag < 0 ? -1 : (int64_t)b->tag + pow10_i64(b->level - 1) * (1 + c);
+    k->tag = (t >= 0 && t <= INT32_MAX) ? (int)t : -1;
     b->kids[c] = k;
     int rc = ht_insert(fr, k);
     if (rc) return rc;
@@ -617,6 +628,20 @@ void fvh_forest_destroy(fvh_forest *fr) {
 RUN *fvh_forest_run(fvh_forest *fr) { return fr ? &fr->run : NULL; }
 int fvh_forest_nleaves(fvh_forest *fr) { return fr ? fr->nleaves : FVG_ERR_ARG; }
 
+int fvh_forest_geometry(fvh_forest *fr, int nroot[3], double x0[3], double x1[3], int bc[6],
+                        int periodic[3], int *neq) {
+  if (!fr) return FVG_ERR_ARG;
+  for (int a = 0; a < 3; ++a) {
+    if (nroot) nroot[a] = fr->nroot[a];
+    if (x0) x0[a] = fr->topo.fv_x0[a];
+    if (x1) x1[a] = fr->topo.fv_x1[a];
+    if (periodic) periodic[a] = fr->periodic[a];
+  }
+  for (int s = 0; s < 6 && bc; ++s) bc[s] = fr->bc[s];
+  if (neq) *neq = fr->neq;
+  return FVG_OK;
+}
+
 int fvh_forest_max_level(fvh_forest *fr) {
   if (!fr) return FVG_ERR_ARG;
   int m = 0;
@@ -771,4 +796,172 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   if (rc) return rc;
   for (int i = 0; i < fr->nleaves; ++i) fr->leaves[i]->split = fr->leaves[i]->merge = 0;
   return fr->nleaves;
+}
+
+/* ------------------------------------------------------------------ restart */
+
+/* Binary restart of this library: the forest's description, the leaf levels in list
+ * order (a depth-first level sequence fixes the tree over given roots) and S var-major,
+ * all native-endian.  The reference has no binary restart; its text layout follows. */
+#define FVH_MAGIC "FVH-RST1"
+
+typedef struct {
+  char magic[8];
+  int32_t neq, nroot[3], bc[6], periodic[3], tstep;
+  double x0[3], x1[3], time;
+  int64_t nleaves;
+} fvh_rst_head;
+
+int fvh_forest_save(fvh_forest *fr, const char *path, int32_t tstep, double time) {
+  if (!fr || !path) return FVG_ERR_ARG;
+  fvh_rst_head h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, FVH_MAGIC, 8);
+  h.neq = fr->neq;
+  for (int a = 0; a < 3; ++a) {
+    h.nroot[a] = fr->nroot[a];
+    h.periodic[a] = fr->periodic[a];
+    h.x0[a] = fr->topo.fv_x0[a];
+    h.x1[a] = fr->topo.fv_x1[a];
+  }
+  for (int s = 0; s < 6; ++s) h.bc[s] = fr->bc[s];
+  h.tstep = tstep;
+  h.time = time;
+  h.nleaves = fr->nleaves;
+  const size_t N = (size_t)fr->nleaves;
+  uint8_t *lev = (uint8_t *)malloc(N ? N : 1);
+  double *S = (double *)malloc((N ? N : 1) * (size_t)fr->neq * sizeof(double));
+  if (!lev || !S) { free(lev); free(S); return FVG_ERR_NOMEM; }
+  for (size_t i = 0; i < N; ++i) lev[i] = (uint8_t)fr->leaves[i]->level;
+  field_io(fr, FVG_FIELD_S, S, 0);
+  FILE *fp = fopen(path, "wb");
+  int rc = fp ? 0 : FVG_ERR_IO;
+  if (!rc && (fwrite(&h, sizeof(h), 1, fp) != 1 || fwrite(lev, 1, N, fp) != N ||
+              fwrite(S, sizeof(double), N * fr->neq, fp) != N * fr->neq))
+    rc = FVG_ERR_IO;
+  if (fp && fclose(fp) != 0) rc = FVG_ERR_IO;
+  free(lev);
+  free(S);
+  return rc;
+}
+
+/* rebuild the subtree of node b from the depth-first leaf levels lev[*pos..n) */
+static int grow_from_levels(struct fvh_forest *fr, BRANCH *b, const uint8_t *lev, size_t n,
+                            size_t *pos) {
+  if (*pos >= n) return FVG_ERR_ARG;
+  if (lev[*pos] == b->level) { ++*pos; return 0; }
+  if (lev[*pos] < b->level || b->level >= 30) return FVG_ERR_ARG;
+  int rc = split_branch(fr, b);
+  for (int c = 0; c < 8 && !rc; ++c) rc = grow_from_levels(fr, b->kids[c], lev, n, pos);
+  return rc;
+}
+
+int fvh_forest_load(const char *path, fvh_forest **out, int32_t *tstep, double *time) {
+  if (!path || !out) return FVG_ERR_ARG;
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return FVG_ERR_IO;
+  fvh_rst_head h;
+  uint8_t *lev = NULL;
+  double *S = NULL;
+  fvh_forest *fr = NULL;
+  int rc = fread(&h, sizeof(h), 1, fp) == 1 ? 0 : FVG_ERR_IO;
+  if (!rc && (memcmp(h.magic, FVH_MAGIC, 8) != 0 || h.nleaves < 1 || h.nleaves > INT32_MAX ||
+              h.neq < 1 || h.neq > 8))
+    rc = FVG_ERR_ARG;
+  const size_t N = rc ? 0 : (size_t)h.nleaves;
+  if (!rc) {
+    lev = (uint8_t *)malloc(N);
+    S = (double *)malloc(N * (size_t)h.neq * sizeof(double));
+    if (!lev || !S) rc = FVG_ERR_NOMEM;
+  }
+  if (!rc && (fread(lev, 1, N, fp) != N || fread(S, sizeof(double), N * h.neq, fp) != N * h.neq))
+    rc = FVG_ERR_IO;
+  fclose(fp);
+  if (!rc) rc = fvh_forest_create(h.nroot, h.x0, h.x1, 0, h.bc, h.periodic, h.neq, &fr);
+  size_t pos = 0;
+  for (int r = 0; !rc && r < fr->nroots; ++r) rc = grow_from_levels(fr, fr->roots[r], lev, N, &pos);
+  if (!rc && pos != N) rc = FVG_ERR_ARG;
+  if (!rc) rc = rebuild(fr);
+  if (!rc) rc = field_io(fr, FVG_FIELD_S, S, 1);
+  free(lev);
+  free(S);
+  if (rc) { fvh_forest_destroy(fr); return rc; }
+  if (tstep) *tstep = h.tstep;
+  if (time) *time = h.time;
+  *out = fr;
+  return FVG_OK;
+}
+
+/* The reference's text restart (restart_save.c:17-45): a header line
+ * "tstep time nleaves ntrees neq" and per leaf in list order "root tag  S0 ... " with
+ * each value as " %15.10e ".  The tag is the leaf's kid path (tagaddress.c:19-27), so
+ * levels above 10 cannot be written. */
+int fvh_forest_save_text(fvh_forest *fr, const char *path, int32_t tstep, double time) {
+  if (!fr || !path) return FVG_ERR_ARG;
+  for (int i = 0; i < fr->nleaves; ++i)
+    if (fr->leaves[i]->tag < 0) return FVG_ERR_UNSUPPORTED;
+  FILE *fp = fopen(path, "w");
+  if (!fp) return FVG_ERR_IO;
+  fprintf(fp, "%d %15.10e %d %d %d \n", tstep, time, fr->nleaves, fr->nroots, fr->neq);
+  for (int i = 0; i < fr->nleaves; ++i) {
+    const BRANCH *b = fr->leaves[i];
+    fprintf(fp, "%d %d ", b->root, b->tag);
+    for (int v = 0; v < fr->neq; ++v) fprintf(fp, " %15.10e ", b->el->S[v]);
+    fprintf(fp, "\n");
+  }
+  return (ferror(fp) | fclose(fp)) ? FVG_ERR_IO : FVG_OK;
+}
+
+/* restart_load.c:30-58 with spawntree.c:40-54,112-125: each leaf line names a root and
+ * a tag whose decimal digits, least significant first, are 1 + the kid index at levels
+ * 2, 3, ...; the path is split open as needed and the leaf takes the listed state.  The
+ * forest must be bare roots of the geometry the restart was written on, and every leaf
+ * of the result must be listed exactly once. */
+int fvh_forest_load_text(fvh_forest *fr, const char *path, int32_t *tstep, double *time) {
+  if (!fr || !path) return FVG_ERR_ARG;
+  for (int r = 0; r < fr->nroots; ++r)
+    if (fr->roots[r]->nkids) return FVG_ERR_STATE;
+  FILE *fp = fopen(path, "r");
+  if (!fp) return FVG_ERR_IO;
+  int ts = 0, nl = 0, nt = 0, nv = 0;
+  double tm = 0.0;
+  int rc = fscanf(fp, "%d %lf %d %d %d", &ts, &tm, &nl, &nt, &nv) == 5 ? 0 : FVG_ERR_IO;
+  if (!rc && (nt != fr->nroots || nv != fr->neq || nl < 1)) rc = FVG_ERR_ARG;
+  bvec got = {0};
+  for (int l = 0; l < nl && !rc; ++l) {
+    int root;
+    long long tag;
+    if (fscanf(fp, "%d %lld", &root, &tag) != 2) { rc = FVG_ERR_IO; break; }
+    if (root < 0 || root >= fr->nroots || tag < 0) { rc = FVG_ERR_ARG; break; }
+    BRANCH *b = fr->roots[root];
+    for (long long t = tag; t > 0 && !rc; t /= 10) {
+      int d = (int)(t % 10);
+      if (d < 1 || d > 8 || b->level >= 30) { rc = FVG_ERR_ARG; break; }
+      rc = split_branch(fr, b);
+      if (!rc) b = b->kids[d - 1];
+    }
+    if (!rc && b->nkids) rc = FVG_ERR_ARG;  /* a listed cell that is not a leaf */
+    for (int v = 0; v < nv && !rc; ++v)
+      if (fscanf(fp, "%lf", &b->el->S[v]) != 1) rc = FVG_ERR_IO;
+    if (!rc) rc = bvec_push(&got, b);
+  }
+  fclose(fp);
+  if (!rc) rc = rebuild(fr);
+  if (!rc && fr->nleaves != nl) rc = FVG_ERR_ARG;
+  /* every leaf listed once: list positions of the listed leaves are a permutation */
+  if (!rc) {
+    uint8_t *seen = (uint8_t *)calloc((size_t)nl, 1);
+    if (!seen) rc = FVG_ERR_NOMEM;
+    for (size_t q = 0; q < got.n && !rc; ++q) {
+      int i = got.p[q]->fv_index;
+      if (got.p[q]->nkids || i < 0 || i >= nl || seen[i]) rc = FVG_ERR_ARG;
+      else seen[i] = 1;
+    }
+    free(seen);
+  }
+  free(got.p);
+  if (rc) return rc;
+  if (tstep) *tstep = ts;
+  if (time) *time = tm;
+  return FVG_OK;
 }

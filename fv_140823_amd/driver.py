@@ -44,6 +44,31 @@ def remesh(forest: Forest, solver: Solver, flags_fn, max_level: int) -> dict:
     return dict(download_s=t1 - t0, adapt_s=t2 - t1, flatten_upload_s=t3 - t2, nleaves=forest.nleaves)
 
 
+def save_restart(forest: Forest, solver: Solver, path: str, tstep: int = 0, time: float = 0.0,
+                 text: bool = False) -> None:
+    """Restart from the device state: S is downloaded into the leaves and written in the
+    binary format (exact) or the reference's text layout (restart_save.c)."""
+    forest.set_state(solver.get_state())
+    (forest.save_text if text else forest.save)(path, tstep, time)
+
+
+def load_restart(path: str, p, device: int = 0):
+    """Forest and solver from a binary restart, the state on the device.
+    Returns (forest, solver, tstep, time)."""
+    forest, tstep, time = Forest.load(path)
+    solver = Solver.from_forest(forest, p, device)
+    return forest, solver, tstep, time
+
+
+def export_tecplot(forest: Forest, solver: Solver, path: str) -> None:
+    """Tecplot FEPOINT file of the device state (exportfield_linked_V3.c): S, gradcalc's
+    SG and the FP64 cell pressure are downloaded into the leaves and written."""
+    solver.gradcalc()
+    forest.set_state(solver.get_state())
+    forest.set_field(FIELD_SG, solver.get_field(FIELD_SG))
+    forest.export_tecplot(path, solver.params.model, solver.pressure())
+
+
 def evolve(forest: Forest, solver: Solver, steps: int, cfl: float, method: int = 3,
            adapt_every: int = 0, flags_fn: Optional[Callable] = None, max_level: int = 0,
            dt_fixed: Optional[float] = None, log: Optional[Callable] = None) -> dict:

@@ -35,7 +35,8 @@ enum {
   FVG_ERR_STATE = -3,       /* call out of order (e.g. no mesh yet) */
   FVG_ERR_NONFINITE = -4,   /* NaN/Inf in RHS or state; first bad cell via fvg_first_bad_cell */
   FVG_ERR_UNSUPPORTED = -5, /* option with no defined behaviour (bc 3/4, model 4, ...) */
-  FVG_ERR_NOMEM = -6
+  FVG_ERR_NOMEM = -6,
+  FVG_ERR_IO = -7           /* a file could not be opened, read or written */
 };
 
 /* precision forms */
@@ -125,6 +126,9 @@ void fvh_forest_destroy(fvh_forest *fr);
 RUN *fvh_forest_run(fvh_forest *fr);          /* the RUN whose topo/con/par describe this forest */
 int fvh_forest_nleaves(fvh_forest *fr);       /* returns the count (>=0) */
 int fvh_forest_max_level(fvh_forest *fr);     /* deepest leaf level (roots = 1) */
+/* The creation arguments back (any pointer may be NULL). */
+int fvh_forest_geometry(fvh_forest *fr, int nroot[3], double x0[3], double x1[3], int bc[6],
+                        int periodic[3], int *neq);
 /* Per leaf (list order): centre xc[3N], level[N] (reference levels, roots = 1). */
 int fvh_forest_leaf_info(fvh_forest *fr, double *xc, int32_t *level);
 /* Copy S/SN/RHS/SF/SG between leaf arrays and var-major host buffers (layouts as FVG_FIELD_*). */
@@ -135,6 +139,27 @@ int fvh_forest_get_field(fvh_forest *fr, int which, double *dst);
  * of the kids (merge.c:92-107).  2:1 face balance is restored by further splits.  Leaves
  * deeper than max_level are not split.  Returns the new leaf count or an error. */
 int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level);
+
+/* ---------------------------------------------------------------- restart / export */
+/* Binary restart of this library (no counterpart in the reference): the forest's box,
+ * roots, side types and periodic flags, neq, tstep and time, the leaf levels in list order
+ * and S var-major in f64; the load rebuilds the same forest with the same S bits. */
+int fvh_forest_save(fvh_forest *fr, const char *path, int32_t tstep, double time);
+int fvh_forest_load(const char *path, fvh_forest **out, int32_t *tstep, double *time);
+/* The reference's text restart: replaces restart_save (restart_save.c:3-45; header
+ * "tstep time nleaves ntrees neq", per leaf "root tag" and S at %15.10e) and
+ * restart_load (restart_load.c:19-63, leaves spawned along their tags as spawntree.c).
+ * The load wants a forest of bare roots (level 0) of the same geometry; every leaf of the
+ * result must be listed once.  Tags hold 10 levels at most (FVG_ERR_UNSUPPORTED). */
+int fvh_forest_save_text(fvh_forest *fr, const char *path, int32_t tstep, double time);
+int fvh_forest_load_text(fvh_forest *fr, const char *path, int32_t *tstep, double *time);
+/* Tecplot FEPOINT brick zone of the leaves of a RUN: replaces exportfield_linked_V3
+ * (exportfield_linked_V3.c:19-572, one rank).  Nodes are the reference's links (coincident
+ * element nodes joined across same-level and coarse-to-fine faces) with member-mean values
+ * of S, p, |grad rho|, |grad rho u|, |grad rho v|, |grad p|; S and SG come from the leaves,
+ * p[ncell] from the caller (fvg_pressure).  model 1..3 picks the variable names (0: take
+ * run->con->model). */
+int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p);
 
 /* ---------------------------------------------------------------- flattener */
 /* Flatten the leaf list of `run` (any RUN whose leaves carry the reference's neighbour and
@@ -167,6 +192,9 @@ int fvg_gradcalc(fvg_ctx *ctx);              /* SG[neq+1][3] from S (gradcalc.c:
  * merge where it is under `coarsen` above base_level.  Leaves SG = gradcalc(S). */
 int fvg_adapt_flags(fvg_ctx *ctx, int kind, double refine, double coarsen, int base_level,
                     int max_level, int8_t *flags);
+/* Cell pressure (host f64 [ncell]) from the device S, with the FP64 EOS of the gradient
+ * kernel: the P column of the export (volmvalues.c:22-40, exportfield_linked_V3.c:390). */
+int fvg_pressure(fvg_ctx *ctx, double *p);
 int fvg_rhs(fvg_ctx *ctx);                   /* RHS from S / SF / SG (RHS.c:3-13) */
 int fvg_rk(fvg_ctx *ctx, int irk, double dt);/* stage formula irk = 0..4 (RK.c:16-51) */
 /* One time step: RK(0), then per stage [faceinterp] [gradcalc] RHS RK(k) with the RHS
