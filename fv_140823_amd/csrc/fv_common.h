@@ -115,16 +115,17 @@ __device__ __forceinline__ T limited(int lim, double eps, bool bnd, T d0, T d1, 
   T grad = (T)0;
   if (bnd) return grad;
   if (lim == FVG_MINMOD) {
-    if (fabs(d0) > (T)1e-10) {
-      T a = -d1 * im, b = d0 * ip;
-      T prod = a * b;
-      if (prod > (T)0) grad = fabs(a) > fabs(b) ? b : a;
-    }
+    /* both tests as selects (no divergent branch): a and b are formed either way and
+     * only picked when |d0| > 1e-10 and a b > 0 (false for NaN), as in the reference */
+    const T a = -d1 * im, b = d0 * ip;
+    const T prod = a * b;
+    const T pick = fabs(a) > fabs(b) ? b : a;
+    grad = (fabs(d0) > (T)1e-10 && prod > (T)0) ? pick : grad;
   } else {
     const T e = (T)eps;
     T a = isinf(im) ? (T)0 : -d1 * im;
     T b = isinf(ip) ? (T)0 : d0 * ip;
-    grad = (a * (b * b + e) + b * (a * a + e)) / (a * a + b * b + (T)2 * e);
+    grad = qdiv((a * (b * b + e) + b * (a * a + e)), (a * a + b * b + (T)2 * e));
   }
   return grad;
 }
@@ -136,6 +137,18 @@ __device__ __forceinline__ int logical_block() {
   const int per = nb >> 3;
   if (per == 0 || (nb & 7)) return b;
   return (b & 7) * per + (b >> 3);
+}
+
+/* Face blocks: [0, b0) take the x group, [b0, b0 + b1) the y group, the rest the z
+ * group (the grid is b0 + b1 + b2 blocks).  With the XCD-aware block order each XCD
+ * works through a stretch of one group.  (Interleaving the three groups block by block,
+ * so that one XCD reads a cell range for all three axes, fetches fewer bytes but ran
+ * slower on 2M cells.)  Returns the axis, blk its block in that group. */
+__device__ __forceinline__ int axis_chunk(int lb, int b0, int b1, int &blk) {
+  if (lb < b0) { blk = lb; return 0; }
+  if (lb < b0 + b1) { blk = lb - b0; return 1; }
+  blk = lb - b0 - b1;
+  return 2;
 }
 
 /* ------------------------------------------------------------------ context */

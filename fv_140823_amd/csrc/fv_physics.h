@@ -24,6 +24,28 @@
 
 #define FV_HD __host__ __device__
 
+/* Quotient and square root of the face physics.  FP64 keeps the correctly rounded
+ * operations (the plain form's parity with the reference rests on them).  FP32, the
+ * mixed form, takes the hardware reciprocal and square root (about 1 ulp each)
+ * instead of the correctly rounded expansions (~10 and ~15 instructions): their error
+ * is of the size of the FP32 rounding the mixed form accepts anyway. */
+template <typename T> FV_HD __forceinline__ T qdiv(T a, T b) { return a / b; }
+template <typename T> FV_HD __forceinline__ T qsqrt(T a) { return sqrt(a); }
+template <> FV_HD __forceinline__ float qdiv<float>(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a / b;
+#endif
+}
+template <> FV_HD __forceinline__ float qsqrt<float>(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(a);
+#else
+  return sqrtf(a);
+#endif
+}
+
 struct DevParams {
   int model, ns, ischeme, limiter, visc_work, neq;
   double gamma, pinf, mu, ymin;
@@ -139,8 +161,8 @@ FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
   T ruL = L.r * uL, ruR = R.r * uR;
   T SR = (uL + L.c) > (uR + R.c) ? (uL + L.c) : (uR + R.c);
   T SL = (uL - L.c) < (uR - R.c) ? (uL - L.c) : (uR - R.c);
-  T SS = ((L.r * (uL * uL) + L.p) - (R.r * (uR * uR) + R.p) - (SL * ruL) + (SR * ruR)) /
-         (ruL - ruR - (SL * L.r) + (SR * R.r));
+  T SS = qdiv(((L.r * (uL * uL) + L.p) - (R.r * (uR * uR) + R.p) - (SL * ruL) + (SR * ruR)),
+              (ruL - ruR - (SL * L.r) + (SR * R.r)));
   int region = 0;
   if (zero <= SL) region = 1;
   else if (SL <= zero && zero <= SS) region = 2;
@@ -197,13 +219,13 @@ FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
   for (int v = 0; v < NEQ; ++v) corr[v] = zero;
   if (region == 2 || region == 3) {
     /* star_region_euler.c:19-37 */
-    T cs = Kr * (SK - uK) / (SK - SS);
+    T cs = qdiv(Kr * (SK - uK), (SK - SS));
     T Us[NEQ];
     Us[0] = cs;
 #pragma unroll
     for (int i = 0; i < 3; ++i) Us[1 + i] = cs * (i == a ? s * SS : Ku[i]);
     if constexpr (MODEL == 1)
-      Us[4] = cs * ((Ke / Kr) + (SS - uK) * (SS + (Kp / (Kr * (SK - uK)))));
+      Us[4] = cs * (qdiv(Ke, Kr) + (SS - uK) * (SS + qdiv(Kp, (Kr * (SK - uK)))));
     if constexpr (MODEL == 3) Us[4] = cs * Ky;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) corr[v] = SK * (Us[v] - Kq[v]);
@@ -247,14 +269,14 @@ template <typename T>
 FV_HD __forceinline__ void make_state_ideal(const DevParams &P, const T *U, FState<T> &s) {
   const T g = (T)P.gamma, pinf = (T)P.pinf;
   s.r = U[0];
-  const T ir = (T)1.0 / s.r;
+  const T ir = qdiv((T)1.0, s.r);
   s.u[0] = U[1] * ir;
   s.u[1] = U[2] * ir;
   s.u[2] = U[3] * ir;
   s.e = U[4];
   T ei = (s.e * ir) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
   s.p = s.r * (g - (T)1.0) * ei - g * pinf;
-  s.c = sqrt(g * (s.p + pinf) * ir);
+  s.c = qsqrt(g * (s.p + pinf) * ir);
   s.y = (T)0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) s.q[v] = U[v];

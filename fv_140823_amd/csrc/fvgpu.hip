@@ -128,12 +128,21 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
   }
 }
 
+/* The flux rows of a block's faces are staged in LDS (one row per thread) and then
+ * written as the block's contiguous range [j0, j0 + nrow) of G with consecutive
+ * words per lane: full-line stores instead of NEQ strided partial ones per thread. */
+template <int NEQ, typename R>
+__device__ __forceinline__ void store_rows(R *__restrict__ G, const R *gbuf, int j0, int nrow) {
+  R *g = G + (size_t)j0 * NEQ;
+  for (int k = threadIdx.x; k < nrow * NEQ; k += FV_BLOCK) g[k] = gbuf[k];
+}
+
 /* one face, both sides (residual_faces_AMR.c:95-150 for the owner side), with
  * the axis a of the face as a compile-time constant */
 template <int MODEL, bool NS, int a, typename T, typename R>
 __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, int j,
                                           const double *__restrict__ S, const R *__restrict__ SF,
-                                          const R *__restrict__ SGm, R *__restrict__ G,
+                                          const R *__restrict__ SGm, R *dst,
                                           int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const int iL = M.fL[j], iR = M.fR[j];
@@ -173,7 +182,7 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
     T g = -(A) * out[v];
-    G[(size_t)j * NEQ + v] = (R)g;
+    dst[v] = (R)g;
   }
   if (!ok) atomicMin(err, iL);
 }
@@ -186,7 +195,7 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
 template <int MODEL, bool NS, int a, typename T, typename R, bool REG>
 __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParams &P, int j,
                                                 const double *__restrict__ S,
-                                                const R *__restrict__ SGm, R *__restrict__ G,
+                                                const R *__restrict__ SGm, R *dst,
                                                 int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const int iL = M.fL[j], iR = M.fR[j];
@@ -223,7 +232,7 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
   ok = hllc_face<MODEL, NS, a, T>(P, L, Rs, s, out) && ok;
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) G[(size_t)j * NEQ + v] = (R)(-(A) * out[v]);
+  for (int v = 0; v < NEQ; ++v) dst[v] = (R)(-(A) * out[v]);
   if (!ok) atomicMin(err, iL);
 }
 
@@ -235,23 +244,35 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
     DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
     int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  __shared__ R gbuf[FV_BLOCK * NEQ];
   const int lb = logical_block();
   const int t = threadIdx.x;
   /* axis a: faces [0, r_a) of the group are regular (a block straddles the end of
    * that range at most once per axis) */
-  if (lb < b0) {
-    const int q = lb * FV_BLOCK + t;
-    if (q < r0) face_fused_body<MODEL, NS, 0, T, R, true>(M, P, q, S, SGm, G, err);
-    else if (q < n0) face_fused_body<MODEL, NS, 0, T, R, false>(M, P, q, S, SGm, G, err);
-  } else if (lb < b0 + b1) {
-    const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < r1) face_fused_body<MODEL, NS, 1, T, R, true>(M, P, n0 + q, S, SGm, G, err);
-    else if (q < n1) face_fused_body<MODEL, NS, 1, T, R, false>(M, P, n0 + q, S, SGm, G, err);
+  int blk;
+  const int ax = axis_chunk(lb, b0, b1, blk);
+  const int q = blk * FV_BLOCK + t;
+  R *dst = gbuf + t * NEQ;
+  int base, n;
+  if (ax == 0) {
+    base = 0;
+    n = n0;
+    if (q < r0) face_fused_body<MODEL, NS, 0, T, R, true>(M, P, q, S, SGm, dst, err);
+    else if (q < n0) face_fused_body<MODEL, NS, 0, T, R, false>(M, P, q, S, SGm, dst, err);
+  } else if (ax == 1) {
+    base = n0;
+    n = n1;
+    if (q < r1) face_fused_body<MODEL, NS, 1, T, R, true>(M, P, n0 + q, S, SGm, dst, err);
+    else if (q < n1) face_fused_body<MODEL, NS, 1, T, R, false>(M, P, n0 + q, S, SGm, dst, err);
   } else {
-    const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true>(M, P, n0 + n1 + q, S, SGm, G, err);
-    else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false>(M, P, n0 + n1 + q, S, SGm, G, err);
+    base = n0 + n1;
+    n = n2;
+    if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true>(M, P, n0 + n1 + q, S, SGm, dst, err);
+    else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false>(M, P, n0 + n1 + q, S, SGm, dst, err);
   }
+  __syncthreads();
+  store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
 }
 
 /* Mixed form of the ideal gas (model 1): MUSCL from the FP32 state copy, the
@@ -265,7 +286,7 @@ template <bool NS, int a, bool REG>
 __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams &P, int j,
                                                const float *__restrict__ Sf,
                                                const float *__restrict__ SGm,
-                                               float *__restrict__ G, int *__restrict__ err) {
+                                               float *dst, int *__restrict__ err) {
   const int iL = M.fL[j], iR = M.fR[j];
   const uint32_t meta = M.fmeta[j];
   const int fl = meta & 7, fr = (meta >> 3) & 7, bc = REG ? 0 : (meta >> 6) & 3,
@@ -317,7 +338,7 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
   const bool ok = hllc_face<1, NS, a, float>(P, L, Rs, s, out);
   const float A = (float)M.area[3 * lev + a];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) G[(size_t)j * 5 + v] = -(A)*out[v];
+  for (int v = 0; v < 5; ++v) dst[v] = -(A)*out[v];
   if (!ok) atomicMin(err, iL);
   return true;
 }
@@ -429,23 +450,23 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
   const int t = threadIdx.x;
   if (t == 0) nlist = 0;
   __syncthreads();
-  int j = -1;
   bool done = true;
-  if (lb < b0) {
-    const int q = lb * FV_BLOCK + t;
-    j = q;
-    if (q < r0) done = face_fast_body<NS, 0, true>(M, P, j, Sf, SGm, G, err);
-    else if (q < n0) done = face_fast_body<NS, 0, false>(M, P, j, Sf, SGm, G, err);
-  } else if (lb < b0 + b1) {
-    const int q = (lb - b0) * FV_BLOCK + t;
-    j = n0 + q;
-    if (q < r1) done = face_fast_body<NS, 1, true>(M, P, j, Sf, SGm, G, err);
-    else if (q < n1) done = face_fast_body<NS, 1, false>(M, P, j, Sf, SGm, G, err);
+  int blk;
+  const int ax = axis_chunk(lb, b0, b1, blk);
+  const int q = blk * FV_BLOCK + t;
+  /* FP32 rows (20 B) are stored directly: staging them through LDS as the FP64
+   * kernels do measured slower here */
+  const int j = ax == 0 ? q : (ax == 1 ? n0 + q : n0 + n1 + q);
+  float *dst = G + (size_t)j * 5;
+  if (ax == 0) {
+    if (q < r0) done = face_fast_body<NS, 0, true>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n0) done = face_fast_body<NS, 0, false>(M, P, j, Sf, SGm, dst, err);
+  } else if (ax == 1) {
+    if (q < r1) done = face_fast_body<NS, 1, true>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n1) done = face_fast_body<NS, 1, false>(M, P, j, Sf, SGm, dst, err);
   } else {
-    const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    j = n0 + n1 + q;
-    if (q < r2) done = face_fast_body<NS, 2, true>(M, P, j, Sf, SGm, G, err);
-    else if (q < n2) done = face_fast_body<NS, 2, false>(M, P, j, Sf, SGm, G, err);
+    if (q < r2) done = face_fast_body<NS, 2, true>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n2) done = face_fast_body<NS, 2, false>(M, P, j, Sf, SGm, dst, err);
   }
   if (!__syncthreads_or(!done)) return;
   if (!done) list[atomicAdd(&nlist, 1)] = j;
@@ -503,18 +524,25 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
                                                    const R *__restrict__ SF,
                                                    const R *__restrict__ SGm,
                                                    R *__restrict__ G, int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  __shared__ R gbuf[FV_BLOCK * NEQ];
   const int lb = logical_block();
   const int t = threadIdx.x;
-  if (lb < b0) {
-    const int j = lb * FV_BLOCK + t;
-    if (j < n0) face_body<MODEL, NS, 0, T, R>(M, P, j, S, SF, SGm, G, err);
-  } else if (lb < b0 + b1) {
-    const int q = (lb - b0) * FV_BLOCK + t;
-    if (q < n1) face_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SF, SGm, G, err);
+  int blk;
+  const int ax = axis_chunk(lb, b0, b1, blk);
+  const int q = blk * FV_BLOCK + t;
+  R *dst = gbuf + t * NEQ;
+  const int base = ax == 0 ? 0 : (ax == 1 ? n0 : n0 + n1);
+  const int n = ax == 0 ? n0 : (ax == 1 ? n1 : n2);
+  if (ax == 0) {
+    if (q < n0) face_body<MODEL, NS, 0, T, R>(M, P, q, S, SF, SGm, dst, err);
+  } else if (ax == 1) {
+    if (q < n1) face_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SF, SGm, dst, err);
   } else {
-    const int q = (lb - b0 - b1) * FV_BLOCK + t;
-    if (q < n2) face_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SF, SGm, G, err);
+    if (q < n2) face_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SF, SGm, dst, err);
   }
+  __syncthreads();
+  store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
 }
 
 /* RHS gather (residual_faces_AMR.c:173-175, order f = 0..5, k = 0..3) fused with
