@@ -159,17 +159,26 @@ def study_vortex(steps=50, ckpt=10, base_level=6):
         g.steps(steps, dt, 3)
         g.sync()
         t_fwd = time.perf_counter() - t0
-        g.set_state(S0)
-        t0 = time.perf_counter()
-        J = g.adj_forward(steps, ckpt, dt)
-        t_fwd_ck = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        lam = g.adj_backward()
-        t_adj = time.perf_counter() - t0
+        res = dict(forward_s=t_fwd)
+        # checkpoints every `ckpt` steps (segments recomputed in the reverse sweep), and
+        # one segment over the whole rollout (every stage input kept in HBM, no recompute)
+        for ck, key in ((ckpt, ""), (steps, "_full_store")):
+            g.set_state(S0)
+            g.sync()
+            t0 = time.perf_counter()
+            J = g.adj_forward(steps, ck, dt)
+            t_fwd_ck = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            lam = g.adj_backward()
+            t_adj = time.perf_counter() - t0
+            res.update({f"J{key}": J, f"forward_with_checkpoints_s{key}": t_fwd_ck,
+                        f"adjoint_s{key}": t_adj, f"adjoint_over_forward{key}": t_adj / t_fwd})
+            if not key:
+                grads[tag] = lam
+            else:
+                res["full_store_grad_equal"] = bool(np.array_equal(lam, grads[tag]))
         g.close()
-        grads[tag] = lam
-        out[tag] = dict(J=J, forward_s=t_fwd, forward_with_checkpoints_s=t_fwd_ck, adjoint_s=t_adj,
-                        adjoint_over_forward=t_adj / t_fwd)
+        out[tag] = res
     lp, lm = grads["plain"][:4], grads["mixed"][:4]
     out["mixed_grad_rel_l2"] = float(np.linalg.norm(lm - lp) / np.linalg.norm(lp))
     return out
