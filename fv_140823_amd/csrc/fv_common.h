@@ -115,12 +115,21 @@ __device__ __forceinline__ T limited(int lim, double eps, bool bnd, T d0, T d1, 
   T grad = (T)0;
   if (bnd) return grad;
   if (lim == FVG_MINMOD) {
-    /* both tests as selects (no divergent branch): a and b are formed either way and
-     * only picked when |d0| > 1e-10 and a b > 0 (false for NaN), as in the reference */
-    const T a = -d1 * im, b = d0 * ip;
-    const T prod = a * b;
-    const T pick = fabs(a) > fabs(b) ? b : a;
-    grad = (fabs(d0) > (T)1e-10 && prod > (T)0) ? pick : grad;
+    if constexpr (sizeof(T) == 4) {
+      /* FP32: both tests as selects (no divergent branch), a and b formed either way
+       * and only picked when |d0| > 1e-10 and a b > 0 (false for NaN); measured
+       * faster than the branches in FP32, slower in FP64.  Same values either way. */
+      const T a = -d1 * im, b = d0 * ip;
+      const T prod = a * b;
+      const T pick = fabs(a) > fabs(b) ? b : a;
+      grad = (fabs(d0) > (T)1e-10 && prod > (T)0) ? pick : grad;
+    } else {
+      if (fabs(d0) > (T)1e-10) {
+        T a = -d1 * im, b = d0 * ip;
+        T prod = a * b;
+        if (prod > (T)0) grad = fabs(a) > fabs(b) ? b : a;
+      }
+    }
   } else {
     const T e = (T)eps;
     T a = isinf(im) ? (T)0 : -d1 * im;

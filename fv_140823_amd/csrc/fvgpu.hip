@@ -128,9 +128,16 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
   }
 }
 
-/* The flux rows of a block's faces are staged in LDS (one row per thread) and then
- * written as the block's contiguous range [j0, j0 + nrow) of G with consecutive
- * words per lane: full-line stores instead of NEQ strided partial ones per thread. */
+/* Flux rows whose size is not a multiple of 16 bytes (5 doubles) are staged in LDS
+ * (one row per thread) and then written as the block's contiguous range
+ * [j0, j0 + nrow) of G with consecutive words per lane: full-line stores instead of
+ * NEQ strided partial ones per thread.  Rows of 4 doubles go out as aligned 16-byte
+ * pairs and FP32 rows are small: both measured faster stored directly. */
+template <int NEQ, typename R>
+struct RowStage {
+  static constexpr bool on = sizeof(R) == 8 && (NEQ * sizeof(R)) % 16 != 0;
+};
+
 template <int NEQ, typename R>
 __device__ __forceinline__ void store_rows(R *__restrict__ G, const R *gbuf, int j0, int nrow) {
   R *g = G + (size_t)j0 * NEQ;
@@ -245,7 +252,8 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
     int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
-  __shared__ R gbuf[FV_BLOCK * NEQ];
+  constexpr bool STAGE = RowStage<NEQ, R>::on;
+  __shared__ R gbuf[STAGE ? FV_BLOCK * NEQ : 1];
   const int lb = logical_block();
   const int t = threadIdx.x;
   /* axis a: faces [0, r_a) of the group are regular (a block straddles the end of
@@ -253,7 +261,8 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
   int blk;
   const int ax = axis_chunk(lb, b0, b1, blk);
   const int q = blk * FV_BLOCK + t;
-  R *dst = gbuf + t * NEQ;
+  const int jq = ax == 0 ? q : (ax == 1 ? n0 + q : n0 + n1 + q);
+  R *dst = STAGE ? gbuf + t * NEQ : G + (size_t)jq * NEQ;
   int base, n;
   if (ax == 0) {
     base = 0;
@@ -271,8 +280,10 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
     if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true>(M, P, n0 + n1 + q, S, SGm, dst, err);
     else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false>(M, P, n0 + n1 + q, S, SGm, dst, err);
   }
-  __syncthreads();
-  store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
+  if constexpr (STAGE) {
+    __syncthreads();
+    store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
+  }
 }
 
 /* Mixed form of the ideal gas (model 1): MUSCL from the FP32 state copy, the
@@ -525,15 +536,16 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
                                                    const R *__restrict__ SGm,
                                                    R *__restrict__ G, int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
-  __shared__ R gbuf[FV_BLOCK * NEQ];
+  constexpr bool STAGE = RowStage<NEQ, R>::on;
+  __shared__ R gbuf[STAGE ? FV_BLOCK * NEQ : 1];
   const int lb = logical_block();
   const int t = threadIdx.x;
   int blk;
   const int ax = axis_chunk(lb, b0, b1, blk);
   const int q = blk * FV_BLOCK + t;
-  R *dst = gbuf + t * NEQ;
   const int base = ax == 0 ? 0 : (ax == 1 ? n0 : n0 + n1);
   const int n = ax == 0 ? n0 : (ax == 1 ? n1 : n2);
+  R *dst = STAGE ? gbuf + t * NEQ : G + (size_t)(base + q) * NEQ;
   if (ax == 0) {
     if (q < n0) face_body<MODEL, NS, 0, T, R>(M, P, q, S, SF, SGm, dst, err);
   } else if (ax == 1) {
@@ -541,8 +553,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
   } else {
     if (q < n2) face_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SF, SGm, dst, err);
   }
-  __syncthreads();
-  store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
+  if constexpr (STAGE) {
+    __syncthreads();
+    store_rows<NEQ, R>(G, gbuf, base + blk * FV_BLOCK, min(FV_BLOCK, n - blk * FV_BLOCK));
+  }
 }
 
 /* RHS gather (residual_faces_AMR.c:173-175, order f = 0..5, k = 0..3) fused with
