@@ -370,23 +370,25 @@ template <typename T>
 FV_HD __forceinline__ void state_ideal_vjp(const DevParams &P, const FState<T> &s, T rb, T ub[3],
                                            T eb, T pb, T cb, T *Ub) {
   const T g = (T)P.gamma, pinf = (T)P.pinf;
-  /* c = sqrt(q), q = g (p + pinf) / r */
-  const T q = g * (s.p + pinf) / s.r;
-  const T qb = cb / ((T)2 * s.c);
-  pb += qb * g / s.r;
-  rb -= qb * q / s.r;
-  /* p = r (g - 1) ei - g pinf, ei = e / r - |u|^2 / 2 */
-  const T ei = (s.e / s.r) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+  /* the forward's ir = 1/r (make_state_ideal); the bars multiply by it */
+  const T ir = qdiv((T)1.0, s.r);
+  /* c = sqrt(q), q = g (p + pinf) ir */
+  const T q = g * (s.p + pinf) * ir;
+  const T qb = qdiv(cb, (T)2 * s.c);
+  pb += qb * g * ir;
+  rb -= qb * q * ir;
+  /* p = r (g - 1) ei - g pinf, ei = e ir - |u|^2 / 2 */
+  const T ei = (s.e * ir) - (T)0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
   rb += pb * (g - (T)1.0) * ei;
   const T eib = pb * s.r * (g - (T)1.0);
-  eb += eib / s.r;
-  rb -= eib * s.e / (s.r * s.r);
+  eb += eib * ir;
+  rb -= eib * s.e * ir * ir;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     ub[i] -= eib * s.u[i];
-    /* u_i = U_{1+i} / r */
-    Ub[1 + i] += ub[i] / s.r;
-    rb -= ub[i] * s.u[i] / s.r;
+    /* u_i = U_{1+i} ir */
+    Ub[1 + i] += ub[i] * ir;
+    rb -= ub[i] * s.u[i] * ir;
   }
   Ub[0] += rb;
   Ub[4] += eb;
@@ -411,7 +413,7 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
   const T SL = lminL ? (uL - L.c) : (uR - R.c);
   const T num = (L.r * (uL * uL) + L.p) - (R.r * (uR * uR) + R.p) - (SL * ruL) + (SR * ruR);
   const T den = (ruL - ruR - (SL * L.r) + (SR * R.r));
-  const T SS = num / den;
+  const T SS = qdiv(num, den);
   int region = 0;
   if (zero <= SL) region = 1;
   else if (SL <= zero && zero <= SS) region = 2;
@@ -433,11 +435,13 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
 #pragma unroll
   for (int v = 0; v < 5; ++v) Fb[v] = ob[v] * s;
   if (region == 2 || region == 3) {
+    /* forward values as hllc_face forms them; the bars multiply by reciprocals */
     const T W = Kr * (SK - uK);
     const T B = SK - SS;
-    const T cs = W / B;
-    const T Y = SS + (Kp / W);
-    const T X = (Ke / Kr) + (SS - uK) * Y;
+    const T cs = qdiv(W, B);
+    const T Y = SS + qdiv(Kp, W);
+    const T X = qdiv(Ke, Kr) + (SS - uK) * Y;
+    const T iKr = qdiv((T)1, Kr), iW = qdiv((T)1, W), iB = qdiv((T)1, B);
     T Us[5];
     Us[0] = cs;
 #pragma unroll
@@ -458,16 +462,16 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
       else Kub[i] += Usb[1 + i] * cs;
     }
     const T Xb = Usb[4] * cs;
-    Keb += Xb / Kr;
-    Krb -= Xb * Ke / (Kr * Kr);
+    Keb += Xb * iKr;
+    Krb -= Xb * Ke * iKr * iKr;
     SSb += Xb * Y;
     uKb -= Xb * Y;
     const T Yb = Xb * (SS - uK);
     SSb += Yb;
-    Kpb += Yb / W;
-    T Wb = -Yb * Kp / (W * W);
-    Wb += csb / B;
-    const T Bb = -csb * cs / B;
+    Kpb += Yb * iW;
+    T Wb = -Yb * Kp * iW * iW;
+    Wb += csb * iB;
+    const T Bb = -csb * cs * iB;
     Krb += Wb * (SK - uK);
     SKb += Wb * Kr;
     uKb -= Wb * Kr;
@@ -507,8 +511,9 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
   if (region == 2) SLb += SKb;
   else if (region == 3) SRb += SKb;
   /* S* = num / den */
-  const T numb = SSb / den;
-  const T denb = -SSb * SS / den;
+  const T iden = qdiv((T)1, den);
+  const T numb = SSb * iden;
+  const T denb = -SSb * SS * iden;
   Lrb += numb * (uL * uL);
   uLb += numb * (T)2 * L.r * uL;
   Lpb += numb;
@@ -537,30 +542,34 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
   state_ideal_vjp<T>(P, R, Rrb, Rub, Reb, Rpb, Rcb, URb);
 }
 
-/* limited slope of reconctruct_V3.c with its partials w.r.t. delta0, delta1 */
+/* limited slope of reconctruct_V3.c with its partials w.r.t. delta0, delta1.  The
+ * distances d_cc enter as the reciprocals the forward limiter uses (ip = 1/d_cc(f),
+ * im = 1/d_cc(fo); inf on a boundary), so the slopes a, b are the forward's own;
+ * the van Albada partials share one reciprocal of the denominator. */
 FV_HD __forceinline__ double limited_d(int lim, double eps, bool bnd, double d0, double d1,
-                                       double dp1cc, double dm1cc, double &g0, double &g1) {
+                                       double ip, double im, double &g0, double &g1) {
   g0 = 0.0;
   g1 = 0.0;
   if (bnd) return 0.0;
   if (lim == 0) {
     if (fabs(d0) > 1e-10) {
-      double a = -d1 / dm1cc, b = d0 / dp1cc;
+      const double a = -d1 * im, b = d0 * ip;
       if (a * b > 0.0) {
-        if (fabs(a) > fabs(b)) { g0 = 1.0 / dp1cc; return b; }
-        g1 = -1.0 / dm1cc;
+        if (fabs(a) > fabs(b)) { g0 = ip; return b; }
+        g1 = -im;
         return a;
       }
     }
     return 0.0;
   }
-  const double a = dm1cc != 0.0 ? -d1 / dm1cc : 0.0;
-  const double b = dp1cc != 0.0 ? d0 / dp1cc : 0.0;
+  const double a = isinf(im) ? 0.0 : -d1 * im;
+  const double b = isinf(ip) ? 0.0 : d0 * ip;
   const double N = a * (b * b + eps) + b * (a * a + eps);
   const double D = a * a + b * b + 2.0 * eps;
-  const double ga = ((b * b + eps) + 2.0 * a * b) / D - N * 2.0 * a / (D * D);
-  const double gb = ((a * a + eps) + 2.0 * a * b) / D - N * 2.0 * b / (D * D);
-  if (dp1cc != 0.0) g0 = gb / dp1cc;
-  if (dm1cc != 0.0) g1 = -ga / dm1cc;
+  const double iD = 1.0 / D;
+  const double ga = ((b * b + eps) + 2.0 * a * b) * iD - N * 2.0 * a * (iD * iD);
+  const double gb = ((a * a + eps) + 2.0 * a * b) * iD - N * 2.0 * b * (iD * iD);
+  if (!isinf(ip)) g0 = gb * ip;
+  if (!isinf(im)) g1 = -ga * im;
   return N / D;
 }

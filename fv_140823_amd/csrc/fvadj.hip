@@ -261,8 +261,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
     for (int v = 0; v < 5; ++v) {
       const double dP = Up[v] - U[v], dM = Um[v] - U[v];
       double g0, g1, h0, h1;
-      limited_d(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc, g0, g1);
-      limited_d(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc, h0, h1);
+      limited_d(P.limiter, P.va_eps, bp, dP, dM, pic, mic, g0, g1);
+      limited_d(P.limiter, P.va_eps, bm, dM, dP, mic, pic, h0, h1);
       /* SF[fp] = U + pcf g(dP, dM), SF[fm] = U + mcf g(dM, dP) */
       double dPb = lp[v] * pcf * g0 + lm[v] * mcf * h1;
       double dMb = lp[v] * pcf * g1 + lm[v] * mcf * h0;
@@ -278,21 +278,42 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
   for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
 }
 
+/* the probe objective's gradient at state S, added to lam: dt Vol dp/dS */
+__device__ __forceinline__ void obj_grad_cell(const DevMesh &M, const DevParams &P,
+                                              const double *__restrict__ S, int i, double dt,
+                                              double *__restrict__ lam) {
+  double U[5];
+  load_S<5>(S, M.ns, i, U);
+  const double c = dt * cell_vol(M, i) * (P.gamma - 1.0);
+  const double r = U[0], u = U[1] / r, v = U[2] / r, w = U[3] / r;
+  lam[i] += c * 0.5 * (u * u + v * v + w * w);
+  lam[(size_t)1 * M.ns + i] -= c * u;
+  lam[(size_t)2 * M.ns + i] -= c * v;
+  lam[(size_t)3 * M.ns + i] -= c * w;
+  lam[(size_t)4 * M.ns + i] += c;
+}
+
 /* recon^T, part 2: nu_i = own bar + bars sent to i by its face neighbours (a
  * neighbour whose delta used the mean of 4 finer cells sends 1/4 to each), then
  * the stage's reverse update:
  *   mode 0 (last stage):  lamSN  = a lam;  lam = b lam + nu
  *   mode 1:               lamSN += a lam;  lam = b lam + nu
- *   mode 2 (first stage): lam = lamSN + a lam + nu */
+ *   mode 2 (first stage): lam = lamSN + a lam + nu
+ * objS (mode 2, optional): the step's input state S^n, whose objective term J_n the
+ * reverse sweep adds next; it is added here (the same additions k_obj_grad makes, in
+ * the same order), saving a launch per step. */
 template <bool NS>
-__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M,
+__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParams P,
                                                              const double *__restrict__ self,
                                                              const double *__restrict__ Pp,
                                                              const double *__restrict__ Pm,
                                                              const double *__restrict__ SGb,
                                                              double *__restrict__ lam,
                                                              double *__restrict__ lamSN, double a,
-                                                             double b, int mode) {
+                                                             double b, int mode,
+                                                             const double *__restrict__ objS,
+                                                             const uint8_t *__restrict__ mask,
+                                                             double dt) {
   const int i = logical_block() * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
   const uint32_t cf = M.cflag[i];
@@ -373,6 +394,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M,
     else if (mode == 1) { lamSN[o] += a * l; lam[o] = b * l + acc[v]; }
     else { lam[o] = lamSN[o] + a * l + acc[v]; }
   }
+  if (objS && mask[i]) obj_grad_cell(M, P, objS, i, dt, lam);
 }
 
 /* double-double accumulation (Knuth's TwoSum): (ah, al) += (bh, bl).  The
@@ -466,15 +488,7 @@ __global__ void k_obj_grad(DevMesh M, DevParams P, const double *__restrict__ S,
                            const uint8_t *__restrict__ mask, double dt, double *__restrict__ lam) {
   const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
   if (i >= M.N || !mask[i]) return;
-  double U[5];
-  load_S<5>(S, M.ns, i, U);
-  const double c = dt * cell_vol(M, i) * (P.gamma - 1.0);
-  const double r = U[0], u = U[1] / r, v = U[2] / r, w = U[3] / r;
-  lam[i] += c * 0.5 * (u * u + v * v + w * w);
-  lam[(size_t)1 * M.ns + i] -= c * u;
-  lam[(size_t)2 * M.ns + i] -= c * v;
-  lam[(size_t)3 * M.ns + i] -= c * w;
-  lam[(size_t)4 * M.ns + i] += c;
+  obj_grad_cell(M, P, S, i, dt, lam);
 }
 
 /* tangent of the reconstruction: SFd = Sd + dcf (g0 dPd + g1 dMd) */
@@ -510,8 +524,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_tlm_recon(DevMesh M, DevParams P,
       const double dP = Up[v] - U[v], dM = Um[v] - U[v];
       const double dPd = Dp[v] - D[v], dMd = Dm[v] - D[v];
       double g0, g1, h0, h1;
-      limited_d(P.limiter, P.va_eps, bp, dP, dM, pcc, mcc, g0, g1);
-      limited_d(P.limiter, P.va_eps, bm, dM, dP, mcc, pcc, h0, h1);
+      limited_d(P.limiter, P.va_eps, bp, dP, dM, pic, mic, g0, g1);
+      limited_d(P.limiter, P.va_eps, bm, dM, dP, mic, pic, h0, h1);
       SFd[((size_t)fp * 5 + v) * M.ns + i] = D[v] + pcf * (g0 * dPd + g1 * dMd);
       SFd[((size_t)fm * 5 + v) * M.ns + i] = D[v] + mcf * (h0 * dMd + h1 * dPd);
     }
@@ -634,8 +648,10 @@ static void obj_step(fvg_ctx *c, const double *S, const double *Sd, int n) {
 }
 
 /* the reverse of one stage taken at the stage's input state S:
- * nu = R'(S)^T (cdt lam / Vol), then lam / lamSN updated per `mode` */
-static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double cb, int mode) {
+ * nu = R'(S)^T (cdt lam / Vol), then lam / lamSN updated per `mode`; objS: the state
+ * whose objective gradient is added after the update (mode 2), or null */
+static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double cb, int mode,
+                      const double *objS = nullptr) {
   fv_adj_state *a = c->adj;
   DevMesh M = dev_mesh(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
@@ -673,11 +689,13 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
                        (const float *)lUL, (const float *)lUR, a->self, a->Pp, a->Pm);
   }
   if (ns)
-    hipLaunchKernelGGL(k_adj_recv_stage<true>, dim3(nc), b, 0, c->stream, M, a->self, a->Pp, a->Pm,
-                       (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode);
+    hipLaunchKernelGGL(k_adj_recv_stage<true>, dim3(nc), b, 0, c->stream, M, c->dp, a->self, a->Pp,
+                       a->Pm, (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode, objS, a->mask,
+                       a->dt);
   else
-    hipLaunchKernelGGL(k_adj_recv_stage<false>, dim3(nc), b, 0, c->stream, M, a->self, a->Pp, a->Pm,
-                       (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode);
+    hipLaunchKernelGGL(k_adj_recv_stage<false>, dim3(nc), b, 0, c->stream, M, c->dp, a->self, a->Pp,
+                       a->Pm, (const double *)a->SGb, a->lam, a->lamSN, ca, cb, mode, objS, a->mask,
+                       a->dt);
 }
 
 /* Segment store: 3 ckpt + 1 states.  Step m of a segment has its stage inputs at
@@ -802,12 +820,15 @@ int fvg_adj_backward(fvg_ctx *c, double *grad_host, void *grad_dev) {
     }
     for (int n = n1 - 1; n >= n0; --n) {
       const int m = n - n0;
-      /* d J_{n+1} / d S^{n+1} */
-      hipLaunchKernelGGL(k_obj_grad, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp,
-                         (const double *)seg_state(a, sg, m + 1, 0), a->mask, dt, a->lam);
+      /* d J_{n+1} / d S^{n+1}: for the last step here; for the others it was added by
+       * the first-stage reverse of step n + 1, whose input state is S^{n+1} */
+      if (n == a->nsteps - 1)
+        hipLaunchKernelGGL(k_obj_grad, dim3(nc), dim3(FV_BLOCK), 0, c->stream, M, c->dp,
+                           (const double *)seg_state(a, sg, m + 1, 0), a->mask, dt, a->lam);
       for (int k = 3; k >= 1; --k)
         stage_vjp(c, seg_state(a, sg, m, k - 1), cst[k - 1] * dt, ast[k - 1], bst[k - 1],
-                  k == 3 ? 0 : (k == 2 ? 1 : 2));
+                  k == 3 ? 0 : (k == 2 ? 1 : 2),
+                  (k == 1 && n >= 1) ? (const double *)seg_state(a, sg, m, 0) : nullptr);
     }
   }
   hipError_t e = hipGetLastError();
