@@ -584,18 +584,20 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   const int nrow = min(FV_BLOCK, M.N - i0);
   const int nval = nrow * NEQ;
   const size_t base = (size_t)i0 * NEQ;
-  /* the first stage of the out-of-place time step has SN == S: read it once */
+  /* The span's S / SN values are loaded into registers first and written to LDS only
+   * after the flux loads below have been issued, so that the two chains of loads
+   * overlap (on small meshes the stage has too few waves to hide one latency behind
+   * another).  The first stage of the out-of-place time step has SN == S: read it
+   * once. */
+  const bool one = FIRST || SN == S;
+  double ls[NEQ], ln[NEQ];
   if constexpr (DO_RK) {
-    if (FIRST || SN == S) {
-      for (int k = t; k < nval; k += FV_BLOCK) {
-        const double x = S[base + k];
-        shS[k] = x;
-        if constexpr (!FIRST) shN[k] = x;
-      }
-    } else {
-      for (int k = t; k < nval; k += FV_BLOCK) {
-        shS[k] = S[base + k];
-        shN[k] = SN[base + k];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) {
+      const int k = t + q * FV_BLOCK;
+      if (k < nval) {
+        ls[q] = S[base + k];
+        if (!one) ln[q] = SN[base + k];
       }
     }
   }
@@ -635,6 +637,16 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) bad |= (rhs[v] != rhs[v]);
     if (bad) atomicMin(err, i);
+  }
+  if constexpr (DO_RK) {
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) {
+      const int k = t + q * FV_BLOCK;
+      if (k < nval) {
+        shS[k] = ls[q];
+        if constexpr (!FIRST) shN[k] = one ? ls[q] : ln[q];
+      }
+    }
   }
   __syncthreads();
   if (live) {
