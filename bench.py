@@ -33,6 +33,8 @@ def _args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="blast", choices=["blast", "sod", "cavitation", "tgv", "vortex"])
     ap.add_argument("--evolve", type=int, default=200, help="untimed AMR steps before timing (blast)")
+    ap.add_argument("--host-flatten", action="store_true",
+                    help="blast evolution: flatten on the host after each adaptation (default: on the device)")
     ap.add_argument("--level", type=int, default=None, help="uniform level (sod/cavitation/tgv) or base level (blast/vortex)")
     ap.add_argument("--refine", type=float, default=0.01, help="blast: split where h|grad rho|/rho exceeds this")
     ap.add_argument("--acc-steps", type=int, default=100)
@@ -91,23 +93,28 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage):
     raise KeyError(kernel)
 
 
-def build_workload(name, evolve, quiet, level=None, refine=0.01):
+def build_workload(name, evolve, quiet, level=None, refine=0.01, device_flatten=True):
     import fv_140823_amd as fv
     from fv_140823_amd import configs, driver
     if name == "blast":
         F, p, meta = configs.blast(level or 5, 2)
-        g = fv.Solver.from_forest(F, p)
+        g = fv.Solver.from_forest(F, p, device_flatten=device_flatten)
         t0 = time.perf_counter()
         base = (level or 5) + 1
         # the density-gradient criterion on the device (the same flags as
         # configs.density_gradient_flags on the downloaded gradients)
         flags = dict(refine=refine, coarsen=refine / 5, base_level=base, kind=0)
         st = driver.evolve(F, g, evolve, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"],
-                           log=lambda s: log(s, quiet))
+                           log=lambda s: log(s, quiet), device_flatten=device_flatten)
         # final adaptation so the timed mesh matches the state
-        driver.remesh(F, g, flags, meta["max_level"])
+        rm = st["remesh"] + [driver.remesh(F, g, flags, meta["max_level"], device_flatten)]
+        per = {k: round(1e3 * sum(r[k] for r in rm) / len(rm), 3)
+               for k in ("download_s", "adapt_s", "flatten_upload_s")}
         meta = dict(meta, evolved_steps=evolve, evolved_time=st["time"],
-                    evolve_wall_s=time.perf_counter() - t0, remesh_count=len(st["remesh"]) + 1)
+                    evolve_wall_s=time.perf_counter() - t0, remesh_count=len(rm),
+                    remesh_ms_mean={"flags": per["download_s"], "host_adapt": per["adapt_s"],
+                                    "flatten": per["flatten_upload_s"]},
+                    flatten="device" if device_flatten else "host")
         F.set_state(g.get_state())
         g.close()
         return F, p, meta
@@ -255,7 +262,7 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     log(f"building workload {a.workload}", a.quiet)
-    F, p, meta = build_workload(a.workload, a.evolve, a.quiet, a.level, a.refine)
+    F, p, meta = build_workload(a.workload, a.evolve, a.quiet, a.level, a.refine, not a.host_flatten)
     mf = mesh_facts(F)
     log(f"mesh: {mf}", a.quiet)
     neq = fv.neq_of(p.model)

@@ -142,6 +142,19 @@ _DECLS = [
     ("fvg_adapt_flags", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.POINTER(C.c_int8)]),
     ("fvg_pressure", C.c_int, [_vp, _dp]),
+    ("fvg_remesh_begin", C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
+    ("fvg_remesh_end", C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, _dp, C.c_int]),
+    ("fvg_has_keys", C.c_int, [_vp]),
+    ("fvh_forest_remesh_device", C.c_int, [_vp, _vp, C.c_int]),
+    ("fvh_forest_leaves", _vp, [_vp]),
+    ("fvh_tables_from_owners", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int32), _dp, _dp, _dp, _dp,
+                                         C.POINTER(C.c_int32)]),
+    ("fvg_get_mesh", C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32), _dp]),
     ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
     ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),

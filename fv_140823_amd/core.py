@@ -248,12 +248,61 @@ class Solver:
         self.nface = 0
 
     @classmethod
-    def from_forest(cls, forest: Forest, p: FvgParams, device: int = 0, upload: bool = True):
+    def from_forest(cls, forest: Forest, p: FvgParams, device: int = 0, upload: bool = True,
+                    device_flatten: bool = False):
+        """A context on the forest's mesh, flattened on the host (fvh_flatten_run) or,
+        with device_flatten, built on the device from the leaf keys (fvg_remesh_*), which
+        lets later re-flattens carry the state over on the device."""
         s = cls(p, device)
-        s.set_mesh(FlatMesh(forest.run))
+        if device_flatten:
+            s.remesh(forest, carry_state=False)
+        else:
+            s.set_mesh(FlatMesh(forest.run))
         if upload:
             s.set_state(forest.get_state())
         return s
+
+    def remesh(self, forest: Forest, carry_state: bool = True) -> None:
+        """Rebuild the mesh on the device from the (adapted) forest's leaf keys.  With
+        carry_state the state comes from the previous device mesh by key (kept, copied
+        from the split parent, or the mean of the 8 merged kids); otherwise S is zero."""
+        self._ck(lib().fvh_forest_remesh_device(forest._h, self._h, int(bool(carry_state))),
+                 "fvh_forest_remesh_device")
+        sz = self.mesh_sizes()
+        self.ncell, self.nface = sz["ncell"], sz["nface"]
+
+    @property
+    def has_keys(self) -> bool:
+        return lib().fvg_has_keys(self._h) == 1
+
+    def mesh_sizes(self) -> dict:
+        s = (C.c_int32 * 12)()
+        n = None
+        self._ck(lib().fvg_get_mesh(self._h, s, n, n, n, n, n, n, n, n, n, n), "fvg_get_mesh")
+        return dict(ncell=s[0], nface=s[1], nface_axis=tuple(s[2:5]), nquad=s[5], nlevel=s[6],
+                    nface_reg=tuple(s[7:10]), has_keys=bool(s[10]))
+
+    def mesh_arrays(self) -> dict:
+        """The device mesh copied back (same keys and shapes as FlatMesh.arrays())."""
+        z = self.mesh_sizes()
+        N, F, Q, nl = z["ncell"], z["nface"], z["nquad"], z["nlevel"]
+        a = dict(cflag=np.empty(N, np.uint32), nbr=np.empty((6, N), np.int32),
+                 quad=np.zeros((Q, 4), np.int32), face_L=np.empty(F, np.int32),
+                 face_R=np.empty(F, np.int32), face_meta=np.empty(F, np.uint32),
+                 face_LL=np.empty(F, np.int32), face_RR=np.empty(F, np.int32),
+                 gslot=np.empty((6, N), np.int32), tab=np.empty(34 * nl, np.float64))
+        u32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint32))
+        i32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+        self._ck(lib().fvg_get_mesh(self._h, None, u32(a["cflag"]), i32(a["nbr"]),
+                                    i32(a["quad"]) if Q else None, i32(a["face_L"]),
+                                    i32(a["face_R"]), u32(a["face_meta"]), i32(a["face_LL"]),
+                                    i32(a["face_RR"]), i32(a["gslot"]), _dptr(a["tab"])),
+                 "fvg_get_mesh")
+        t = a.pop("tab")
+        a.update(h=t[:3 * nl].reshape(nl, 3), dist=t[3 * nl:21 * nl].reshape(nl, 3, 3, 2),
+                 area=t[21 * nl:24 * nl].reshape(nl, 3), vol=t[24 * nl:25 * nl].copy(),
+                 nface_axis=z["nface_axis"], nface_reg=z["nface_reg"])
+        return a
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -314,10 +314,26 @@ struct fvg_ctx {
 
   /* adjoint / tangent buffers (fvadj.hip), released with the mesh */
   struct fv_adj_state *adj = nullptr;
+
+  /* leaf keys of a mesh built on the device (fvmesh.hip) and their hash table
+   * (key -> cell); the next device re-flatten carries the state over by key.
+   * Null for a mesh uploaded by fvg_set_mesh. */
+  uint64_t *keys = nullptr, *hkeys = nullptr;
+  int32_t *hval = nullptr;
+  int hbits = 0;
+  /* a device re-flatten between fvg_remesh_begin and fvg_remesh_end */
+  struct fv_remesh *rm = nullptr;
 };
 
 /* releases ctx->adj; defined in fvadj.hip */
 void fv_adj_release(fvg_ctx *c);
+/* releases ctx->rm; defined in fvmesh.hip */
+void fv_remesh_release(fvg_ctx *c);
+/* mesh arrays and tables (defined in fvgpu.hip, shared with fvmesh.hip) */
+int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
+                  const int32_t nreg[3], bool topo);
+int fv_upload_tables(fvg_ctx *c, int nl, const double *h, const double *dist, const double *area,
+                     const double *vol);
 /* forward pieces shared with fvadj.hip (defined in fvgpu.hip) */
 void fv_launch_recon(fvg_ctx *c, const double *S);
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs);

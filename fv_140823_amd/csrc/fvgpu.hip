@@ -857,7 +857,8 @@ static int check_params(const fvg_params *p) {
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, c->tab,
-                  c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf};
+                  c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf, c->keys,
+                  c->hkeys, c->hval};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->fLL = c->fRR = c->gslot = nullptr;
@@ -866,8 +867,86 @@ static void free_mesh(fvg_ctx *c) {
   c->SF = c->SGm = c->G = nullptr;
   c->Sf = nullptr;
   c->Sx = nullptr;
+  c->keys = c->hkeys = nullptr;
+  c->hval = nullptr;
+  c->hbits = 0;
   c->sf_valid = false;
   c->has_mesh = false;
+}
+
+/* Frees the previous mesh and allocates the arrays of a mesh of N cells, F faces, Q
+ * quads and nl table rows; the state arrays are zeroed (padding included) so that no
+ * kernel reads uninitialised memory.  topo = false leaves cflag / nbr to the caller
+ * (the device re-flatten hands over the ones it built). */
+int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
+                  const int32_t nreg[3], bool topo) {
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  c->N = N;
+  c->ns = (int)round_up(N, 64);
+  c->F = F;
+  c->nfp = (int)round_up(F, 64);
+  c->Q = Q;
+  c->nlevel = nl;
+  for (int a = 0; a < 3; ++a) {
+    c->nfa[a] = nfa[a];
+    c->nreg[a] = nreg[a];
+  }
+  const int ns = c->ns, neq = c->neq;
+  const size_t rs = c->rsize;
+  if (topo) {
+    HIPCHK(c, hipMalloc(&c->cflag, (size_t)ns * 4));
+    HIPCHK(c, hipMalloc(&c->nbr, (size_t)6 * ns * 4));
+  }
+  HIPCHK(c, hipMalloc(&c->gslot, (size_t)6 * ns * 4));
+  HIPCHK(c, hipMalloc(&c->quad, (size_t)4 * (Q ? Q : 1) * 4));
+  HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fLL, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fRR, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
+  HIPCHK(c, hipMalloc(&c->tab, (size_t)34 * nl * 8));
+  HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SN, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->RHS, (size_t)neq * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SG, (size_t)(neq + 1) * 3 * ns * 8));
+  HIPCHK(c, hipMalloc(&c->SF, (size_t)6 * neq * ns * rs));
+  HIPCHK(c, hipMalloc(&c->SGm, (size_t)9 * ns * rs));
+  HIPCHK(c, hipMalloc(&c->G, (size_t)neq * c->nfp * rs));
+  HIPCHK(c, hipMalloc(&c->Sx, (size_t)neq * ns * 8));
+  if (rs == 4 && c->prm.model == 1) HIPCHK(c, hipMalloc(&c->Sf, (size_t)neq * ns * 4));
+  c->sf_valid = false;
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SN, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->Sx, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
+  HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)9 * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
+  return FVG_OK;
+}
+
+/* Per-level tables (h, distance() factors, areas, volumes and the reciprocals of
+ * the d_cc factors) into c->tab, nl rows each. */
+int fv_upload_tables(fvg_ctx *c, int nl, const double *h, const double *dist, const double *area,
+                     const double *vol) {
+  std::vector<double> tab((size_t)34 * nl, 0.0);
+  memcpy(tab.data(), h, (size_t)3 * nl * 8);
+  memcpy(tab.data() + 3 * nl, dist, (size_t)18 * nl * 8);
+  memcpy(tab.data() + 21 * nl, area, (size_t)3 * nl * 8);
+  memcpy(tab.data() + 24 * nl, vol, (size_t)nl * 8);
+  /* reciprocals of the d_cc factors (a zero distance gives inf, as 1/0 does) */
+  for (int k = 0; k < 9 * nl; ++k) {
+    const double dcc = dist[2 * k + 1];
+    tab[25 * nl + k] = dcc != 0.0 ? 1.0 / dcc : HUGE_VAL;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream));
+  /* the host vector must outlive the copy */
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVG_OK;
 }
 
 extern "C" {
@@ -900,6 +979,7 @@ int fvg_destroy(fvg_ctx *c) {
   if (!c) return FVG_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fv_remesh_release(c);
   free_mesh(c);
   if (c->err) (void)hipFree(c->err);
   if (c->red) (void)hipFree(c->red);
@@ -994,41 +1074,9 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
     if ((mt & 7) > 5 || ((mt >> 3) & 7) > 5 || lev < 1 || lev >= nl || ((mt >> 6) & 3) == 3)
       return set_err(c, FVG_ERR_ARG, "bad face meta");
   }
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  free_mesh(c);
-  c->N = N;
-  c->ns = (int)round_up(N, 64);
-  c->F = F;
-  c->nfp = (int)round_up(F, 64);
-  c->Q = Q;
-  c->nlevel = nl;
-  for (int a = 0; a < 3; ++a) {
-    c->nfa[a] = m->nface_axis[a];
-    c->nreg[a] = m->nface_reg[a];
-  }
-  const int ns = c->ns, neq = c->neq;
-  const size_t rs = c->rsize;
-  HIPCHK(c, hipMalloc(&c->cflag, (size_t)ns * 4));
-  HIPCHK(c, hipMalloc(&c->nbr, (size_t)6 * ns * 4));
-  HIPCHK(c, hipMalloc(&c->gslot, (size_t)6 * ns * 4));
-  HIPCHK(c, hipMalloc(&c->quad, (size_t)4 * (Q ? Q : 1) * 4));
-  HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fLL, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fRR, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->tab, (size_t)34 * nl * 8));
-  HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SN, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->RHS, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SG, (size_t)(neq + 1) * 3 * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SF, (size_t)6 * neq * ns * rs));
-  HIPCHK(c, hipMalloc(&c->SGm, (size_t)9 * ns * rs));
-  HIPCHK(c, hipMalloc(&c->G, (size_t)neq * c->nfp * rs));
-  HIPCHK(c, hipMalloc(&c->Sx, (size_t)neq * ns * 8));
-  if (rs == 4 && c->prm.model == 1) HIPCHK(c, hipMalloc(&c->Sf, (size_t)neq * ns * 4));
-  c->sf_valid = false;
+  int rc = fv_alloc_mesh(c, N, F, Q, nl, m->nface_axis, m->nface_reg, true);
+  if (rc) return rc;
+  const int ns = c->ns;
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemcpyAsync(c->cflag, m->cflag, (size_t)N * 4, hipMemcpyHostToDevice, st));
   for (int f = 0; f < 6; ++f) {
@@ -1048,26 +1096,8 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
     HIPCHK(c, hipMemsetAsync(c->fRR, 0xff, (size_t)F * 4, st));
   }
   HIPCHK(c, hipMemcpyAsync(c->fmeta, m->face_meta, (size_t)F * 4, hipMemcpyHostToDevice, st));
-  std::vector<double> tab((size_t)34 * nl, 0.0);
-  memcpy(tab.data(), m->h, (size_t)3 * nl * 8);
-  memcpy(tab.data() + 3 * nl, m->dist, (size_t)18 * nl * 8);
-  memcpy(tab.data() + 21 * nl, m->area, (size_t)3 * nl * 8);
-  memcpy(tab.data() + 24 * nl, m->vol, (size_t)nl * 8);
-  /* reciprocals of the d_cc factors (a zero distance gives inf, as 1/0 does) */
-  for (int k = 0; k < 9 * nl; ++k) {
-    const double dcc = m->dist[2 * k + 1];
-    tab[25 * nl + k] = dcc != 0.0 ? 1.0 / dcc : HUGE_VAL;
-  }
-  HIPCHK(c, hipMemcpyAsync(c->tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st));
-  /* zero the padding and every state array so no kernel reads uninitialised memory */
-  HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
-  HIPCHK(c, hipMemsetAsync(c->SN, 0, (size_t)neq * ns * 8, st));
-  HIPCHK(c, hipMemsetAsync(c->Sx, 0, (size_t)neq * ns * 8, st));
-  HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
-  HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
-  HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
-  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)9 * ns * rs, st));
-  HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
+  rc = fv_upload_tables(c, nl, m->h, m->dist, m->area, m->vol);
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(st));
   c->has_mesh = true;
   return FVG_OK;

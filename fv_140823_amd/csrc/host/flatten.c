@@ -61,6 +61,78 @@ static int set_table(double *slot, double v, unsigned char *have) {
   return close_rel(*slot, v) ? 0 : FVG_ERR_UNSUPPORTED;
 }
 
+/* Finer levels that only appear as sub-face partners have their h from 2:1 kids; any
+ * level without h up to maxlev + 1 takes half the coarser one.  Then the distance.c
+ * factors per level L of the cell, axis a, class. */
+static void fill_h_and_dist(double *h, unsigned char *hav_h, double *dist, int maxlev) {
+  for (int L = 1; L <= maxlev + 1 && L < FV_MAXLEV; ++L)
+    for (int a = 0; a < 3; ++a)
+      if (!hav_h[3 * L + a] && L > 1 && hav_h[3 * (L - 1) + a]) {
+        h[3 * L + a] = 0.5 * h[3 * (L - 1) + a];
+        hav_h[3 * L + a] = 1;
+      }
+  for (int L = 1; L <= maxlev; ++L)
+    for (int a = 0; a < 3; ++a) {
+      double *d = &dist[((L * 3 + a) * 3) * 2];
+      double hL = h[3 * L + a];
+      double t = 0.5 * sqrt(hL * hL); /* same level (distance.c:85-87) */
+      d[0] = 0.5 * t;
+      d[1] = t;
+      t = 0.5 * sqrt(hL * hL); /* finer neighbour: its parent's centre (distance.c:112-114) */
+      d[2] = 0.5 * t;
+      d[3] = 0.5 * t + 0.25 * t;
+      double hC = L > 1 ? h[3 * (L - 1) + a] : 0.0;
+      t = 0.25 * sqrt(hC * hC); /* coarser neighbour: own parent's centre (distance.c:141-143) */
+      d[4] = 0.25 * t;
+      d[5] = 0.25 * t + 0.5 * t;
+    }
+}
+
+int fvh_tables_from_owners(BRANCH *const *lv, int n, const int32_t *own, double *h, double *dist,
+                           double *area, double *vol, int32_t *nlevel) {
+  if (!lv || n <= 0 || !own || !h || !dist || !area || !vol || !nlevel) return FVG_ERR_ARG;
+  memset(h, 0, sizeof(double) * FV_MAXLEV * 3);
+  memset(dist, 0, sizeof(double) * FV_MAXLEV * 18);
+  memset(area, 0, sizeof(double) * FV_MAXLEV * 3);
+  memset(vol, 0, sizeof(double) * FV_MAXLEV);
+  unsigned char hav_h[FV_MAXLEV * 3] = {0};
+  int maxlev = 0;
+  for (int L = 1; L < FV_MAXLEV; ++L) {
+    const int iv = own[32 + L], ih = own[L];
+    if (iv >= n || ih >= n) return FVG_ERR_ARG;
+    if (iv >= 0) {
+      if (lv[iv]->level != L) return FVG_ERR_ARG;
+      vol[L] = lv[iv]->cl->Vol;
+      maxlev = L;
+    }
+    if (ih >= 0) {
+      /* the first leaf of level L or L + 1: its own size, or its parent's */
+      const BRANCH *b = lv[ih];
+      if (b->level != L && (b->level != L + 1 || !b->prnt)) return FVG_ERR_ARG;
+      for (int a = 0; a < 3; ++a) {
+        h[3 * L + a] = cell_h(b->level == L ? b : b->prnt, a);
+        hav_h[3 * L + a] = 1;
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      const int ia = own[64 + 3 * L + a];
+      if (ia < 0) continue;
+      if (ia >= n) return FVG_ERR_ARG;
+      /* its first face of axis a whose finer side has level L, first sub-face */
+      const BRANCH *b = lv[ia];
+      int f = 0;
+      for (; f < 6; ++f)
+        if (FV_FACE_AXIS[f] == a && (b->nsfc[f] == 4 ? b->level + 1 : b->level) == L) break;
+      if (f == 6) return FVG_ERR_ARG;
+      area[3 * L + a] = b->cl->Area[f][0];
+    }
+  }
+  if (maxlev == 0 || maxlev >= FV_MAXLEV - 1) return FVG_ERR_ARG;
+  fill_h_and_dist(h, hav_h, dist, maxlev);
+  *nlevel = maxlev + 2;
+  return FVG_OK;
+}
+
 void fvh_flat_destroy(fvh_flat *fl) {
   if (!fl) return;
   free(fl->cflag); free(fl->nbr); free(fl->quad_nbr); free(fl->face_L); free(fl->face_R);
@@ -112,30 +184,7 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
     }
   }
   if (rc) goto fail;
-  /* finer levels that only appear as sub-face partners still have their h from 2:1 kids;
-     fill any level without h by halving the coarser one */
-  for (int L = 1; L <= maxlev + 1 && L < FV_MAXLEV; ++L)
-    for (int a = 0; a < 3; ++a)
-      if (!hav_h[3 * L + a] && L > 1 && hav_h[3 * (L - 1) + a]) {
-        fl->h[3 * L + a] = 0.5 * fl->h[3 * (L - 1) + a];
-        hav_h[3 * L + a] = 1;
-      }
-  /* distance.c factors per level L of the cell, axis a, class */
-  for (int L = 1; L <= maxlev; ++L)
-    for (int a = 0; a < 3; ++a) {
-      double *d = &fl->dist[((L * 3 + a) * 3) * 2];
-      double hL = fl->h[3 * L + a];
-      double t = 0.5 * sqrt(hL * hL); /* same level (distance.c:85-87) */
-      d[0] = 0.5 * t;
-      d[1] = t;
-      t = 0.5 * sqrt(hL * hL); /* finer neighbour: its parent's centre (distance.c:112-114) */
-      d[2] = 0.5 * t;
-      d[3] = 0.5 * t + 0.25 * t;
-      double hC = L > 1 ? fl->h[3 * (L - 1) + a] : 0.0;
-      t = 0.25 * sqrt(hC * hC); /* coarser neighbour: own parent's centre (distance.c:141-143) */
-      d[4] = 0.25 * t;
-      d[5] = 0.25 * t + 0.5 * t;
-    }
+  fill_h_and_dist(fl->h, hav_h, fl->dist, maxlev);
   /* ---- per-leaf classes and neighbours */
   int nquad = 0;
   for (int i = 0; i < N && !rc; ++i) {

@@ -650,6 +650,35 @@ int fvh_forest_max_level(fvh_forest *fr) {
   return m;
 }
 
+BRANCH *const *fvh_forest_leaves(fvh_forest *fr) { return fr ? (BRANCH *const *)fr->leaves : NULL; }
+
+int fvh_forest_remesh_device(fvh_forest *fr, fvg_ctx *ctx, int carry_state) {
+  if (!fr || !ctx) return FVG_ERR_ARG;
+  const int n = fr->nleaves;
+  if (n <= 0) return FVG_ERR_ARG;
+  int32_t *lev = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *ijk = (int32_t *)malloc((size_t)3 * n * sizeof(int32_t));
+  double *tab = (double *)malloc(sizeof(double) * 32 * (3 + 18 + 3 + 1));
+  if (!lev || !ijk || !tab) { free(lev); free(ijk); free(tab); return FVG_ERR_NOMEM; }
+  for (int i = 0; i < n; ++i) {
+    const BRANCH *b = fr->leaves[i];
+    lev[i] = b->level;
+    ijk[3 * i] = b->fv_ijk[0];
+    ijk[3 * i + 1] = b->fv_ijk[1];
+    ijk[3 * i + 2] = b->fv_ijk[2];
+  }
+  int32_t own[FVG_REMESH_OWNERS];
+  int rc = fvg_remesh_begin(ctx, n, lev, ijk, fr->nroot, fr->periodic, fr->bc, own);
+  free(lev);
+  free(ijk);
+  double *h = tab, *dist = tab + 32 * 3, *area = dist + 32 * 18, *vol = area + 32 * 3;
+  int32_t nl = 0;
+  if (!rc) rc = fvh_tables_from_owners(fr->leaves, n, own, h, dist, area, vol, &nl);
+  if (!rc) rc = fvg_remesh_end(ctx, nl, h, dist, area, vol, carry_state);
+  free(tab);
+  return rc;
+}
+
 int fvh_forest_leaf_info(fvh_forest *fr, double *xc, int32_t *level) {
   if (!fr) return FVG_ERR_ARG;
   for (int i = 0; i < fr->nleaves; ++i) {

@@ -167,6 +167,12 @@ int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p);
 int fvh_flatten_run(RUN *run, fvh_flat **out);
 const fvg_mesh *fvh_flat_mesh(const fvh_flat *fl);
 void fvh_flat_destroy(fvh_flat *fl);
+/* The per-level tables of leaves lv[0..n) (list order) exactly as fvh_flatten_run derives
+ * them, evaluated only at the leaves that first set each entry (owners, as
+ * fvg_remesh_begin reports them; -1 for an entry no leaf sets).  h, dist, area, vol hold
+ * 32 rows; *nlevel receives the rows used (deepest level + 2). */
+int fvh_tables_from_owners(BRANCH *const *lv, int n, const int32_t *owners, double *h,
+                           double *dist, double *area, double *vol, int32_t *nlevel);
 
 /* ---------------------------------------------------------------- device context */
 int fvg_create(int device, const fvg_params *p, fvg_ctx **out);
@@ -195,6 +201,40 @@ int fvg_adapt_flags(fvg_ctx *ctx, int kind, double refine, double coarsen, int b
 /* Cell pressure (host f64 [ncell]) from the device S, with the FP64 EOS of the gradient
  * kernel: the P column of the export (volmvalues.c:22-40, exportfield_linked_V3.c:390). */
 int fvg_pressure(fvg_ctx *ctx, double *p);
+
+/* ---------------------------------------------------------------- device re-flatten
+ * After a host adaptation the mesh is rebuilt on the device from the leaves' keys, and
+ * the state carried over from the previous mesh by key (no counterpart in the reference,
+ * whose meshadaptation rebuilds its lists on the host; this replaces fvh_flatten_run +
+ * fvg_set_mesh + the state round trip through the host for forests of this library).
+ * begin: ncell leaves in list order, level[i] (roots = 1) and ijk[3i..3i+2], the global
+ * cell index at that level (roots form an nroot grid); periodic axes and the bc of the six
+ * domain sides (-x,+x,-y,+y,-z,+z; 1 or 2).  Builds classes, neighbours and face counts on
+ * the device and returns, in owners[FVG_REMESH_OWNERS], the leaves that first set each
+ * per-level table entry (rows [0,32) h, [32,64) volume, [64,160) area per level and
+ * axis; -1 if unset).  FVG_ERR_STATE if a face has no leaf within one level (2:1).
+ * end: the tables (32-row arrays as fvh_tables_from_owners fills them, nlevel rows used)
+ * complete the mesh, which replaces the current one.  carry_state != 0 fills S from the
+ * previous mesh (which must have been built by this path): a leaf present before keeps its
+ * S, a split leaf's kids take the parent's S, a merged parent the mean of its 8 kids. */
+#define FVG_REMESH_OWNERS 160
+int fvg_remesh_begin(fvg_ctx *ctx, int32_t ncell, const int32_t *level, const int32_t *ijk,
+                     const int32_t nroot[3], const int32_t periodic[3], const int32_t bc[6],
+                     int32_t *owners);
+int fvg_remesh_end(fvg_ctx *ctx, int32_t nlevel, const double *h, const double *dist,
+                   const double *area, const double *vol, int carry_state);
+int fvg_has_keys(const fvg_ctx *ctx);  /* 1 if the current mesh was built by fvg_remesh_* */
+/* The forest's leaves in list order (valid until the next adaptation). */
+BRANCH *const *fvh_forest_leaves(fvh_forest *fr);
+/* Both steps for a forest of this library: keys from its leaf list, tables from its leaves. */
+int fvh_forest_remesh_device(fvh_forest *fr, fvg_ctx *ctx, int carry_state);
+/* The device mesh back on the host (for comparison with a host-flattened mesh; no
+ * counterpart in the reference).  sizes[12] = ncell, nface, nface_axis[3], nquad, nlevel,
+ * nface_reg[3], has_keys, 0.  Any array may be null; lengths as in fvg_mesh, tab holds
+ * 34 * nlevel doubles (h, dist, area, vol, 1/d_cc). */
+int fvg_get_mesh(fvg_ctx *ctx, int32_t *sizes, uint32_t *cflag, int32_t *nbr, int32_t *quad_nbr,
+                 int32_t *face_L, int32_t *face_R, uint32_t *face_meta, int32_t *face_LL,
+                 int32_t *face_RR, int32_t *gslot, double *tab);
 int fvg_rhs(fvg_ctx *ctx);                   /* RHS from S / SF / SG (RHS.c:3-13) */
 int fvg_rk(fvg_ctx *ctx, int irk, double dt);/* stage formula irk = 0..4 (RK.c:16-51) */
 /* One time step: RK(0), then per stage [faceinterp] [gradcalc] RHS RK(k) with the RHS

@@ -1,0 +1,164 @@
+"""Device re-flatten (fvmesh.hip) against the host flattener, and the state carried over
+on the device against the host round trip.
+
+- the mesh built on the device from the leaf keys equals fvh_flatten_run's, array by array
+  and table by table, bit for bit (a box with non-dyadic cell sizes, uniform, 2:1 periodic,
+  two roots, the blast after its pre-refinement);
+- after adaptations with splits and merges, the state carried over on the device equals
+  the host path's (download, split copies and merge means on the leaves, upload), bit for
+  bit, and so do the steps that follow, plain and mixed;
+- the blast evolved with adaptation every 10 steps ends in the same state either way;
+- carrying the state needs a device-built mesh; keys that break 2:1 are refused.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fv_140823_amd as fv
+from fv_140823_amd import configs, driver
+from fv_140823_amd._capi import lib
+from helpers import amr, state, uniform
+from test_remesh_tables import skewed_amr
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("cflag", "nbr", "quad", "face_L", "face_R", "face_meta", "face_LL", "face_RR", "gslot",
+        "h", "dist", "area", "vol", "nface_axis", "nface_reg")
+
+
+def host_arrays(F):
+    fl = fv.FlatMesh(F.run)
+    A = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in fl.arrays().items()}
+    fl.close()
+    return A
+
+
+def assert_same_mesh(g, F):
+    A, D = host_arrays(F), g.mesh_arrays()
+    for k in KEYS:
+        a, d = A[k], D[k]
+        if isinstance(a, tuple):
+            assert a == d, k
+        else:
+            assert a.shape == d.shape and np.array_equal(a, d), k
+
+
+def _forests():
+    yield "skewed", skewed_amr()
+    yield "uniform", uniform(3, 5)
+    yield "amr periodic", amr(3, 5, periodic=(1, 0, 1))
+    yield "two roots", fv.Forest((2, 1, 1), (0, 0, 0), (2, 1, 1), 2, (2, 2, 1, 2, 2, 2), (0, 0, 0), 5)
+    F, _, _ = configs.blast(3, 2)
+    yield "blast", F
+
+
+@pytest.mark.parametrize("name,F", list(_forests()), ids=lambda v: v if isinstance(v, str) else "")
+def test_device_mesh_equals_host_flattener(name, F):
+    print(f"case device re-flatten {name} leaves={F.nleaves}", flush=True)
+    g = fv.Solver.from_forest(F, fv.params(1), device_flatten=True)
+    assert g.has_keys
+    assert_same_mesh(g, F)
+    g.close()
+
+
+def _flags(F, k, max_level):
+    """Split a slab near x = 0.2 (moving with k), merge above x = 0.55: both kinds."""
+    xc, lev = F.leaf_info()
+    f = np.zeros(F.nleaves, np.int8)
+    f[(np.abs(xc[:, 0] - 0.2 - 0.05 * k) < 0.08) & (lev < max_level)] = 1
+    f[(xc[:, 0] > 0.55) & (lev > 4)] = -1
+    return f
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 1, 0)])
+def test_carried_state_equals_host_round_trip(precision, periodic):
+    print(f"case carry {precision} periodic={periodic}", flush=True)
+    F1, F2 = amr(3, 5, periodic=periodic), amr(3, 5, periodic=periodic)
+    xc, _ = F1.leaf_info()
+    S0 = state(1, xc, seed=5)
+    F1.set_state(S0)
+    F2.set_state(S0)
+    P = fv.params(1, precision=precision)
+    g1 = fv.Solver.from_forest(F1, P, device_flatten=True)
+    g2 = fv.Solver.from_forest(F2, P)
+    dt = 0.5 * g2.cfl_dt(0.4)
+    g1.steps(3, dt, 3)
+    g2.steps(3, dt, 3)
+    assert np.array_equal(g1.get_state(), g2.get_state())
+    max_level = 6
+    merged = split = 0
+    for k in range(3):
+        flags = _flags(F1, k, max_level)
+        right = int((F1.leaf_info()[0][:, 0] > 0.55).sum())
+        F1.adapt(flags, max_level)
+        g1.remesh(F1, carry_state=True)
+        F2.set_state(g2.get_state())
+        F2.adapt(flags, max_level)
+        g2.set_mesh(fv.FlatMesh(F2.run))
+        g2.set_state(F2.get_state())
+        split += int((flags > 0).sum())
+        merged += right - int((F1.leaf_info()[0][:, 0] > 0.55).sum())
+        assert F1.nleaves == F2.nleaves
+        assert_same_mesh(g1, F2)
+        assert np.array_equal(g1.get_state(), g2.get_state()), f"cycle {k}"
+        g1.steps(3, dt, 3)
+        g2.steps(3, dt, 3)
+        assert np.array_equal(g1.get_state(), g2.get_state()), f"steps after cycle {k}"
+    assert split > 0 and merged > 0
+    g1.close()
+    g2.close()
+
+
+def test_blast_evolution_device_and_host_flatten_agree():
+    print("case blast evolve device vs host flatten", flush=True)
+    out = []
+    for dev in (True, False):
+        F, p, meta = configs.blast(3, 2)
+        g = fv.Solver.from_forest(F, p, device_flatten=dev)
+        flags = dict(refine=0.01, coarsen=0.002, base_level=4, kind=0)
+        st = driver.evolve(F, g, 30, meta["cfl"], 3, 10, flags, meta["max_level"], device_flatten=dev)
+        assert st["remesh"] and all(r["carried_on_device"] == dev for r in st["remesh"])
+        out.append((F.nleaves, g.get_state()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_carry_needs_a_device_mesh():
+    print("case carry without keys", flush=True)
+    F = amr(2, 5)
+    g = fv.Solver.from_forest(F, fv.params(1))
+    assert not g.has_keys
+    with pytest.raises(fv.FvError):
+        g.remesh(F, carry_state=True)
+    g.close()
+
+
+def test_keys_breaking_two_to_one_are_refused():
+    """Level-2 cells with one octant refined twice: a level-4 cell then faces a level-2
+    cell across x = 1/2."""
+    print("case 2:1 violation refused", flush=True)
+    keys = []
+    for c in range(8):
+        i, j, k = c & 1, (c >> 1) & 1, (c >> 2) & 1
+        if c:
+            keys.append((2, i, j, k))
+    for c in range(8):
+        i, j, k = c & 1, (c >> 1) & 1, (c >> 2) & 1
+        if c != 7:
+            keys.append((3, i, j, k))
+    for c in range(8):
+        keys.append((4, 2 + (c & 1), 2 + ((c >> 1) & 1), 2 + ((c >> 2) & 1)))
+    lev = np.array([k[0] for k in keys], np.int32)
+    ijk = np.array([k[1:] for k in keys], np.int32).ravel()
+    g = fv.Solver(fv.params(1))
+    i3 = lambda *v: (C.c_int32 * len(v))(*v)
+    own = (C.c_int32 * 160)()
+    p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    rc = lib().fvg_remesh_begin(g._h, len(keys), p32(lev), p32(ijk), i3(1, 1, 1), i3(0, 0, 0),
+                                i3(2, 2, 2, 2, 2, 2), own)
+    assert rc < 0
+    assert b"2:1" in lib().fvg_last_error(g._h)
+    g.close()
