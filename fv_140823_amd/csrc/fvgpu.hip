@@ -608,26 +608,42 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   uint32_t cf = 0;
   if (live) {
     cf = M.cflag[i];
+    /* the six slots and the first flux row of each face are loaded up front, outside
+     * any branch, so that the loads of all faces are in flight together (a load
+     * under a per-face branch is waited for before the next face's is issued); a
+     * face without a slot reads row 0 and ignores it; only the three further rows
+     * of a finer face are loaded in its branch */
+    int g[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) g[f] = M.gslot[(size_t)f * M.ns + i];
+    R r0[6][NEQ];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      const int g = M.gslot[(size_t)f * M.ns + i];
+      const size_t j0 = g[f] >= 0 ? (size_t)(g[f] >> 1) : 0;
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) r0[f][v] = G[j0 * NEQ + v];
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
       const int cls = (cf >> (2 * f)) & 3;
       double ff[NEQ];
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) ff[v] = 0.0;
-      if (g >= 0) {
-        const int jf = g >> 1;
+      if (g[f] >= 0) {
+        const int jf = g[f] >> 1;
         if (cls == CLS_FINER) {
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
+          for (int v = 0; v < NEQ; ++v) ff[v] += (double)r0[f][v];
+#pragma unroll
+          for (int k = 1; k < 4; ++k)
 #pragma unroll
             for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)(jf + k) * NEQ + v];
-        } else if (g & 1) {
+        } else if (g[f] & 1) {
 #pragma unroll
-          for (int v = 0; v < NEQ; ++v) ff[v] += -(double)G[(size_t)jf * NEQ + v];
+          for (int v = 0; v < NEQ; ++v) ff[v] += -(double)r0[f][v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NEQ; ++v) ff[v] += (double)G[(size_t)jf * NEQ + v];
+          for (int v = 0; v < NEQ; ++v) ff[v] += (double)r0[f][v];
         }
       }
 #pragma unroll
