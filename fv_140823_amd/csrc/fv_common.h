@@ -68,8 +68,14 @@ __device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
                                          const double *Uown, double *Un) {
   int cls = (cf >> (2 * f)) & 3;
   int nb = M.nbr[(size_t)f * M.ns + i];
+  /* the single-neighbour row is loaded outside the class branches (the cell's own
+   * row stands in for the other classes), so that the loads of several calls are in
+   * flight together */
+  double Ur[NEQ];
+  load_S<NEQ>(S, M.ns, (cls == CLS_SAME || cls == CLS_COARSER) ? nb : i, Ur);
   if (cls == CLS_SAME || cls == CLS_COARSER) {
-    load_S<NEQ>(S, M.ns, nb, Un);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) Un[v] = Ur[v];
   } else if (cls == CLS_FINER) {
     int q[4];
 #pragma unroll

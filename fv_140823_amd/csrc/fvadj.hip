@@ -186,21 +186,26 @@ __device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ l
                                        double *out) {
   const int g = M.gslot[(size_t)f * M.ns + i];
   const int cls = (cf >> (2 * f)) & 3;
+  /* the first row is loaded outside the branches (row 0 stands in for a face
+   * without a slot), so that the loads of several calls are in flight together */
+  const size_t j0 = g >= 0 ? (size_t)(g >> 1) : 0;
+  const R *src = (cls != CLS_FINER && (g & 1)) ? lUR : lUL;
+  R r0[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) r0[v] = src[j0 * 5 + v];
 #pragma unroll
   for (int v = 0; v < 5; ++v) out[v] = 0.0;
   if (g < 0) return;
-  const int jf = g >> 1;
   if (cls == CLS_FINER) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int v = 0; v < 5; ++v) out[v] += (double)r0[v];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) out[v] += (double)lUL[(size_t)(jf + k) * 5 + v];
-  } else if (g & 1) {
+    for (int k = 1; k < 4; ++k)
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = (double)lUR[(size_t)jf * 5 + v];
+      for (int v = 0; v < 5; ++v) out[v] += (double)lUL[(j0 + k) * 5 + v];
   } else {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = (double)lUL[(size_t)jf * 5 + v];
+    for (int v = 0; v < 5; ++v) out[v] = (double)r0[v];
   }
 }
 
@@ -321,6 +326,27 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParam
   double acc[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) acc[v] = self[(size_t)v * M.ns + i];
+  /* the six neighbours and the bars a single neighbour sent are loaded before any
+   * branch (the cell's own index stands in for the other classes), so that all of
+   * them are in flight together; the sums below keep their order */
+  int nbs[6];
+  double rin[6][5];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int ax = q >> 1, side = q & 1;
+    const int f = side == 0 ? (ax == 0 ? 3 : (ax == 1 ? 0 : 4)) : (ax == 0 ? 1 : (ax == 1 ? 2 : 5));
+    nbs[q] = M.nbr[(size_t)f * M.ns + i];
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int ax = q >> 1, side = q & 1;
+    const int f = side == 0 ? (ax == 0 ? 3 : (ax == 1 ? 0 : 4)) : (ax == 0 ? 1 : (ax == 1 ? 2 : 5));
+    const int cls = (cf >> (2 * f)) & 3;
+    const double *Pin = (side == 0 ? Pp : Pm) + ax * pl;
+    const int src = (cls == CLS_SAME || cls == CLS_COARSER) ? nbs[q] : i;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) rin[q][v] = Pin[(size_t)v * M.ns + src];
+  }
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
@@ -331,13 +357,14 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParam
       const int f = side == 0 ? fm : fp;
       const double *Pin = (side == 0 ? Pp : Pm) + ax * pl;
       const int cls = (cf >> (2 * f)) & 3;
-      const int nb = M.nbr[(size_t)f * M.ns + i];
+      const int nb = nbs[2 * ax + side];
+      const double *r = rin[2 * ax + side];
       if (cls == CLS_SAME) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + nb];
+        for (int v = 0; v < 5; ++v) acc[v] += r[v];
       } else if (cls == CLS_COARSER) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[v] += Pin[(size_t)v * M.ns + nb] / ((double)4);
+        for (int v = 0; v < 5; ++v) acc[v] += r[v] / ((double)4);
       } else if (cls == CLS_FINER) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
