@@ -81,6 +81,18 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
   }
   constexpr int V0 = FULL ? 0 : 1;
   constexpr int V1 = FULL ? NEQ : 4;
+  /* the six neighbours and the row of each single neighbour are loaded before any
+   * branch (the cell's own row stands in for the other classes), so that all of them
+   * are in flight together; the sums below keep their order */
+  int nbs[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) nbs[f] = M.nbr[(size_t)f * M.ns + i];
+  double Ur[6][NEQ];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int cls = (cf >> (2 * f)) & 3;
+    load_S<NEQ>(S, M.ns, (cls == CLS_SAME || cls == CLS_COARSER) ? nbs[f] : i, Ur[f]);
+  }
   /* per axis d, faces of that axis in increasing face id */
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -94,7 +106,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
       const int f = t == 0 ? fa : fb;
       const double nd = (double)c_face_sign[f];
       const int cls = (cf >> (2 * f)) & 3;
-      const int nb = M.nbr[(size_t)f * M.ns + i];
+      const int nb = nbs[f];
       const int nk = cls == CLS_FINER ? 4 : 1;
       const double A = M.area[3 * (cls == CLS_FINER ? L + 1 : L) + d];
       for (int k = 0; k < nk; ++k) {
@@ -108,7 +120,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
 #pragma unroll
             for (int v = 0; v < NEQ; ++v) Un[v] = U[v];
           }
-        } else load_S<NEQ>(S, M.ns, nb, Un);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NEQ; ++v) Un[v] = Ur[f][v];
+        }
 #pragma unroll
         for (int v = V0; v < V1; ++v) acc[v] += 0.5 * (Un[v] + U[v]) * nd * A / V;
         if constexpr (FULL) {
