@@ -166,6 +166,35 @@ __device__ __forceinline__ int axis_chunk(int lb, int b0, int b1, int &blk) {
   return 2;
 }
 
+/* Face blocks of a launch over the three axis groups, group a holding n_a faces of
+ * which the first r_a are regular (ceil(n_a / FV_BLOCK) blocks per group, the grid
+ * their sum).  The blocks holding any irregular face (domain boundary, hanging face,
+ * or a stencil across a level change: the longest chains of dependent loads) are
+ * dispatched first, so that they start early and land round-robin on all XCDs; the
+ * purely regular blocks follow in the XCD-contiguous order of logical_block.  (In
+ * group order the irregular tails of the three groups fell on three of the eight XCDs
+ * and started last.)  Returns the axis; blk is the block's index within its group. */
+__device__ __forceinline__ int face_block(int n0, int n1, int n2, int r0, int r1, int r2, int &blk) {
+  const int b = blockIdx.x;
+  const int rb0 = r0 / FV_BLOCK, rb1 = r1 / FV_BLOCK, rb2 = r2 / FV_BLOCK;
+  const int ib0 = (n0 + FV_BLOCK - 1) / FV_BLOCK - rb0;
+  const int ib1 = (n1 + FV_BLOCK - 1) / FV_BLOCK - rb1;
+  const int ib2 = (n2 + FV_BLOCK - 1) / FV_BLOCK - rb2;
+  const int nib = ib0 + ib1 + ib2;
+  if (b < nib) {
+    if (b < ib0) { blk = rb0 + b; return 0; }
+    if (b < ib0 + ib1) { blk = rb1 + (b - ib0); return 1; }
+    blk = rb2 + (b - ib0 - ib1);
+    return 2;
+  }
+  const int nrb = rb0 + rb1 + rb2, c = b - nib, per = nrb >> 3;
+  const int lb = (per == 0 || (nrb & 7)) ? c : (c & 7) * per + (c >> 3);
+  if (lb < rb0) { blk = lb; return 0; }
+  if (lb < rb0 + rb1) { blk = lb - rb0; return 1; }
+  blk = lb - rb0 - rb1;
+  return 2;
+}
+
 /* ------------------------------------------------------------------ context */
 
 /* MUSCL face state of cell c on face f (faceinterp.c:33-44 for one face).  The

@@ -130,21 +130,19 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
 
 template <typename T, typename R, bool NS>
 __global__ __launch_bounds__(FV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_face(
-    DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
+    DevMesh M, DevParams P, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const double *__restrict__ lam, double cdt,
     const R *__restrict__ SGm, R *__restrict__ lUL, R *__restrict__ lUR, R *__restrict__ lG) {
-  const int lb = logical_block();
-  const int t = threadIdx.x;
-  if (lb < b0) {
-    const int q = lb * FV_BLOCK + t;
+  int blk;
+  const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
+  const int q = blk * FV_BLOCK + threadIdx.x;
+  if (ax == 0) {
     if (q < r0) adj_face_body<0, T, R, true, NS>(M, P, q, S, lam, cdt, SGm, lUL, lUR, lG);
     else if (q < n0) adj_face_body<0, T, R, false, NS>(M, P, q, S, lam, cdt, SGm, lUL, lUR, lG);
-  } else if (lb < b0 + b1) {
-    const int q = (lb - b0) * FV_BLOCK + t;
+  } else if (ax == 1) {
     if (q < r1) adj_face_body<1, T, R, true, NS>(M, P, n0 + q, S, lam, cdt, SGm, lUL, lUR, lG);
     else if (q < n1) adj_face_body<1, T, R, false, NS>(M, P, n0 + q, S, lam, cdt, SGm, lUL, lUR, lG);
   } else {
-    const int q = (lb - b0 - b1) * FV_BLOCK + t;
     if (q < r2)
       adj_face_body<2, T, R, true, NS>(M, P, n0 + n1 + q, S, lam, cdt, SGm, lUL, lUR, lG);
     else if (q < n2)
@@ -692,10 +690,10 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
     const double *SGm = (const double *)c->SGm;
     double *lUL = (double *)a->lamUL, *lUR = (double *)a->lamUR, *lG = (double *)a->lamG;
     if (ns)
-      hipLaunchKernelGGL((k_adj_face<double, double, true>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+      hipLaunchKernelGGL((k_adj_face<double, double, true>), gf, b, 0, c->stream, M, c->dp,
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     else
-      hipLaunchKernelGGL((k_adj_face<double, double, false>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+      hipLaunchKernelGGL((k_adj_face<double, double, false>), gf, b, 0, c->stream, M, c->dp,
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     if (ns)
       hipLaunchKernelGGL(k_adj_sg<double>, dim3(nc), b, 0, c->stream, M, (const double *)lG, a->SGb);
@@ -705,10 +703,10 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
     const float *SGm = (const float *)c->SGm;
     float *lUL = (float *)a->lamUL, *lUR = (float *)a->lamUR, *lG = (float *)a->lamG;
     if (ns)
-      hipLaunchKernelGGL((k_adj_face<float, float, true>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+      hipLaunchKernelGGL((k_adj_face<float, float, true>), gf, b, 0, c->stream, M, c->dp,
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     else
-      hipLaunchKernelGGL((k_adj_face<float, float, false>), gf, b, 0, c->stream, M, c->dp, b0, b1,
+      hipLaunchKernelGGL((k_adj_face<float, float, false>), gf, b, 0, c->stream, M, c->dp,
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     if (ns)
       hipLaunchKernelGGL(k_adj_sg<float>, dim3(nc), b, 0, c->stream, M, (const float *)lG, a->SGb);

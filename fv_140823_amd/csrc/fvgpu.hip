@@ -263,18 +263,17 @@ template <int MODEL, bool NS, typename T, typename R>
  * spill); the viscous variants need 96-98 VGPRs and ask for fewer waves instead
  * of spilling */
 __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k_face_fused(
-    DevMesh M, DevParams P, int b0, int b1, int n0, int n1, int n2, int r0, int r1, int r2,
+    DevMesh M, DevParams P, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
     int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   constexpr bool STAGE = RowStage<NEQ, R>::on;
   __shared__ R gbuf[STAGE ? FV_BLOCK * NEQ : 1];
-  const int lb = logical_block();
   const int t = threadIdx.x;
   /* axis a: faces [0, r_a) of the group are regular (a block straddles the end of
    * that range at most once per axis) */
   int blk;
-  const int ax = axis_chunk(lb, b0, b1, blk);
+  const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
   const int q = blk * FV_BLOCK + t;
   const int jq = ax == 0 ? q : (ax == 1 ? n0 + q : n0 + n1 + q);
   R *dst = STAGE ? gbuf + t * NEQ : G + (size_t)jq * NEQ;
@@ -462,8 +461,7 @@ __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
  * and rebuilt by the block itself (no other block, no extra launch); blocks with
  * no such face skip it after one block-wide vote. */
 template <bool NS>
-__global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, DevParams P, int b0,
-                                                                   int b1, int n0, int n1, int n2,
+__global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, DevParams P, int n0, int n1, int n2,
                                                                    int r0, int r1, int r2,
                                                                    const double *__restrict__ S,
                                                                    const float *__restrict__ Sf,
@@ -472,13 +470,12 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
                                                                    int *__restrict__ err) {
   __shared__ int list[FV_BLOCK];
   __shared__ int nlist;
-  const int lb = logical_block();
   const int t = threadIdx.x;
   if (t == 0) nlist = 0;
   __syncthreads();
   bool done = true;
   int blk;
-  const int ax = axis_chunk(lb, b0, b1, blk);
+  const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
   const int q = blk * FV_BLOCK + t;
   /* FP32 rows (20 B) are stored directly: staging them through LDS as the FP64
    * kernels do measured slower here */
@@ -1275,10 +1272,10 @@ static void launch_face_fused_m(fvg_ctx *c, const double *S) {
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->rsize == 8)
-    hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0,
-                       b1, n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const double *)c->SGm, (double *)c->G, c->err);
+    hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp,
+                       n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const double *)c->SGm, (double *)c->G, c->err);
   else
-    hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1,
+    hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp,
                        n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const float *)c->SGm, (float *)c->G, c->err);
 }
 
@@ -1398,11 +1395,11 @@ static void launch_face_fast(fvg_ctx *c, const double *S) {
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   if (c->prm.ns)
-    hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2,
+    hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
                        (const float *)c->SGm, (float *)c->G, c->err);
   else
-    hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1, n2,
+    hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
                        (const float *)c->SGm, (float *)c->G, c->err);
 }
