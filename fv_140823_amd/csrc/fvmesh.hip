@@ -13,10 +13,11 @@
  *                  the coarser cell (the lookups of the forest's resolve_face, with
  *                  only leaves in the table); class, neighbour, the count of finer
  *                  faces, and the leaf that first sets each per-level table entry
- *   k_rm_count     per cell and axis: its owned faces, regular or not
- *   k_scan_*       exclusive sums of the 7 count columns (quad and face offsets)
- *   k_rm_fill      quads, faces at their final positions (per axis: regular faces
- *                  first, each part in owner order, as the host flattener orders
+ *   k_rm_count     per cell and axis: its owned faces of each kind (regular,
+ *                  domain boundary, hanging, other same-level)
+ *   k_scan_*       exclusive sums of the 13 count columns (quad and face offsets)
+ *   k_rm_fill      quads, faces at their final positions (per axis the kinds in
+ *                  that order, each in owner order, as the host flattener orders
  *                  them), gather slots, regular stencils
  *   k_rm_carry     state of each new cell from the old mesh: same key -> copy; an
  *                  old ancestor -> copy (split.c: kids copy the parent's S); the 8
@@ -38,7 +39,8 @@ constexpr uint64_t KEY_EMPTY = ~0ull;
 constexpr int KEY_BITS = 19; /* global cell index per axis < 2^19 */
 constexpr int MAX_LEVEL = 30;
 constexpr int SCAN_TILE = 2048; /* cells per scan block: 8 per thread */
-constexpr int NCOL = 7;         /* count columns: (reg, other) x 3 axes, finer faces */
+constexpr int NKIND = 4;        /* face kinds in an axis group: regular, boundary, hanging, other */
+constexpr int NCOL = 3 * NKIND + 1; /* count columns: kinds x 3 axes, then finer faces */
 
 enum { RM_DUP = 1, RM_BALANCE = 2, RM_BC = 4, RM_SELF = 8, RM_CARRY = 16 };
 
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(256) void k_rm_classify(int n, int ns, const uint64
     nbr[(size_t)f * ns + i] = nb;
   }
   cflag[i] = cf;
-  cnt[(size_t)6 * n + i] = nfin;
+  cnt[(size_t)(3 * NKIND) * n + i] = nfin;
   if (e) atomicOr(err, e);
   atomicMin(&own[L], i);
   if (L > 1) atomicMin(&own[L - 1], i);
@@ -215,20 +217,16 @@ __global__ __launch_bounds__(256) void k_rm_count(int n, int ns, const uint32_t 
   if (i >= n) return;
   const uint32_t cf = cflag[i];
   for (int a = 0; a < 3; ++a) {
-    int reg = 0, oth = 0;
+    int k[NKIND] = {0, 0, 0, 0};
     for (int t = 0; t < 2; ++t) {
       const int f = c_axf[a][t];
       const int cls = (cf >> (2 * f)) & 3;
       const int nb = nbr[(size_t)f * ns + i];
-      if (cls == CLS_BOUNDARY) oth += 1;
-      else if (cls == CLS_FINER) oth += 4;
-      else if (cls == CLS_SAME && nb > i) {
-        if (regular_face(cflag, cf, f, nb)) ++reg;
-        else ++oth;
-      }
+      if (cls == CLS_BOUNDARY) k[1] += 1;
+      else if (cls == CLS_FINER) k[2] += 4;
+      else if (cls == CLS_SAME && nb > i) k[regular_face(cflag, cf, f, nb) ? 0 : 3] += 1;
     }
-    cnt[(size_t)(2 * a) * n + i] = reg;
-    cnt[(size_t)(2 * a + 1) * n + i] = oth;
+    for (int q = 0; q < NKIND; ++q) cnt[(size_t)(NKIND * a + q) * n + i] = k[q];
   }
 }
 
@@ -311,14 +309,14 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *__restrict__ 
 struct FillOut {
   int32_t *nbr, *quad, *fL, *fR, *fLL, *fRR, *gslot;
   uint32_t *fmeta;
+  int32_t ks[3 * NKIND]; /* first face of each kind of each axis */
 };
 
 __global__ __launch_bounds__(256) void k_rm_fill(int n, int ns, const uint64_t *__restrict__ keys,
                                                  const uint64_t *__restrict__ hk,
                                                  const int32_t *__restrict__ hv, int bits, Geo g,
                                                  const uint32_t *__restrict__ cflag,
-                                                 const int32_t *__restrict__ off, int s0, int s1,
-                                                 int s2, int r0, int r1, int r2, FillOut o) {
+                                                 const int32_t *__restrict__ off, FillOut o) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   int L, c[3];
@@ -326,7 +324,7 @@ __global__ __launch_bounds__(256) void k_rm_fill(int n, int ns, const uint64_t *
   const uint32_t cf = cflag[i];
   /* quads, in face order, from this cell's offset */
   int qf[6];
-  int q = off[(size_t)6 * n + i];
+  int q = off[(size_t)(3 * NKIND) * n + i];
   for (int f = 0; f < 6; ++f) {
     qf[f] = -1;
     if (((cf >> (2 * f)) & 3) != CLS_FINER) continue;
@@ -337,16 +335,15 @@ __global__ __launch_bounds__(256) void k_rm_fill(int n, int ns, const uint64_t *
     for (int k = 0; k < 4; ++k) o.quad[4 * q + k] = hfind(hk, hv, bits, subface_key(L, m, f, k));
     ++q;
   }
-  const int start[3] = {s0, s1, s2}, nreg[3] = {r0, r1, r2};
   for (int a = 0; a < 3; ++a) {
-    int rp = start[a] + off[(size_t)(2 * a) * n + i];
-    int op = start[a] + nreg[a] + off[(size_t)(2 * a + 1) * n + i];
+    int pos[NKIND];
+    for (int k = 0; k < NKIND; ++k) pos[k] = o.ks[NKIND * a + k] + off[(size_t)(NKIND * a + k) * n + i];
     for (int t = 0; t < 2; ++t) {
       const int f = c_axf[a][t], fo = c_fopp[f];
       const int cls = (cf >> (2 * f)) & 3;
       if (cls == CLS_BOUNDARY) {
         const int bc = -1 - o.nbr[(size_t)f * ns + i];
-        const int j = op++;
+        const int j = pos[1]++;
         o.fL[j] = i;
         o.fR[j] = i;
         o.fmeta[j] = (uint32_t)f | ((uint32_t)f << 3) | ((uint32_t)bc << 6) | ((uint32_t)L << 8);
@@ -354,10 +351,10 @@ __global__ __launch_bounds__(256) void k_rm_fill(int n, int ns, const uint64_t *
         o.fRR[j] = -1;
         o.gslot[(size_t)f * ns + i] = j << 1;
       } else if (cls == CLS_FINER) {
-        o.gslot[(size_t)f * ns + i] = op << 1;
+        o.gslot[(size_t)f * ns + i] = pos[2] << 1;
         for (int k = 0; k < 4; ++k) {
           const int r = o.quad[4 * qf[f] + k];
-          const int j = op++;
+          const int j = pos[2]++;
           o.fL[j] = i;
           o.fR[j] = r;
           o.fmeta[j] = (uint32_t)f | ((uint32_t)fo << 3) | ((uint32_t)(L + 1) << 8);
@@ -369,7 +366,7 @@ __global__ __launch_bounds__(256) void k_rm_fill(int n, int ns, const uint64_t *
         const int nb = o.nbr[(size_t)f * ns + i];
         if (nb <= i) continue;
         const bool reg = regular_face(cflag, cf, f, nb);
-        const int j = reg ? rp++ : op++;
+        const int j = reg ? pos[0]++ : pos[3]++;
         o.fL[j] = i;
         o.fR[j] = nb;
         o.fmeta[j] = (uint32_t)f | ((uint32_t)fo << 3) | ((uint32_t)L << 8);
@@ -524,7 +521,7 @@ int fvg_remesh_begin(fvg_ctx *c, int32_t n, const int32_t *level, const int32_t 
     return set_err(c, (e & (RM_BC | RM_SELF)) ? FVG_ERR_UNSUPPORTED : FVG_ERR_STATE, remesh_msg(e));
   }
   int64_t nface = 0;
-  for (int k = 0; k < 6; ++k) nface += host[1 + k];
+  for (int k = 0; k < 3 * NKIND; ++k) nface += host[1 + k];
   if (nface >= (1ll << 30)) {
     fv_remesh_release(c);
     return set_err(c, FVG_ERR_UNSUPPORTED, "remesh: too many faces");
@@ -545,14 +542,18 @@ int fvg_remesh_end(fvg_ctx *c, int32_t nlevel, const double *h, const double *di
   (void)hipSetDevice(c->device);
   hipStream_t st = c->stream;
   const int n = r->n;
-  int32_t nfa[3], nreg[3];
+  int32_t nfa[3], nreg[3], ks[3 * NKIND];
   int F = 0;
   for (int a = 0; a < 3; ++a) {
-    nreg[a] = r->tot[2 * a];
-    nfa[a] = r->tot[2 * a] + r->tot[2 * a + 1];
+    nreg[a] = r->tot[NKIND * a];
+    nfa[a] = 0;
+    for (int k = 0; k < NKIND; ++k) {
+      ks[NKIND * a + k] = F + nfa[a];
+      nfa[a] += r->tot[NKIND * a + k];
+    }
     F += nfa[a];
   }
-  const int Q = r->tot[6];
+  const int Q = r->tot[3 * NKIND];
   /* the old state and keys stay alive for the carry-over */
   double *So = c->S;
   uint64_t *okeys = c->keys, *ohk = c->hkeys;
@@ -567,10 +568,10 @@ int fvg_remesh_end(fvg_ctx *c, int32_t nlevel, const double *h, const double *di
     c->nbr = r->nbr;
     r->cflag = nullptr;
     r->nbr = nullptr;
-    FillOut o{c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta};
+    FillOut o{c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, {}};
+    memcpy(o.ks, ks, sizeof(ks));
     hipLaunchKernelGGL(k_rm_fill, dim3(grid_of((size_t)n, 256)), dim3(256), 0, st, n, c->ns, r->keys,
-                       r->hk, r->hv, r->bits, r->g, c->cflag, r->off, 0, nfa[0], nfa[0] + nfa[1],
-                       nreg[0], nreg[1], nreg[2], o);
+                       r->hk, r->hv, r->bits, r->g, c->cflag, r->off, o);
     if (carry_state) {
       hipLaunchKernelGGL(k_fill_i32v, dim3(1), dim3(64), 0, st, r->info, (size_t)1, 0);
       hipLaunchKernelGGL(k_rm_carry, dim3(grid_of((size_t)n * c->neq, 256)), dim3(256), 0, st, n,

@@ -10,8 +10,9 @@
  * Unique faces, grouped by axis (x, y, z); a same-level face is owned by the
  * lower leaf index, a hanging face by the coarse side (4 consecutive sub-faces
  * k = 0..3 in the order of neigtr[f][k], elconnfin.c:41-45), a boundary face by
- * its leaf.  Within an axis the regular faces come first, then the others, each
- * part in owner-leaf order.  A regular face is an interior same-level face whose
+ * its leaf.  Within an axis the regular faces come first, then the domain boundary
+ * faces, the hanging faces and the other same-level faces, each part in owner-leaf
+ * order.  A regular face is an interior same-level face whose
  * two cells also have same-level neighbours on their far sides along the axis:
  * its MUSCL stencil is four cells of one level (face_LL, face_L, face_R,
  * face_RR), and the kernels evaluate it without the class and neighbour lookups.
@@ -43,6 +44,14 @@ struct fvh_flat {
   double area[FV_MAXLEV * 3];
   double vol[FV_MAXLEV];
 };
+
+/* order of the faces within an axis group: regular, domain boundary, the 4 sub-faces
+ * of a hanging face, then the other same-level faces (a stencil across a level
+ * change); each kind in owner order.  The kernels' waves then mostly hold faces of
+ * one kind and take one branch. */
+static int face_kind(int reg, int bc, int owner_cls) {
+  return reg ? 0 : (bc != 0 ? 1 : (owner_cls == FV_CLS_FINER ? 2 : 3));
+}
 
 static int close_rel(double a, double b) {
   double s = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
@@ -321,8 +330,8 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
     for (int a = 0; a < 3; ++a) {
       const int64_t end = start + nfa[a];
       int64_t nreg = 0;
-      for (int pass = 0; pass < 2; ++pass) {
-        int64_t pos = pass == 0 ? start : start + nreg;
+      int64_t pos = start;
+      for (int pass = 0; pass < 4; ++pass) {
         for (int64_t q = start; q < end; ++q) {
           const int iL = fl->face_L[q], iR = fl->face_R[q];
           const uint32_t mt = fl->face_meta[q];
@@ -332,7 +341,7 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
           const int reg = bc == 0 && ((cL >> (2 * f)) & 3) == FV_CLS_SAME &&
                           ((cL >> (2 * fL2)) & 3) == FV_CLS_SAME &&
                           ((cR >> (2 * fR2)) & 3) == FV_CLS_SAME;
-          if (reg != (pass == 0)) continue;
+          if (face_kind(reg, bc, (cL >> (2 * f)) & 3) != pass) continue;
           perm[q] = (int32_t)pos;
           nL[pos] = iL;
           nR[pos] = iR;
