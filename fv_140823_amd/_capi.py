@@ -167,6 +167,8 @@ _DECLS = [
     ("fvg_face_jvp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     ("fvg_limiter_host", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_double, _dp, _dp, _dp]),
+    ("fvg_limiter_axis_host", C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.POINTER(C.c_double)]),
     ("fvg_set_state_device", C.c_int, [_vp, _vp]),
     ("fvg_get_state_device", C.c_int, [_vp, _vp]),
     ("fv_gpu_attach", C.c_int, [_vp, C.c_int, C.POINTER(FvgParams), C.c_int]),

@@ -63,11 +63,10 @@ __device__ __forceinline__ void load_S(const double *__restrict__ S, int ns, int
 /* neighbour state across face f for MUSCL deltas (facevalues_MUSCL.c:18-49 and
  * the bc ghost of deltavalues.c:12-13) */
 template <int NEQ>
-__device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
-                                         const double *__restrict__ S, int i, uint32_t cf, int f,
-                                         const double *Uown, double *Un) {
+__device__ __forceinline__ void nb_state_at(const DevMesh &M, const DevParams &P,
+                                            const double *__restrict__ S, int i, uint32_t cf,
+                                            int f, int nb, const double *Uown, double *Un) {
   int cls = (cf >> (2 * f)) & 3;
-  int nb = M.nbr[(size_t)f * M.ns + i];
   /* the single-neighbour row is loaded outside the class branches (the cell's own
    * row stands in for the other classes), so that the loads of several calls are in
    * flight together */
@@ -97,6 +96,14 @@ __device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
       for (int v = 0; v < NEQ; ++v) Un[v] = Uown[v];
     }
   }
+}
+
+/* the same with the neighbour index read here */
+template <int NEQ>
+__device__ __forceinline__ void nb_state(const DevMesh &M, const DevParams &P,
+                                         const double *__restrict__ S, int i, uint32_t cf, int f,
+                                         const double *Uown, double *Un) {
+  nb_state_at<NEQ>(M, P, S, i, cf, f, M.nbr[(size_t)f * M.ns + i], Uown, Un);
 }
 
 __device__ __forceinline__ void face_dist(const DevMesh &M, uint32_t cf, int f, double &dcf,

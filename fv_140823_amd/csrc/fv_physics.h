@@ -573,3 +573,26 @@ FV_HD __forceinline__ double limited_d(int lim, double eps, bool bnd, double d0,
   if (!isinf(im)) g1 = -ga * im;
   return N / D;
 }
+
+/* The partials of both face states of one axis of a cell: SF[fp] uses
+ * g(dP, dM; ip, im) -> (g0, g1), SF[fm] uses g(dM, dP; im, ip) -> (h0, h1).  For van
+ * Albada the second slope is the negative of the first (the limiter is symmetric in
+ * its two one-sided slopes and odd, and the second call's slopes are the first's,
+ * swapped and negated: a' = -b, b' = -a), so h0 = -g1 and h1 = -g0; these are the
+ * values the second call computes, bit for bit, from one reciprocal instead of two.
+ * Minmod keeps two calls: its 1e-10 gate looks at the first slope only. */
+FV_HD __forceinline__ void limited_d_axis(int lim, double eps, bool bp, bool bm, double dP,
+                                          double dM, double ip, double im, double &g0,
+                                          double &g1, double &h0, double &h1) {
+  if (lim == 0) {
+    limited_d(lim, eps, bp, dP, dM, ip, im, g0, g1);
+    limited_d(lim, eps, bm, dM, dP, im, ip, h0, h1);
+    return;
+  }
+  double c0, c1;
+  limited_d(lim, eps, false, dP, dM, ip, im, c0, c1);
+  g0 = bp ? 0.0 : c0;
+  g1 = bp ? 0.0 : c1;
+  h0 = (bm || isinf(im)) ? 0.0 : -c1;
+  h1 = (bm || isinf(ip)) ? 0.0 : -c0;
+}

@@ -179,10 +179,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_sg(DevMesh M, const R *__restr
 
 /* bar of the face state SF[f] of cell i, gathered from the face(s) of slot f */
 template <typename R>
-__device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ lUL,
-                                       const R *__restrict__ lUR, int i, uint32_t cf, int f,
-                                       double *out) {
-  const int g = M.gslot[(size_t)f * M.ns + i];
+__device__ __forceinline__ void sf_bar_at(const R *__restrict__ lUL, const R *__restrict__ lUR,
+                                          uint32_t cf, int f, int g, double *out) {
   const int cls = (cf >> (2 * f)) & 3;
   /* the first row is loaded outside the branches (row 0 stands in for a face
    * without a slot), so that the loads of several calls are in flight together */
@@ -205,6 +203,14 @@ __device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ l
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = (double)r0[v];
   }
+}
+
+/* the same with the slot read here */
+template <typename R>
+__device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ lUL,
+                                       const R *__restrict__ lUR, int i, uint32_t cf, int f,
+                                       double *out) {
+  sf_bar_at<R>(lUL, lUR, cf, f, M.gslot[(size_t)f * M.ns + i], out);
 }
 
 /* recon^T, part 1: own bar and the bars sent to the +/- neighbour of each axis */
@@ -242,30 +248,39 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
   }
   double U[5];
   load_S<5>(S, M.ns, i, U);
+  /* the neighbours and slots of all six faces are read before the axis loop, so that
+   * each axis waits for one round of loads (its states and bars), not two */
+  const int n0 = M.nbr[i], n1 = M.nbr[(size_t)M.ns + i], n2 = M.nbr[(size_t)2 * M.ns + i];
+  const int n3 = M.nbr[(size_t)3 * M.ns + i], n4 = M.nbr[(size_t)4 * M.ns + i];
+  const int n5 = M.nbr[(size_t)5 * M.ns + i];
+  const int s0 = M.gslot[i], s1 = M.gslot[(size_t)M.ns + i], s2 = M.gslot[(size_t)2 * M.ns + i];
+  const int s3 = M.gslot[(size_t)3 * M.ns + i], s4 = M.gslot[(size_t)4 * M.ns + i];
+  const int s5 = M.gslot[(size_t)5 * M.ns + i];
   /* one axis at a time (not unrolled): the three axes' 20 live values each would
    * otherwise be interleaved and cost the kernel most of its occupancy */
 #pragma unroll 1
   for (int ax = 0; ax < 3; ++ax) {
     const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
     const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
+    const int np = ax == 0 ? n1 : (ax == 1 ? n2 : n5), nm = ax == 0 ? n3 : (ax == 1 ? n0 : n4);
+    const int gp = ax == 0 ? s1 : (ax == 1 ? s2 : s5), gm = ax == 0 ? s3 : (ax == 1 ? s0 : s4);
     double Up[5], Um[5], lp[5], lm[5];
-    nb_state<5>(M, P, S, i, cf, fp, U, Up);
-    nb_state<5>(M, P, S, i, cf, fm, U, Um);
-    sf_bar<R>(M, lUL, lUR, i, cf, fp, lp);
-    sf_bar<R>(M, lUL, lUR, i, cf, fm, lm);
+    nb_state_at<5>(M, P, S, i, cf, fp, np, U, Up);
+    nb_state_at<5>(M, P, S, i, cf, fm, nm, U, Um);
+    sf_bar_at<R>(lUL, lUR, cf, fp, gp, lp);
+    sf_bar_at<R>(lUL, lUR, cf, fm, gm, lm);
     double pcf, pcc, mcf, mcc, pic, mic;
     face_dist(M, cf, fp, pcf, pcc, pic);
     face_dist(M, cf, fm, mcf, mcc, mic);
     const int cp = (cf >> (2 * fp)) & 3, cm = (cf >> (2 * fm)) & 3;
     const bool bp = cp == CLS_BOUNDARY, bm = cm == CLS_BOUNDARY;
-    const bool ghost_p = bp && (-1 - M.nbr[(size_t)fp * M.ns + i]) == 2;
-    const bool ghost_m = bm && (-1 - M.nbr[(size_t)fm * M.ns + i]) == 2;
+    const bool ghost_p = bp && (-1 - np) == 2;
+    const bool ghost_m = bm && (-1 - nm) == 2;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const double dP = Up[v] - U[v], dM = Um[v] - U[v];
       double g0, g1, h0, h1;
-      limited_d(P.limiter, P.va_eps, bp, dP, dM, pic, mic, g0, g1);
-      limited_d(P.limiter, P.va_eps, bm, dM, dP, mic, pic, h0, h1);
+      limited_d_axis(P.limiter, P.va_eps, bp, bm, dP, dM, pic, mic, g0, g1, h0, h1);
       /* SF[fp] = U + pcf g(dP, dM), SF[fm] = U + mcf g(dM, dP) */
       double dPb = lp[v] * pcf * g0 + lm[v] * mcf * h1;
       double dMb = lp[v] * pcf * g1 + lm[v] * mcf * h0;
@@ -549,8 +564,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_tlm_recon(DevMesh M, DevParams P,
       const double dP = Up[v] - U[v], dM = Um[v] - U[v];
       const double dPd = Dp[v] - D[v], dMd = Dm[v] - D[v];
       double g0, g1, h0, h1;
-      limited_d(P.limiter, P.va_eps, bp, dP, dM, pic, mic, g0, g1);
-      limited_d(P.limiter, P.va_eps, bm, dM, dP, mic, pic, h0, h1);
+      limited_d_axis(P.limiter, P.va_eps, bp, bm, dP, dM, pic, mic, g0, g1, h0, h1);
       SFd[((size_t)fp * 5 + v) * M.ns + i] = D[v] + pcf * (g0 * dPd + g1 * dMd);
       SFd[((size_t)fm * 5 + v) * M.ns + i] = D[v] + mcf * (h0 * dMd + h1 * dPd);
     }
@@ -1062,6 +1076,13 @@ int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc
                      double *g, double *g0, double *g1) {
   if (!g || !g0 || !g1) return FVG_ERR_ARG;
   *g = limited_d(limiter, eps, false, d0, d1, dp1cc, dm1cc, *g0, *g1);
+  return FVG_OK;
+}
+
+int fvg_limiter_axis_host(int limiter, double eps, int bp, int bm, double dP, double dM,
+                          double ip, double im, double *out) {
+  if (!out) return FVG_ERR_ARG;
+  limited_d_axis(limiter, eps, bp != 0, bm != 0, dP, dM, ip, im, out[0], out[1], out[2], out[3]);
   return FVG_OK;
 }
 

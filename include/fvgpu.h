@@ -293,6 +293,12 @@ int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL,
                       const double *ULd, const double *URd, double *outd);
 int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc, double dm1cc,
                      double *g, double *g0, double *g1);
+/* Both face states' partials of one axis (the adjoint's reconstruction transpose):
+ * out = (g0, g1) of SF[fp] = g(dP, dM; ip, im) and (h0, h1) of SF[fm] = g(dM, dP; im, ip),
+ * zero on a boundary face (bp, bm).  Host check of the shared van Albada evaluation; no
+ * counterpart in the reference. */
+int fvg_limiter_axis_host(int limiter, double eps, int bp, int bm, double dP, double dM,
+                          double ip, double im, double *out);
 /* device-to-device state copies on the context's stream (torch interop) */
 int fvg_set_state_device(fvg_ctx *ctx, const void *dS /* device [neq][ncell] f64 */);
 int fvg_get_state_device(fvg_ctx *ctx, void *dS);

@@ -188,3 +188,36 @@ def test_ns_face_vjp_matches_finite_differences(regime, axis, sign, work):
             fd = ob @ (flux_ns(p, axis, sign, *Xp) - flux_ns(p, axis, sign, *Xm)) / (2 * h)
             g = bars[side][m]
             assert abs(g - fd) <= 1e-6 * max(1.0, abs(fd)), (side, m, g, fd)
+
+
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_limiter_axis_equals_two_calls(limiter):
+    """The adjoint evaluates both face states' partials of an axis at once (for van
+    Albada from one reciprocal: the second slope is the negative of the first); this is
+    bitwise the two separate evaluations, boundary faces, inf reciprocals (boundary
+    distances) and the minmod gate included."""
+    rng = np.random.default_rng(11)
+    cases = []
+    for _ in range(400):
+        dP, dM = rng.standard_normal(2) * 10.0 ** rng.integers(-12, 1, 2)
+        ip, im = rng.uniform(2.0, 10.0, 2)
+        cases.append((dP, dM, ip, im))
+    cases += [(0.0, 0.3, 4.0, 4.0), (0.2, 0.0, 4.0, 4.0), (0.0, 0.0, 4.0, 4.0), (1e-11, 0.2, 4.0, 4.0),
+              (0.2, -1e-11, 4.0, 4.0), (0.1, 0.2, np.inf, 4.0), (0.1, 0.2, 4.0, np.inf),
+              (-0.1, 0.2, np.inf, np.inf), (0.3, -0.3, 4.0, 4.0)]
+    out = (C.c_double * 4)()
+
+    def one(bnd, d0, d1, p, m):
+        if bnd:
+            return 0.0, 0.0
+        g, g0, g1 = C.c_double(), C.c_double(), C.c_double()
+        L.fvg_limiter_host(limiter, 1e-12, d0, d1, p, m, C.byref(g), C.byref(g0), C.byref(g1))
+        return g0.value, g1.value
+
+    for dP, dM, ip, im in cases:
+        for bp in (0, 1):
+            for bm in (0, 1):
+                assert L.fvg_limiter_axis_host(limiter, 1e-12, bp, bm, dP, dM, ip, im, out) == 0
+                want = one(bp, dP, dM, ip, im) + one(bm, dM, dP, im, ip)
+                got = tuple(out)
+                assert np.array_equal(np.array(got), np.array(want)), (limiter, dP, dM, ip, im, bp, bm, got, want)

@@ -87,9 +87,9 @@ def test_blast_conserves_mass_and_energy_through_adaptation():
     xc, _ = F.leaf_info()
     print("leaves", nleaves, "remeshes", len(st["remesh"]), flush=True)
     assert len(st["remesh"]) == 19 and nleaves > 100000
-    d = _drift(t0, t1, np.abs(t0))
-    print("drift", d, flush=True)
-    assert d[0] < 1e-13 and d[4] < 1e-13 and d[5] < 1e-13, d
+    d = _drift(t0[[0, 4, 5]], t1[[0, 4, 5]], np.abs(t0[[0, 4, 5]]))
+    print("drift of mass, energy, volume", d, flush=True)
+    assert np.all(d < 1e-13), d
     # the blast has stayed inside: the cells near the walls still hold the ambient
     # state to round-off, so no mass or energy has left through the walls
     edge = np.any((xc < 0.05) | (xc > 0.95), axis=1)
