@@ -166,7 +166,8 @@ def time_form(F, p, steps, warmup, dt):
     g.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-kernel durations from HIP events recorded on the context's stream
+    # per-kernel durations from HIP events on the context's stream, around batches
+    # of back-to-back launches of each stage kernel (fvg_time_steps)
     ev_total, kms = g.time_steps(max(3, steps // 3), dt, 3)
     g.close()
     return wall, kms

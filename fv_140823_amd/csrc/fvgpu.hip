@@ -1667,31 +1667,54 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
     c->ev_ok = true;
   }
-  double sum[FV_NKT] = {0};
-  int cnt[FV_NKT] = {0};
   reset_err(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipEvent_t &t0 = c->ev[2 * 64], &t1 = c->ev[2 * 64 + 1];
   HIPCHK(c, hipEventRecord(t0, c->stream));
-  for (int s = 0; s < nsteps; ++s) {
-    int nev = 0, kid[64];
-    time_step(c, method, dt, false, c->ev, &nev, kid);
-    HIPCHK(c, hipEventSynchronize(c->ev[nev - 1]));
-    for (int q = 0; q + 1 < nev; ++q) {
-      if (kid[q] < 0) continue;
-      float ms = 0.f;
-      HIPCHK(c, hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
-      sum[kid[q]] += ms;
-      cnt[kid[q]] += 1;
-    }
-  }
+  for (int s = 0; s < nsteps; ++s) time_step(c, method, dt, false, nullptr, nullptr, nullptr);
   HIPCHK(c, hipEventRecord(t1, c->stream));
   HIPCHK(c, hipEventSynchronize(t1));
   float tot = 0.f;
   HIPCHK(c, hipEventElapsedTime(&tot, t0, t1));
   if (ms_total) *ms_total = tot;
-  if (ms_kernel)
-    for (int k = 0; k < FV_NKT; ++k) ms_kernel[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  if (ms_kernel) {
+    /* Mean duration of each kernel of a stage, from batches of kReps back-to-back
+     * launches between two events (an event pair around every launch would add the
+     * launch latency to each).  Every launch repeats the same work: the gradient and
+     * face kernels rewrite SGm and G from the current S, the gather writes the spare
+     * buffer (its FP32 copy, in the mixed form, into the unused SF array).  The
+     * gather's mean weighs the first stage (SN = S, read once) 1 : 2 against the
+     * others, as in a step of the three-stage method. */
+    constexpr int kReps = 10;
+    const bool fast = fast_mixed(c);
+    if (fast) ensure_sf(c);
+    bool ev_fail = false;
+    auto batch = [&](auto &&launch) -> double {
+      float ms = 0.f;
+      ev_fail |= hipEventRecord(c->ev[0], c->stream) != hipSuccess;
+      for (int r = 0; r < kReps; ++r) launch();
+      ev_fail |= hipEventRecord(c->ev[1], c->stream) != hipSuccess;
+      ev_fail |= hipEventSynchronize(c->ev[1]) != hipSuccess;
+      ev_fail |= hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) != hipSuccess;
+      return (double)ms / kReps;
+    };
+    for (int k = 0; k < FV_NKT; ++k) ms_kernel[k] = 0.0;
+    float *sf_dummy = fast ? (float *)c->SF : nullptr;
+    if (c->prm.ns) ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
+    ms_kernel[2] = batch([&] {
+      if (fast) launch_face_fast(c, c->S);
+      else if (c->prm.ischeme == 2) dispatch_face_fused(c, c->S);
+      else dispatch_face(c, c->S);
+    });
+    const double g1 = batch([&] {
+      launch_gather(c, StagePtrs{c->S, c->S, c->Sx, sf_dummy}, false, true, false, 1, dt);
+    });
+    const double g2 = batch([&] {
+      launch_gather(c, StagePtrs{c->S, c->SN, c->Sx, sf_dummy}, false, true, false, 2, dt);
+    });
+    ms_kernel[3] = (g1 + 2.0 * g2) / 3.0;
+    if (ev_fail) return set_err(c, FVG_ERR_HIP, "kernel timing events failed");
+  }
   int32_t bad;
   int rc = read_err(c, &bad);
   if (rc) return rc;
