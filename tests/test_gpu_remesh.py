@@ -45,6 +45,7 @@ def assert_same_mesh(g, F):
 
 
 def _forests():
+    yield "single leaf", uniform(0, 5)
     yield "skewed", skewed_amr()
     yield "uniform", uniform(3, 5)
     yield "amr periodic", amr(3, 5, periodic=(1, 0, 1))
@@ -71,18 +72,19 @@ def _flags(F, k, max_level):
     return f
 
 
-@pytest.mark.parametrize("model,precision,periodic", [
-    (1, "plain", (0, 0, 0)), (1, "mixed", (0, 0, 0)), (1, "plain", (1, 1, 0)), (1, "mixed", (1, 1, 0)),
-    (2, "plain", (0, 0, 0)), (2, "mixed", (1, 0, 0))])
-def test_carried_state_equals_host_round_trip(model, precision, periodic):
-    print(f"case carry model={model} {precision} periodic={periodic}", flush=True)
+@pytest.mark.parametrize("model,precision,periodic,ns", [
+    (1, "plain", (0, 0, 0), 0), (1, "mixed", (0, 0, 0), 0), (1, "plain", (1, 1, 0), 0),
+    (1, "mixed", (1, 1, 0), 0), (1, "plain", (1, 1, 1), 1), (2, "plain", (0, 0, 0), 0),
+    (2, "mixed", (1, 0, 0), 0)])
+def test_carried_state_equals_host_round_trip(model, precision, periodic, ns):
+    print(f"case carry model={model} {precision} periodic={periodic} ns={ns}", flush=True)
     neq = fv.neq_of(model)
     F1, F2 = amr(3, neq, periodic=periodic), amr(3, neq, periodic=periodic)
     xc, _ = F1.leaf_info()
     S0 = state(model, xc, seed=5)
     F1.set_state(S0)
     F2.set_state(S0)
-    P = (fv.params(1, precision=precision) if model == 1 else
+    P = (fv.params(1, precision=precision, ns=ns, mu=0.01 if ns else 0.0) if model == 1 else
          configs.barotropic_cavitation_params(ischeme=2, precision=precision))
     g1 = fv.Solver.from_forest(F1, P, device_flatten=True)
     g2 = fv.Solver.from_forest(F2, P)
