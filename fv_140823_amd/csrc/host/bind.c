@@ -110,6 +110,10 @@ static int push(fv_binding *b, int which) {
 int fv_gpu_attach(RUN *run, int device, const fvg_params *p, int compat) {
   if (!run || !p) return FVG_ERR_ARG;
   if (find(run)) return FVG_ERR_STATE;
+  /* MUSCL on primitive variables: the reference's cons2primtv divides by a density
+   * that facevalues_MUSCL never sets (cons2primtv.c:10-13), so there is no defined
+   * result to reproduce */
+  if (run->con && run->con->iprimtv != 0) return FVG_ERR_UNSUPPORTED;
   fv_binding *b = NULL;
   for (int i = 0; i < FV_MAX_BIND; ++i)
     if (!g_bind[i].run) { b = &g_bind[i]; break; }

@@ -193,3 +193,31 @@ def test_unsupported_boundary_rejected():
     F = uniform(2, 5, bc=(3, 2, 2, 2, 2, 2))
     with pytest.raises(fv.FvError):
         fv.FlatMesh(F.run)
+
+
+class _CON(C.Structure):
+    """the leading fields of CON (fv_strdata.h), up to iprimtv"""
+    _fields_ = [("level", C.c_int), ("ns", C.c_int), ("turb", C.c_int), ("i_case", C.c_int),
+                ("nprimitiv", C.c_int), ("ymin", C.c_double), ("model", C.c_int), ("multimat", C.c_int),
+                ("nstep", C.c_int), ("dt", C.c_double), ("eos", C.c_int), ("verbose", C.c_int),
+                ("halfdt", C.c_int), ("limiter", C.c_int), ("istep", C.c_int), ("tstep", C.c_int),
+                ("time", C.c_double), ("rank", C.c_int), ("size", C.c_int), ("neq", C.c_int),
+                ("neq_temp", C.c_int), ("ischeme", C.c_int), ("iprimtv", C.c_int), ("method", C.c_int)]
+
+
+class _RUN(C.Structure):
+    _fields_ = [("con", C.POINTER(_CON)), ("par", C.c_void_p), ("topo", C.c_void_p)]
+
+
+def test_primitive_reconstruction_refused():
+    """iprimtv = 1 is refused before any device call: the reference's cons2primtv reads
+    a density its loader never set, so there is no defined result to match."""
+    F = uniform(2, 5)
+    run = C.cast(F.run, C.POINTER(_RUN)).contents
+    assert run.con.contents.neq == 5 and run.con.contents.iprimtv == 0
+    run.con.contents.iprimtv = 1
+    try:
+        rc = fv.lib().fv_gpu_attach(F.run, 0, C.byref(fv.params(1)), 1)
+        assert rc == _capi.FVG_ERR_UNSUPPORTED
+    finally:
+        run.con.contents.iprimtv = 0
