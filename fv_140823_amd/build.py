@@ -35,7 +35,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-CC_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wno-unused-function"]
+CC_FLAGS = ["-O2", "-fPIC", "-std=c11", "-pthread", "-Wall", "-Wno-unused-function"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
@@ -82,7 +82,7 @@ def build_library(verbose: bool = False) -> pathlib.Path:
     if failed:
         raise RuntimeError("compilation failed")
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -14,14 +14,71 @@
  *     arithmetic mean of the kids (merge.c:92-107), with 2:1 face balance.
  * Periodic pairing of opposite sides is an extension (the reference has none).
  */
+#define _POSIX_C_SOURCE 200809L
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "fvgpu.h"
 #include "fv_tables.h"
+
+/* ------------------------------------------------------------------ threads */
+
+/* [0, n) split into equal chunks over up to FVH_THREADS host threads (the caller takes
+ * the first); fn(ctx, lo, hi) may touch only what belongs to its range.  The result
+ * never depends on the split.  Plain pthreads: the process may already hold another
+ * OpenMP runtime (torch's). */
+#define FVH_THREADS 16
+typedef struct {
+  void (*fn)(void *, int, int);
+  void *ctx;
+  int lo, hi;
+} par_task;
+
+static void *par_run(void *a) {
+  par_task *t = (par_task *)a;
+  t->fn(t->ctx, t->lo, t->hi);
+  return NULL;
+}
+
+static int host_threads(void) {
+  static int n = 0;
+  if (!n) {
+    long c = sysconf(_SC_NPROCESSORS_ONLN);
+    n = c < 1 ? 1 : (c > FVH_THREADS ? FVH_THREADS : (int)c);
+  }
+  return n;
+}
+
+/* grain: the fewest items worth a thread of its own */
+static void par_for(int n, int grain, void (*fn)(void *, int, int), void *ctx) {
+  int T = host_threads();
+  if (grain < 1) grain = 1;
+  if (T > n / grain) T = n / grain;
+  if (T <= 1) {
+    if (n > 0) fn(ctx, 0, n);
+    return;
+  }
+  pthread_t th[FVH_THREADS];
+  par_task task[FVH_THREADS];
+  int started[FVH_THREADS] = {0};
+  for (int k = 0; k < T; ++k) {
+    task[k].fn = fn;
+    task[k].ctx = ctx;
+    task[k].lo = (int)((int64_t)n * k / T);
+    task[k].hi = (int)((int64_t)n * (k + 1) / T);
+  }
+  for (int k = 1; k < T; ++k) started[k] = pthread_create(&th[k], NULL, par_run, &task[k]) == 0;
+  fn(ctx, task[0].lo, task[0].hi);
+  for (int k = 1; k < T; ++k) {
+    if (started[k]) pthread_join(th[k], NULL);
+    else fn(ctx, task[k].lo, task[k].hi);
+  }
+}
 
 struct fvh_forest {
   RUN run;
@@ -374,16 +431,25 @@ static int merge_branch(struct fvh_forest *fr, BRANCH *b) {
   return 0;
 }
 
-static void collect_leaves(BRANCH *b, BRANCH ***out, int *n, int *cap) {
+/* appends the leaves of b's subtree in depth-first order; FVG_ERR_NOMEM leaves *out
+ * valid (and owned by the caller) with what was collected */
+static int collect_leaves(BRANCH *b, BRANCH ***out, int *n, int *cap) {
   if (b->nkids == 0) {
     if (*n == *cap) {
-      *cap = *cap ? 2 * *cap : 1024;
-      *out = (BRANCH **)realloc(*out, (size_t)*cap * sizeof(BRANCH *));
+      int c = *cap ? 2 * *cap : 1024;
+      BRANCH **q = (BRANCH **)realloc(*out, (size_t)c * sizeof(BRANCH *));
+      if (!q) return FVG_ERR_NOMEM;
+      *out = q;
+      *cap = c;
     }
     (*out)[(*n)++] = b;
-    return;
+    return 0;
   }
-  for (int c = 0; c < 8; ++c) collect_leaves(b->kids[c], out, n, cap);
+  for (int c = 0; c < 8; ++c) {
+    int rc = collect_leaves(b->kids[c], out, n, cap);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 /* neighbour across face f of leaf b: fills neigtr/lfc_neigs/neigfc/nsfc and fc[f].bc */
@@ -451,22 +517,97 @@ static int bvec_push(bvec *v, BRANCH *b) {
   return 0;
 }
 
-/* leaf list in depth-first order with list links and indices */
-static void relist(struct fvh_forest *fr) {
-  int n = 0, cap = fr->nleaves > 0 ? fr->nleaves : 0;
-  BRANCH **lv = fr->leaves;
-  for (int r = 0; r < fr->nroots; ++r) collect_leaves(fr->roots[r], &lv, &n, &cap);
-  fr->leaves = lv;
-  fr->nleaves = n;
-  for (int i = 0; i < n; ++i) {
-    BRANCH *b = lv[i];
+/* leaf list in depth-first order with list links and indices.  The subtrees of a
+ * frontier, widened in depth-first order until there is work for the threads, are
+ * collected in parallel and concatenated in order. */
+typedef struct {
+  BRANCH **p;
+  int n, cap, rc;
+} lvec;
+
+typedef struct {
+  BRANCH *const *front;
+  lvec *out;
+  BRANCH **lv;
+  BRANCH *glob;
+  int n;
+} relist_ctx;
+
+static void relist_collect(void *v, int lo, int hi) {
+  relist_ctx *r = (relist_ctx *)v;
+  for (int t = lo; t < hi; ++t)
+    r->out[t].rc = collect_leaves(r->front[t], &r->out[t].p, &r->out[t].n, &r->out[t].cap);
+}
+
+static void relist_link(void *v, int lo, int hi) {
+  relist_ctx *r = (relist_ctx *)v;
+  for (int i = lo; i < hi; ++i) {
+    BRANCH *b = r->lv[i];
     b->fv_index = i;
-    b->lprv = i ? lv[i - 1] : fr->glob;
-    b->lnxt = (i + 1 < n) ? lv[i + 1] : NULL;
+    b->lprv = i ? r->lv[i - 1] : r->glob;
+    b->lnxt = (i + 1 < r->n) ? r->lv[i + 1] : NULL;
   }
-  fr->topo.locl = n ? lv[0] : NULL;
-  fr->topo.nleaves = n;
-  fr->topo.tleaves = n;
+}
+
+static int relist(struct fvh_forest *fr) {
+  bvec a = {0}, b = {0};
+  int rc = 0;
+  for (int r = 0; r < fr->nroots && !rc; ++r) rc = bvec_push(&a, fr->roots[r]);
+  while (!rc && a.n < 256) {
+    int any = 0;
+    b.n = 0;
+    for (size_t q = 0; q < a.n && !rc; ++q) {
+      BRANCH *x = a.p[q];
+      if (x->nkids) {
+        any = 1;
+        for (int c = 0; c < 8 && !rc; ++c) rc = bvec_push(&b, x->kids[c]);
+      } else {
+        rc = bvec_push(&b, x);
+      }
+    }
+    bvec t = a;
+    a = b;
+    b = t;
+    if (!any) break;
+  }
+  free(b.p);
+  const int nf = (int)a.n;
+  lvec *out = rc ? NULL : (lvec *)calloc((size_t)(nf ? nf : 1), sizeof(lvec));
+  if (!out) {
+    free(a.p);
+    return FVG_ERR_NOMEM;
+  }
+  relist_ctx ctx = {a.p, out, NULL, fr->glob, 0};
+  par_for(nf, 4, relist_collect, &ctx);
+  int64_t n = 0;
+  for (int t = 0; t < nf; ++t) {
+    n += out[t].n;
+    if (out[t].rc) rc = out[t].rc;
+  }
+  if (!rc && n > INT32_MAX) rc = FVG_ERR_ARG;
+  BRANCH **lv = rc ? NULL : (BRANCH **)realloc(fr->leaves, (size_t)(n ? n : 1) * sizeof(BRANCH *));
+  if (lv) {
+    fr->leaves = lv;
+    int64_t o = 0;
+    for (int t = 0; t < nf; ++t) {
+      memcpy(lv + o, out[t].p, (size_t)out[t].n * sizeof(BRANCH *));
+      o += out[t].n;
+    }
+  } else if (!rc) {
+    rc = FVG_ERR_NOMEM;
+  }
+  for (int t = 0; t < nf; ++t) free(out[t].p);
+  free(out);
+  free(a.p);
+  if (rc) return rc;
+  fr->nleaves = (int)n;
+  ctx.lv = fr->leaves;
+  ctx.n = (int)n;
+  par_for(n, 4096, relist_link, &ctx);
+  fr->topo.locl = n ? fr->leaves[0] : NULL;
+  fr->topo.nleaves = (int)n;
+  fr->topo.tleaves = (int)n;
+  return 0;
 }
 
 /* neighbours across all six faces, normals, areas, volume and centre of leaf b */
@@ -481,13 +622,29 @@ static int refresh_leaf(struct fvh_forest *fr, BRANCH *b) {
 }
 
 /* every leaf refreshed: after the forest is created */
-static int rebuild(struct fvh_forest *fr) {
-  relist(fr);
-  for (int i = 0; i < fr->nleaves; ++i) {
-    int rc = refresh_leaf(fr, fr->leaves[i]);
-    if (rc) return rc;
+/* refresh_leaf over a list of leaves, in parallel: each call reads the hash and its
+ * neighbours' nodes, which do not change here, and writes only its own leaf */
+typedef struct {
+  struct fvh_forest *fr;
+  BRANCH *const *lv;
+  const int *idx; /* null: all of lv */
+  int rc;
+} refresh_ctx;
+
+static void refresh_part(void *v, int lo, int hi) {
+  refresh_ctx *r = (refresh_ctx *)v;
+  for (int q = lo; q < hi; ++q) {
+    int rc = refresh_leaf(r->fr, r->lv[r->idx ? r->idx[q] : q]);
+    if (rc) __atomic_store_n(&r->rc, rc, __ATOMIC_RELAXED);
   }
-  return 0;
+}
+
+static int rebuild(struct fvh_forest *fr) {
+  int rc = relist(fr);
+  if (rc) return rc;
+  refresh_ctx ctx = {fr, fr->leaves, NULL, 0};
+  par_for(fr->nleaves, 2048, refresh_part, &ctx);
+  return ctx.rc;
 }
 
 /* position of the same-level cell across face f of node b, wrapped on periodic
@@ -523,7 +680,8 @@ static void mark_face(const BRANCH *b, int f, uint8_t *mark) {
  * face, its kids or the coarser cell), so only those are refreshed; the nodes of
  * every other leaf and of its neighbours are as before. */
 static int rebuild_changed(struct fvh_forest *fr, const bvec *chg) {
-  relist(fr);
+  int rc = relist(fr);
+  if (rc) return rc;
   uint8_t *mark = (uint8_t *)calloc((size_t)fr->nleaves + 1, 1);
   if (!mark) return FVG_ERR_NOMEM;
   for (size_t q = 0; q < chg->n; ++q) {
@@ -542,11 +700,18 @@ static int rebuild_changed(struct fvh_forest *fr, const bvec *chg) {
       }
     }
   }
-  int rc = 0;
-  for (int i = 0; i < fr->nleaves && !rc; ++i)
-    if (mark[i]) rc = refresh_leaf(fr, fr->leaves[i]);
+  int nm = 0;
+  for (int i = 0; i < fr->nleaves; ++i) nm += mark[i];
+  int *idx = (int *)malloc((size_t)(nm ? nm : 1) * sizeof(int));
+  if (!idx) { free(mark); return FVG_ERR_NOMEM; }
+  nm = 0;
+  for (int i = 0; i < fr->nleaves; ++i)
+    if (mark[i]) idx[nm++] = i;
   free(mark);
-  return rc;
+  refresh_ctx ctx = {fr, fr->leaves, idx, 0};
+  par_for(nm, 2048, refresh_part, &ctx);
+  free(idx);
+  return ctx.rc;
 }
 
 /* ------------------------------------------------------------------ public */
@@ -601,8 +766,9 @@ int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3]
   for (int l = 0; l < level; ++l) {
     int n = 0, cap = 0;
     BRANCH **lv = NULL;
-    for (int q = 0; q < fr->nroots; ++q) collect_leaves(fr->roots[q], &lv, &n, &cap);
-    int rc = ht_reserve(fr, fr->hcount + (size_t)8 * n);
+    int rc = 0;
+    for (int q = 0; q < fr->nroots && !rc; ++q) rc = collect_leaves(fr->roots[q], &lv, &n, &cap);
+    if (!rc) rc = ht_reserve(fr, fr->hcount + (size_t)8 * n);
     for (int i = 0; i < n && !rc; ++i) rc = split_branch(fr, lv[i]);
     free(lv);
     if (rc) { fvh_forest_destroy(fr); return rc; }
@@ -650,6 +816,22 @@ int fvh_forest_max_level(fvh_forest *fr) {
   return m;
 }
 
+typedef struct {
+  BRANCH *const *lv;
+  int32_t *lev, *ijk;
+} keys_ctx;
+
+static void keys_part(void *v, int lo, int hi) {
+  keys_ctx *k = (keys_ctx *)v;
+  for (int i = lo; i < hi; ++i) {
+    const BRANCH *b = k->lv[i];
+    k->lev[i] = b->level;
+    k->ijk[3 * i] = b->fv_ijk[0];
+    k->ijk[3 * i + 1] = b->fv_ijk[1];
+    k->ijk[3 * i + 2] = b->fv_ijk[2];
+  }
+}
+
 BRANCH *const *fvh_forest_leaves(fvh_forest *fr) { return fr ? (BRANCH *const *)fr->leaves : NULL; }
 
 int fvh_forest_remesh_device(fvh_forest *fr, fvg_ctx *ctx, int carry_state) {
@@ -660,13 +842,8 @@ int fvh_forest_remesh_device(fvh_forest *fr, fvg_ctx *ctx, int carry_state) {
   int32_t *ijk = (int32_t *)malloc((size_t)3 * n * sizeof(int32_t));
   double *tab = (double *)malloc(sizeof(double) * 32 * (3 + 18 + 3 + 1));
   if (!lev || !ijk || !tab) { free(lev); free(ijk); free(tab); return FVG_ERR_NOMEM; }
-  for (int i = 0; i < n; ++i) {
-    const BRANCH *b = fr->leaves[i];
-    lev[i] = b->level;
-    ijk[3 * i] = b->fv_ijk[0];
-    ijk[3 * i + 1] = b->fv_ijk[1];
-    ijk[3 * i + 2] = b->fv_ijk[2];
-  }
+  keys_ctx kc = {fr->leaves, lev, ijk};
+  par_for(n, 8192, keys_part, &kc);
   int32_t own[FVG_REMESH_OWNERS];
   int rc = fvg_remesh_begin(ctx, n, lev, ijk, fr->nroot, fr->periodic, fr->bc, own);
   free(lev);

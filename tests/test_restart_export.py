@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import fv_140823_amd as fv
+from fv_140823_amd import configs
 from fv_140823_amd._capi import FIELD_SG
 from helpers import amr, uniform
 
@@ -38,6 +39,27 @@ def test_binary_restart_round_trip(tmp_path, periodic):
         assert np.array_equal(a, b)
     A, B = _flat_arrays(F), _flat_arrays(G)
     assert A.keys() == B.keys()
+    for k in A:
+        assert np.array_equal(A[k], B[k]), k
+
+
+def test_incremental_adaptation_equals_a_full_rebuild(tmp_path):
+    """At a size where the host splits the leaf list and the leaf refresh over threads
+    (tens of thousands of leaves): the forest after adaptations, whose neighbours and
+    geometry were refreshed only around the changed nodes, flattens to the same mesh as
+    the forest rebuilt whole from its restart file."""
+    F, _, meta = configs.blast(5, 2)
+    for k in range(2):
+        xc, lev = F.leaf_info()
+        r = np.sqrt(((xc - 0.5) ** 2).sum(axis=1))
+        f = np.zeros(F.nleaves, np.int8)
+        f[(np.abs(r - 0.2 - 0.03 * k) < 0.04) & (lev < meta["max_level"])] = 1
+        f[r < 0.12] = -1
+        F.adapt(f, meta["max_level"])
+    assert F.nleaves > 40000
+    F.save(tmp_path / "r.bin", 0, 0.0)
+    G, _, _ = fv.Forest.load(tmp_path / "r.bin")
+    A, B = _flat_arrays(F), _flat_arrays(G)
     for k in A:
         assert np.array_equal(A[k], B[k]), k
 
