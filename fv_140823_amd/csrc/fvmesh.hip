@@ -33,7 +33,8 @@
 
 #include "fv_common.h"
 
-namespace {
+/* a named namespace: profilers print kernels of an anonymous one without a name */
+namespace fvmesh {
 
 constexpr uint64_t KEY_EMPTY = ~0ull;
 constexpr int KEY_BITS = 19; /* global cell index per axis < 2^19 */
@@ -146,18 +147,13 @@ __global__ __launch_bounds__(256) void k_rm_insert(int n, const uint64_t *__rest
   }
 }
 
-/* owners: [0, 32) h rows (leaves of level L or L + 1), [32, 64) volumes (level L),
- * [64, 160) areas (level lf of the finer side, axis a) */
-__global__ __launch_bounds__(256) void k_rm_classify(int n, int ns, const uint64_t *__restrict__ keys,
-                                                     const uint64_t *__restrict__ hk,
-                                                     const int32_t *__restrict__ hv, int bits,
-                                                     Geo g, uint32_t *__restrict__ cflag,
-                                                     int32_t *__restrict__ nbr,
-                                                     int32_t *__restrict__ cnt,
-                                                     int32_t *__restrict__ own,
-                                                     int *__restrict__ err) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+/* class and neighbour of each face of cell i; the cell's owner entries go to sown */
+__device__ inline void classify_cell(int i, int n, int ns, const uint64_t *__restrict__ keys,
+                                     const uint64_t *__restrict__ hk,
+                                     const int32_t *__restrict__ hv, int bits, const Geo &g,
+                                     uint32_t *__restrict__ cflag, int32_t *__restrict__ nbr,
+                                     int32_t *__restrict__ cnt, int *sown,
+                                     int *__restrict__ err) {
   int L, c[3];
   unpack_key(keys[i], L, c);
   uint32_t cf = (uint32_t)L << 16;
@@ -193,14 +189,42 @@ __global__ __launch_bounds__(256) void k_rm_classify(int n, int ns, const uint64
   cflag[i] = cf;
   cnt[(size_t)(3 * NKIND) * n + i] = nfin;
   if (e) atomicOr(err, e);
-  atomicMin(&own[L], i);
-  if (L > 1) atomicMin(&own[L - 1], i);
-  atomicMin(&own[32 + L], i);
+  atomicMin(&sown[L], i);
+  if (L > 1) atomicMin(&sown[L - 1], i);
+  atomicMin(&sown[32 + L], i);
   for (int a = 0; a < 3; ++a) {
     const int fa = c_axf[a][0], fb = c_axf[a][1];
     const bool fin_a = ((cf >> (2 * fa)) & 3) == CLS_FINER, fin_b = ((cf >> (2 * fb)) & 3) == CLS_FINER;
-    if (!fin_a || !fin_b) atomicMin(&own[64 + 3 * L + a], i);
-    if ((fin_a || fin_b) && L + 1 < 32) atomicMin(&own[64 + 3 * (L + 1) + a], i);
+    if (!fin_a || !fin_b) atomicMin(&sown[64 + 3 * L + a], i);
+    if ((fin_a || fin_b) && L + 1 < 32) atomicMin(&sown[64 + 3 * (L + 1) + a], i);
+  }
+}
+
+/* owners: [0, 32) h rows (leaves of level L or L + 1), [32, 64) volumes (level L),
+ * [64, 160) areas (level lf of the finer side, axis a).  Every cell names itself for up
+ * to 8 entries and a whole level shares one entry, so the minima are taken in LDS first:
+ * a block sends at most one global atomicMin per entry it touched, and none where the
+ * entry already holds an earlier cell (a stale read only costs an extra atomic). */
+constexpr int NOWN = 160;
+static_assert(NOWN == FVG_REMESH_OWNERS, "owner table size");
+
+__global__ __launch_bounds__(256) void k_rm_classify(int n, int ns, const uint64_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ hk,
+                                                     const int32_t *__restrict__ hv, int bits,
+                                                     Geo g, uint32_t *__restrict__ cflag,
+                                                     int32_t *__restrict__ nbr,
+                                                     int32_t *__restrict__ cnt,
+                                                     int32_t *__restrict__ own,
+                                                     int *__restrict__ err) {
+  __shared__ int sown[NOWN];
+  for (int t = threadIdx.x; t < NOWN; t += 256) sown[t] = INT_MAX;
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) classify_cell(i, n, ns, keys, hk, hv, bits, g, cflag, nbr, cnt, sown, err);
+  __syncthreads();
+  for (int t = threadIdx.x; t < NOWN; t += 256) {
+    const int v = sown[t];
+    if (v != INT_MAX && v < own[t]) atomicMin(&own[t], v);
   }
 }
 
@@ -414,7 +438,8 @@ __global__ __launch_bounds__(256) void k_rm_carry(int n, int neq, Geo g, const u
 
 inline int grid_of(size_t n, int b) { return (int)((n + b - 1) / b); }
 
-}  // namespace
+}  // namespace fvmesh
+using namespace fvmesh;
 
 struct fv_remesh {
   int n = 0, ns = 0, bits = 0;
