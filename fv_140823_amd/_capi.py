@@ -156,6 +156,7 @@ _DECLS = [
                                C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32), _dp]),
     ("fvg_adj_forward", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
+    ("fvg_adj_checkpoint_spacing", C.c_int, [_vp]),
     ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),
     ("fvg_tlm", C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp, _dp]),

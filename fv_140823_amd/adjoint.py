@@ -3,7 +3,7 @@
     J = sum_{n=1..nsteps} dt sum_{i in probe} p(S^n_i) Vol_i
 
 Forward runs the rollout on the device (fvg_adj_forward, checkpoints every
-`ckpt` steps); backward runs the hand-written discrete adjoint
+`ckpt` steps; 0 keeps the whole rollout in HBM where it fits); backward runs the hand-written discrete adjoint
 (fvg_adj_backward) and returns dJ/dS^0.  Tensors stay on the GPU: the C ABI
 copies device to device on the solver's stream, and both sides synchronise
 torch's current stream first.

@@ -429,10 +429,17 @@ class Solver:
                  "fvg_adj_set_probe")
 
     def adj_forward(self, nsteps: int, ckpt: int, dt: float) -> float:
+        """The rollout with S kept every ckpt steps; ckpt = 0 keeps the whole rollout
+        where it fits in half the free device memory (else the widest spacing that does)."""
         J = C.c_double()
         self._ck(lib().fvg_adj_forward(self._h, int(nsteps), int(ckpt), float(dt), C.byref(J)),
                  "fvg_adj_forward")
         return J.value
+
+    @property
+    def adj_checkpoint_spacing(self) -> int:
+        """The checkpoint spacing of the last adj_forward (0 before one)."""
+        return int(lib().fvg_adj_checkpoint_spacing(self._h))
 
     def adj_objective(self):
         """(hi, lo) of the last rollout's objective, J = hi + lo unevaluated."""

@@ -794,12 +794,11 @@ int fvg_adj_set_probe(fvg_ctx *c, const uint8_t *mask) {
 int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   int rc = adj_check(c);
   if (rc) return rc;
-  if (nsteps < 1 || ckpt < 1 || !J) return set_err(c, FVG_ERR_ARG, "bad nsteps/ckpt");
+  if (nsteps < 1 || ckpt < 0 || !J) return set_err(c, FVG_ERR_ARG, "bad nsteps/ckpt");
   (void)hipSetDevice(c->device);
   rc = adj_alloc(c);
   if (rc) return rc;
   fv_adj_state *a = c->adj;
-  const int nck = (nsteps + ckpt - 1) / ckpt;
   const size_t pl = a->plane * 8;
   if (a->ckpts) (void)hipFree(a->ckpts);
   if (a->seg) (void)hipFree(a->seg);
@@ -807,6 +806,21 @@ int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   a->ckpts = a->seg = a->Jsteps = nullptr;
   a->fwd_done = false;
   a->seg_valid = -1;
+  if (ckpt == 0) {
+    /* the widest spacing whose stores, (3 ckpt + 1) segment states and one checkpoint per
+     * segment, fit in half the free device memory: the whole rollout where it fits, so
+     * that the reverse sweep recomputes nothing */
+    size_t fr = 0, tot = 0;
+    HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    const size_t budget = fr / 2;
+    ckpt = 1;
+    for (int k = nsteps; k >= 1; --k) {
+      const size_t need = ((size_t)3 * k + 1 + (size_t)(nsteps + k - 1) / k) * pl;
+      if (need <= budget) { ckpt = k; break; }
+    }
+  }
+  if (ckpt > nsteps) ckpt = nsteps;
+  const int nck = (nsteps + ckpt - 1) / ckpt;
   HIPCHK(c, hipMalloc(&a->ckpts, (size_t)nck * pl));
   HIPCHK(c, hipMalloc(&a->seg, (size_t)(3 * ckpt + 1) * pl));
   HIPCHK(c, hipMalloc(&a->Jsteps, (size_t)2 * nsteps * 8));
@@ -841,6 +855,11 @@ int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
   return FVG_OK;
+}
+
+int fvg_adj_checkpoint_spacing(const fvg_ctx *c) {
+  if (!c) return FVG_ERR_ARG;
+  return (c->adj && c->adj->fwd_done) ? c->adj->ckpt : 0;
 }
 
 int fvg_adj_backward(fvg_ctx *c, double *grad_host, void *grad_dev) {

@@ -162,8 +162,9 @@ def study_vortex(steps=50, ckpt=10, base_level=6):
         t_fwd = time.perf_counter() - t0
         res = dict(forward_s=t_fwd)
         # checkpoints every `ckpt` steps (segments recomputed in the reverse sweep), and
-        # one segment over the whole rollout (every stage input kept in HBM, no recompute)
-        for ck, key in ((ckpt, ""), (steps, "_full_store")):
+        # the spacing chosen from the free device memory: one segment over the whole
+        # rollout here (every stage input kept in HBM, no recompute)
+        for ck, key in ((ckpt, ""), (0, "_full_store")):
             g.set_state(S0)
             g.sync()
             t0 = time.perf_counter()
@@ -183,7 +184,8 @@ def study_vortex(steps=50, ckpt=10, base_level=6):
             del lam_dev
             res.update({f"J{key}": J, f"forward_with_checkpoints_s{key}": t_fwd_ck,
                         f"adjoint_s{key}": t_adj, f"adjoint_over_forward{key}": t_adj / t_fwd,
-                        f"gradient_download_s{key}": t_dl})
+                        f"gradient_download_s{key}": t_dl,
+                        f"checkpoint_spacing{key}": g.adj_checkpoint_spacing})
             if not key:
                 grads[tag] = lam
             else:

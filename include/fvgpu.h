@@ -263,8 +263,13 @@ int fvg_time_steps(fvg_ctx *ctx, int nsteps, int method, double dt, double *ms_t
  * steps at a fixed dt from the context's S (model 1, inviscid; either limiter).  This is an
  * extension: the reference has no adjoint. */
 int fvg_adj_set_probe(fvg_ctx *ctx, const uint8_t *mask /* host [ncell], 1 = in the probe box */);
-/* Runs the rollout, keeping S every `ckpt` steps; *J = objective; S ends at the final state. */
+/* Runs the rollout, keeping S every `ckpt` steps; *J = objective; S ends at the final state.
+ * ckpt = 0 picks the widest spacing whose stores fit in half the free device memory: the
+ * whole rollout where it fits, so that the reverse sweep recomputes nothing.  The spacing
+ * does not change the gradient. */
 int fvg_adj_forward(fvg_ctx *ctx, int nsteps, int ckpt, double dt, double *J);
+/* The checkpoint spacing of the last fvg_adj_forward (0 before one). */
+int fvg_adj_checkpoint_spacing(const fvg_ctx *ctx);
 /* The objective of the last fvg_adj_forward / fvg_tlm as an unevaluated double-double sum
  * J_hi_lo[0] + J_hi_lo[1] (accumulated with error-free transformations), for finite
  * differences finer than the rounding of J itself. */

@@ -4,7 +4,8 @@ Navier-Stokes:
     two independent derivative codes (hand-written reverse vs dual numbers);
   - against central finite differences of J (FP64, eps = 1e-6 relative);
   - the torch.autograd.Function wrapper returns the same gradient;
-  - checkpoint spacing does not change the gradient.
+  - checkpoint spacing does not change the gradient (given, chosen from the free device
+    memory, or wider than the rollout).
 """
 import numpy as np
 import pytest
@@ -118,9 +119,11 @@ def test_checkpoint_spacing_invariant():
     S0 = F.get_state()
     dt = 0.5 * g.cfl_dt(0.4)
     out = []
-    for ck in (1, 3, 9):
+    # 0: the spacing chosen from the free device memory, the whole rollout at this size
+    for ck in (1, 3, 9, 0, 20):
         g.set_state(S0)
         out.append((g.adj_forward(9, ck, dt), g.adj_backward()))
+        assert g.adj_checkpoint_spacing == (9 if ck in (0, 20) else ck)
     for J, lam in out[1:]:
         assert J == out[0][0]
         assert np.array_equal(lam, out[0][1])
