@@ -243,7 +243,7 @@ static int ncell_axis(const struct fvh_forest *fr, int level, int a) {
   return fr->nroot[a] << (level - 1);
 }
 
-static void set_nodes(struct fvh_forest *fr, BRANCH *b) {
+static void set_nodes(const struct fvh_forest *fr, BRANCH *b) {
   double L[3];
   int N[3];
   for (int a = 0; a < 3; ++a) {
@@ -357,7 +357,7 @@ static int64_t pow10_i64(int e) {
   return r;
 }
 
-static BRANCH *new_branch(struct fvh_forest *fr, BRANCH *prnt, int level, const int ijk[3],
+static BRANCH *new_branch(const struct fvh_forest *fr, BRANCH *prnt, int level, const int ijk[3],
                           int root) {
   BRANCH *b = branch_alloc(fr->neq);
   if (!b) return NULL;
@@ -375,13 +375,21 @@ static BRANCH *new_branch(struct fvh_forest *fr, BRANCH *prnt, int level, const 
   return b;
 }
 
-static int split_branch(struct fvh_forest *fr, BRANCH *b) {
-  if (b->nkids) return 0;
+/* The 8 kids of leaf b, allocated and initialised but not yet in the tree or the hash
+ * (b->kids is filled, b->nkids stays 0).  Touches only b and the new blocks, so
+ * different leaves may be done on different threads.  On failure nothing is left. */
+static int split_kids(const struct fvh_forest *fr, BRANCH *b) {
   for (int c = 0; c < 8; ++c) {
     int ijk[3] = {2 * b->fv_ijk[0] + (c & 1), 2 * b->fv_ijk[1] + ((c >> 1) & 1),
                   2 * b->fv_ijk[2] + ((c >> 2) & 1)};
     BRANCH *k = new_branch(fr, b, b->level + 1, ijk, b->root);
-    if (!k) return FVG_ERR_NOMEM;
+    if (!k) {
+      for (int q = 0; q < c; ++q) {
+        free(b->kids[q]);
+        b->kids[q] = NULL;
+      }
+      return FVG_ERR_NOMEM;
+    }
     for (int v = 0; v < fr->neq; ++v) {
       k->el->S[v] = b->el->S[v];
       k->el->SG[v][0] = b->el->SG[v][0];
@@ -393,12 +401,33 @@ static int split_branch(struct fvh_forest *fr, BRANCH *b) {
     int64_t t = b->tag < 0 ? -1 : (int64_t)b->tag + pow10_i64(b->level - 1) * (1 + c);
     k->tag = (t >= 0 && t <= INT32_MAX) ? (int)t : -1;
     b->kids[c] = k;
-    int rc = ht_insert(fr, k);
-    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* kids made by split_kids into the hash and the tree */
+static int split_link(struct fvh_forest *fr, BRANCH *b) {
+  for (int c = 0; c < 8; ++c) {
+    int rc = ht_insert(fr, b->kids[c]);
+    if (rc) {
+      for (int q = 0; q < c; ++q) ht_remove(fr, b->kids[q]);
+      for (int q = 0; q < 8; ++q) {
+        free(b->kids[q]);
+        b->kids[q] = NULL;
+      }
+      return rc;
+    }
   }
   b->nkids = 8;
   b->nsplit = 8;
   return 0;
+}
+
+/* a leaf's kids may already have been made (split_kids) and wait to be linked */
+static int split_branch(struct fvh_forest *fr, BRANCH *b) {
+  if (b->nkids) return 0;
+  int rc = b->kids[0] ? 0 : split_kids(fr, b);
+  return rc ? rc : split_link(fr, b);
 }
 
 static void free_subtree(BRANCH *b) {
@@ -949,6 +978,20 @@ static int split_queue(struct fvh_forest *fr, BRANCH *b, bvec *chg, bvec *work) 
   return rc;
 }
 
+typedef struct {
+  const struct fvh_forest *fr;
+  BRANCH *const *leaves;
+  int rc;
+} kids_ctx;
+
+static void kids_part(void *v, int lo, int hi) {
+  kids_ctx *k = (kids_ctx *)v;
+  for (int q = lo; q < hi; ++q) {
+    int rc = split_kids(k->fr, k->leaves[q]);
+    if (rc) __atomic_store_n(&k->rc, rc, __ATOMIC_RELAXED);
+  }
+}
+
 int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   if (!fr || !flag) return FVG_ERR_ARG;
   const int N = fr->nleaves;
@@ -974,8 +1017,26 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   for (int i = 0; i < N; ++i) nflag += lv[i]->split;
   bvec chg = {0}, work = {0};
   int rc = ht_reserve(fr, fr->hcount + 8 * nflag);
-  for (int i = 0; i < N && !rc; ++i)
-    if (lv[i]->split) rc = split_queue(fr, lv[i], &chg, &work);
+  /* the flagged leaves' kids are made on several threads, then linked in list order */
+  BRANCH **sp = rc ? NULL : (BRANCH **)malloc((nflag ? nflag : 1) * sizeof(BRANCH *));
+  if (!rc && !sp) rc = FVG_ERR_NOMEM;
+  if (!rc) {
+    size_t q = 0;
+    for (int i = 0; i < N; ++i)
+      if (lv[i]->split) sp[q++] = lv[i];
+    kids_ctx kc = {fr, sp, 0};
+    par_for((int)nflag, 256, kids_part, &kc);
+    rc = kc.rc;
+    for (size_t k = 0; k < nflag && !rc; ++k) rc = split_queue(fr, sp[k], &chg, &work);
+    /* after a failure, kids made but not linked are dropped */
+    for (size_t k = 0; k < nflag && rc; ++k)
+      if (!sp[k]->nkids)
+        for (int c = 0; c < 8; ++c) {
+          free(sp[k]->kids[c]);
+          sp[k]->kids[c] = NULL;
+        }
+  }
+  free(sp);
   /* 2:1 face balance by further splits */
   while (!rc && work.n) {
     BRANCH *c = work.p[--work.n];

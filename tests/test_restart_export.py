@@ -43,19 +43,50 @@ def test_binary_restart_round_trip(tmp_path, periodic):
         assert np.array_equal(A[k], B[k]), k
 
 
+def _keys(xc, lev):
+    """(level, i, j, k) of leaves of the unit box with one root"""
+    h = 1.0 / (1 << (lev.astype(np.int64) - 1))
+    ijk = np.round(xc / h[:, None] - 0.5).astype(np.int64)
+    return [(int(l), int(a), int(b), int(c)) for l, (a, b, c) in zip(lev, ijk)]
+
+
 def test_incremental_adaptation_equals_a_full_rebuild(tmp_path):
-    """At a size where the host splits the leaf list and the leaf refresh over threads
-    (tens of thousands of leaves): the forest after adaptations, whose neighbours and
-    geometry were refreshed only around the changed nodes, flattens to the same mesh as
-    the forest rebuilt whole from its restart file."""
+    """At a size where the host splits the kid allocation, the leaf list and the leaf
+    refresh over threads (tens of thousands of leaves): the leaves carry the split and
+    merge states, and the forest after adaptations, whose neighbours and geometry were
+    refreshed only around the changed nodes, flattens to the same mesh as the forest
+    rebuilt whole from its restart file."""
     F, _, meta = configs.blast(5, 2)
+    rng = np.random.default_rng(3)
     for k in range(2):
         xc, lev = F.leaf_info()
+        S = rng.random((5, F.nleaves))
+        F.set_state(S)
+        old = {key: q for q, key in enumerate(_keys(xc, lev))}
         r = np.sqrt(((xc - 0.5) ** 2).sum(axis=1))
         f = np.zeros(F.nleaves, np.int8)
         f[(np.abs(r - 0.2 - 0.03 * k) < 0.04) & (lev < meta["max_level"])] = 1
         f[r < 0.12] = -1
         F.adapt(f, meta["max_level"])
+        # the kids made on the host's threads carry their parent's state (split.c:492-502),
+        # merged parents the mean of their 8 kids in kid order (merge.c:101)
+        xc2, lev2 = F.leaf_info()
+        S2 = F.get_state()
+        nsplit = nmerge = 0
+        for q, (L, i, j, kk) in enumerate(_keys(xc2, lev2)):
+            if (L, i, j, kk) in old:
+                want = S[:, old[(L, i, j, kk)]]
+            elif (L - 1, i >> 1, j >> 1, kk >> 1) in old:
+                want = S[:, old[(L - 1, i >> 1, j >> 1, kk >> 1)]]
+                nsplit += 1
+            else:
+                want = np.zeros(5)
+                for c in range(8):
+                    want = want + S[:, old[(L + 1, 2 * i + (c & 1), 2 * j + ((c >> 1) & 1),
+                                            2 * kk + (c >> 2))]] / 8.0
+                nmerge += 1
+            assert np.array_equal(S2[:, q], want), (L, i, j, kk)
+        assert nsplit > 8 * 512 and nmerge > 0
     assert F.nleaves > 40000
     F.save(tmp_path / "r.bin", 0, 0.0)
     G, _, _ = fv.Forest.load(tmp_path / "r.bin")
