@@ -436,9 +436,10 @@ static void free_subtree(BRANCH *b) {
 }
 
 /* merge.c:85-107: the parent's volume is computed before the kids' gradients are
- * volume-weighted into it (a parent created and split in one pass has none yet);
- * the kids leave the hash */
-static int merge_branch(struct fvh_forest *fr, BRANCH *b) {
+ * volume-weighted into it (a parent created and split in one pass has none yet).
+ * Touches only b and reads its kids, so different parents may be done on different
+ * threads. */
+static void merge_average(const struct fvh_forest *fr, BRANCH *b) {
   const int neq = fr->neq;
   set_volume(b);
   for (int v = 0; v < neq; ++v) {
@@ -450,6 +451,10 @@ static int merge_branch(struct fvh_forest *fr, BRANCH *b) {
       for (int d = 0; d < 3; ++d) b->el->SG[v][d] += k->el->SG[v][d] * k->cl->Vol / b->cl->Vol;
     }
   }
+}
+
+/* the kids of a merged parent leave the hash and the tree */
+static void merge_unlink(struct fvh_forest *fr, BRANCH *b) {
   for (int c = 0; c < 8; ++c) {
     ht_remove(fr, b->kids[c]);
     free_subtree(b->kids[c]);
@@ -457,7 +462,6 @@ static int merge_branch(struct fvh_forest *fr, BRANCH *b) {
   }
   b->nkids = 0;
   b->nsplit = 0;
-  return 0;
 }
 
 /* appends the leaves of b's subtree in depth-first order; FVG_ERR_NOMEM leaves *out
@@ -992,29 +996,78 @@ static void kids_part(void *v, int lo, int hi) {
   }
 }
 
+/* the per-leaf and per-candidate passes of an adaptation, each on the thread pool: every
+ * item writes only its own entries and nodes */
+typedef struct {
+  const struct fvh_forest *fr;
+  BRANCH *const *lv;
+  const int8_t *flag;
+  int max_level;
+  uint8_t *split, *first; /* per leaf */
+  BRANCH *const *cand;
+  uint8_t *keep; /* per candidate */
+} pass_ctx;
+
+static void flag_part(void *v, int lo, int hi) {
+  pass_ctx *p = (pass_ctx *)v;
+  for (int i = lo; i < hi; ++i) {
+    BRANCH *b = p->lv[i];
+    b->split = (p->flag[i] > 0 && b->level < p->max_level);
+    b->merge = (p->flag[i] < 0);
+    p->split[i] = (uint8_t)b->split;
+  }
+}
+
+static void cand_part(void *v, int lo, int hi) {
+  pass_ctx *p = (pass_ctx *)v;
+  for (int i = lo; i < hi; ++i) {
+    const BRANCH *b = p->lv[i];
+    int ok = b->level > 1 && b->prnt->kids[0] == b;
+    for (int c = 0; c < 8 && ok; ++c) ok = (b->prnt->kids[c]->nkids == 0 && b->prnt->kids[c]->merge);
+    p->first[i] = (uint8_t)ok;
+  }
+}
+
+static void merge_decide_part(void *v, int lo, int hi) {
+  pass_ctx *p = (pass_ctx *)v;
+  for (int q = lo; q < hi; ++q) {
+    const BRANCH *b = p->cand[q];
+    int ok = 1;
+    for (int c = 0; c < 8 && ok; ++c) ok = (b->kids[c]->nkids == 0);
+    p->keep[q] = (uint8_t)(ok && merge_allowed(p->fr, b));
+  }
+}
+
+static void merge_average_part(void *v, int lo, int hi) {
+  pass_ctx *p = (pass_ctx *)v;
+  for (int q = lo; q < hi; ++q) merge_average(p->fr, p->cand[q]);
+}
+
+static void clear_part(void *v, int lo, int hi) {
+  pass_ctx *p = (pass_ctx *)v;
+  for (int i = lo; i < hi; ++i) p->lv[i]->split = p->lv[i]->merge = 0;
+}
+
 int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   if (!fr || !flag) return FVG_ERR_ARG;
   const int N = fr->nleaves;
   BRANCH **lv = fr->leaves;
-  for (int i = 0; i < N; ++i) {
-    lv[i]->split = (flag[i] > 0 && lv[i]->level < max_level);
-    lv[i]->merge = (flag[i] < 0);
-  }
-  /* merge candidates on the old tree: parents whose 8 kids are leaves flagged -1 */
-  int ncand = 0;
+  /* per leaf: flagged split, and first kid of a merge candidate (a parent on the old
+   * tree whose 8 kids are leaves flagged -1) */
+  uint8_t *sm = (uint8_t *)malloc((size_t)2 * N + 1);
   BRANCH **cand = (BRANCH **)malloc((size_t)(N / 8 + 1) * sizeof(BRANCH *));
-  if (!cand) return FVG_ERR_NOMEM;
-  for (int i = 0; i < N; ++i) {
-    BRANCH *b = lv[i];
-    if (b->level <= 1 || b->prnt->kids[0] != b) continue;
-    BRANCH *p = b->prnt;
-    int ok = 1;
-    for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0 && p->kids[c]->merge);
-    if (ok) cand[ncand++] = p;
-  }
-  /* room for the flagged splits; balance splits grow the hash as they go */
+  if (!sm || !cand) { free(sm); free(cand); return FVG_ERR_NOMEM; }
+  pass_ctx pc = {fr, lv, flag, max_level, sm, sm + N, NULL, NULL};
+  par_for(N, 8192, flag_part, &pc);
+  par_for(N, 8192, cand_part, &pc);
+  int ncand = 0;
   size_t nflag = 0;
-  for (int i = 0; i < N; ++i) nflag += lv[i]->split;
+  for (int i = 0; i < N; ++i) {
+    nflag += sm[i];
+    if (sm[N + i]) cand[ncand++] = lv[i]->prnt;
+  }
+  free(sm);
+  /* room for the flagged splits; balance splits grow the hash as they go */
   bvec chg = {0}, work = {0};
   int rc = ht_reserve(fr, fr->hcount + 8 * nflag);
   /* the flagged leaves' kids are made on several threads, then linked in list order */
@@ -1044,24 +1097,31 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   }
   free(work.p);
   if (rc) { free(cand); free(chg.p); return rc; }
-  /* decide every merge on the same (post-split) tree, then apply */
+  /* decide every merge on the same (post-split) tree, then apply: the decisions and the
+   * means on the thread pool, the unlinking in order */
   int nm = 0;
-  for (int q = 0; q < ncand; ++q) {
-    BRANCH *p = cand[q];
-    int ok = 1;
-    for (int c = 0; c < 8 && ok; ++c) ok = (p->kids[c]->nkids == 0);
-    if (ok && merge_allowed(fr, p)) cand[nm++] = p;
+  uint8_t *keep = rc ? NULL : (uint8_t *)malloc((size_t)ncand + 1);
+  if (!rc && !keep) rc = FVG_ERR_NOMEM;
+  if (!rc) {
+    pc.cand = cand;
+    pc.keep = keep;
+    par_for(ncand, 1024, merge_decide_part, &pc);
+    for (int q = 0; q < ncand; ++q)
+      if (keep[q]) cand[nm++] = cand[q];
+    par_for(nm, 1024, merge_average_part, &pc);
+    for (int q = 0; q < nm && !rc; ++q) {
+      merge_unlink(fr, cand[q]);
+      rc = bvec_push(&chg, cand[q]);
+    }
   }
-  for (int q = 0; q < nm && !rc; ++q) {
-    merge_branch(fr, cand[q]);
-    rc = bvec_push(&chg, cand[q]);
-  }
+  free(keep);
   free(cand);
   /* fr->leaves still lists freed kids of merged parents: rebuild before any use */
   if (!rc) rc = rebuild_changed(fr, &chg);
   free(chg.p);
   if (rc) return rc;
-  for (int i = 0; i < fr->nleaves; ++i) fr->leaves[i]->split = fr->leaves[i]->merge = 0;
+  pc.lv = fr->leaves;
+  par_for(fr->nleaves, 8192, clear_part, &pc);
   return fr->nleaves;
 }
 
