@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fvgpu.h"
@@ -365,7 +366,29 @@ struct fvg_ctx {
   int hbits = 0;
   /* a device re-flatten between fvg_remesh_begin and fvg_remesh_end */
   struct fv_remesh *rm = nullptr;
+  /* mesh blocks (fv_dmalloc / fv_dfree): handed out, and freed but kept for reuse */
+  std::vector<std::pair<void *, size_t>> dlive, dcache;
 };
+
+/* Device blocks of the mesh arrays.  A rebuild after an adaptation frees and allocates a
+ * few hundred MB of arrays of nearly the same sizes; hipMalloc / hipFree of them cost
+ * milliseconds, so freed blocks are kept and handed out again to a request that fits
+ * (at most twice its size).  Every free happens after the context's stream has drained,
+ * and every later use of a block is ordered on that stream.  slack adds a quarter of
+ * room to a fresh block, for meshes that grow from one rebuild to the next.  The cache
+ * holds at most as many bytes as are live once a rebuild is done (fv_dcache_trim), and
+ * is emptied when an allocation fails and at fvg_destroy. */
+hipError_t fv_dmalloc_raw(fvg_ctx *c, void **p, size_t bytes, bool slack);
+template <class T>
+inline hipError_t fv_dmalloc(fvg_ctx *c, T **p, size_t bytes, bool slack) {
+  void *q = nullptr;
+  const hipError_t e = fv_dmalloc_raw(c, &q, bytes, slack);
+  *p = (T *)q;
+  return e;
+}
+void fv_dfree(fvg_ctx *c, void *p);
+void fv_dcache_trim(fvg_ctx *c);
+void fv_dcache_clear(fvg_ctx *c);
 
 /* releases ctx->adj; defined in fvadj.hip */
 void fv_adj_release(fvg_ctx *c);

@@ -882,13 +882,72 @@ static int check_params(const fvg_params *p) {
   return FVG_OK;
 }
 
+/* a block size rounded up to an eighth of its leading power of two */
+static size_t block_class(size_t b) {
+  if (b <= 4096) return 4096;
+  const size_t q = (size_t)1 << (63 - __builtin_clzll((unsigned long long)b) - 3);
+  return (b + q - 1) / q * q;
+}
+
+hipError_t fv_dmalloc_raw(fvg_ctx *c, void **p, size_t bytes, bool slack) {
+  *p = nullptr;
+  if (!bytes) bytes = 1;
+  int best = -1;
+  for (int k = 0; k < (int)c->dcache.size(); ++k) {
+    const size_t cap = c->dcache[k].second;
+    if (cap >= bytes && cap <= 2 * bytes + 4096 && (best < 0 || cap < c->dcache[best].second)) best = k;
+  }
+  if (best >= 0) {
+    c->dlive.push_back(c->dcache[best]);
+    c->dcache.erase(c->dcache.begin() + best);
+    *p = c->dlive.back().first;
+    return hipSuccess;
+  }
+  const size_t cap = block_class(slack ? bytes + bytes / 4 : bytes);
+  hipError_t e = hipMalloc(p, cap);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fv_dcache_clear(c);
+    e = hipMalloc(p, cap);
+  }
+  if (e == hipSuccess) c->dlive.emplace_back(*p, cap);
+  else *p = nullptr;
+  return e;
+}
+
+void fv_dfree(fvg_ctx *c, void *p) {
+  if (!p) return;
+  for (size_t k = 0; k < c->dlive.size(); ++k)
+    if (c->dlive[k].first == p) {
+      c->dcache.push_back(c->dlive[k]);
+      c->dlive.erase(c->dlive.begin() + (long)k);
+      return;
+    }
+  (void)hipFree(p);
+}
+
+void fv_dcache_trim(fvg_ctx *c) {
+  size_t live = 0, cached = 0;
+  for (auto &b : c->dlive) live += b.second;
+  for (auto &b : c->dcache) cached += b.second;
+  while (!c->dcache.empty() && cached > live) {
+    cached -= c->dcache.front().second;
+    (void)hipFree(c->dcache.front().first);
+    c->dcache.erase(c->dcache.begin());
+  }
+}
+
+void fv_dcache_clear(fvg_ctx *c) {
+  for (auto &b : c->dcache) (void)hipFree(b.first);
+  c->dcache.clear();
+}
+
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, c->tab,
                   c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf, c->keys,
                   c->hkeys, c->hval};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  for (void *p : ptrs) fv_dfree(c, p);
   c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->fLL = c->fRR = c->gslot = nullptr;
   c->fmeta = nullptr; c->tab = nullptr;
   c->S = c->SN = c->RHS = c->SG = nullptr;
@@ -923,27 +982,28 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   }
   const int ns = c->ns, neq = c->neq;
   const size_t rs = c->rsize;
+  const bool slack = !topo; /* a mesh rebuilt by adaptation may grow next time */
   if (topo) {
-    HIPCHK(c, hipMalloc(&c->cflag, (size_t)ns * 4));
-    HIPCHK(c, hipMalloc(&c->nbr, (size_t)6 * ns * 4));
+    HIPCHK(c, fv_dmalloc(c, &c->cflag, (size_t)ns * 4, slack));
+    HIPCHK(c, fv_dmalloc(c, &c->nbr, (size_t)6 * ns * 4, slack));
   }
-  HIPCHK(c, hipMalloc(&c->gslot, (size_t)6 * ns * 4));
-  HIPCHK(c, hipMalloc(&c->quad, (size_t)4 * (Q ? Q : 1) * 4));
-  HIPCHK(c, hipMalloc(&c->fL, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fR, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fLL, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fRR, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->fmeta, (size_t)c->nfp * 4));
-  HIPCHK(c, hipMalloc(&c->tab, (size_t)34 * nl * 8));
-  HIPCHK(c, hipMalloc(&c->S, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SN, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->RHS, (size_t)neq * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SG, (size_t)(neq + 1) * 3 * ns * 8));
-  HIPCHK(c, hipMalloc(&c->SF, (size_t)6 * neq * ns * rs));
-  HIPCHK(c, hipMalloc(&c->SGm, (size_t)9 * ns * rs));
-  HIPCHK(c, hipMalloc(&c->G, (size_t)neq * c->nfp * rs));
-  HIPCHK(c, hipMalloc(&c->Sx, (size_t)neq * ns * 8));
-  if (rs == 4 && c->prm.model == 1) HIPCHK(c, hipMalloc(&c->Sf, (size_t)neq * ns * 4));
+  HIPCHK(c, fv_dmalloc(c, &c->gslot, (size_t)6 * ns * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->quad, (size_t)4 * (Q ? Q : 1) * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->fL, (size_t)c->nfp * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->fR, (size_t)c->nfp * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->fLL, (size_t)c->nfp * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->fRR, (size_t)c->nfp * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->fmeta, (size_t)c->nfp * 4, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->tab, (size_t)34 * nl * 8, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->S, (size_t)neq * ns * 8, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->SN, (size_t)neq * ns * 8, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->RHS, (size_t)neq * ns * 8, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->SG, (size_t)(neq + 1) * 3 * ns * 8, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->SF, (size_t)6 * neq * ns * rs, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)9 * ns * rs, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->G, (size_t)neq * c->nfp * rs, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->Sx, (size_t)neq * ns * 8, slack));
+  if (rs == 4 && c->prm.model == 1) HIPCHK(c, fv_dmalloc(c, &c->Sf, (size_t)neq * ns * 4, slack));
   c->sf_valid = false;
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
@@ -954,6 +1014,7 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
   HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)9 * ns * rs, st));
   HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
+  fv_dcache_trim(c);
   return FVG_OK;
 }
 
@@ -1009,6 +1070,7 @@ int fvg_destroy(fvg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   fv_remesh_release(c);
   free_mesh(c);
+  fv_dcache_clear(c);
   if (c->err) (void)hipFree(c->err);
   if (c->red) (void)hipFree(c->red);
   if (c->redu) (void)hipFree(c->redu);

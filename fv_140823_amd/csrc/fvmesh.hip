@@ -456,8 +456,7 @@ void fv_remesh_release(fvg_ctx *c) {
   if (!c || !c->rm) return;
   fv_remesh *r = c->rm;
   void *ptrs[] = {r->keys, r->hk, r->hv, r->cflag, r->nbr, r->cnt, r->off, r->tsum, r->info};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  for (void *p : ptrs) fv_dfree(c, p);
   delete r;
   c->rm = nullptr;
 }
@@ -511,15 +510,15 @@ int fvg_remesh_begin(fvg_ctx *c, int32_t n, const int32_t *level, const int32_t 
   const size_t cap = (size_t)1 << bits, ns = (size_t)r->ns;
   const int ntile = grid_of((size_t)n, SCAN_TILE);
   hipStream_t st = c->stream;
-  HIPCHK(c, hipMalloc(&r->keys, (size_t)n * 8));
-  HIPCHK(c, hipMalloc(&r->hk, cap * 8));
-  HIPCHK(c, hipMalloc(&r->hv, cap * 4));
-  HIPCHK(c, hipMalloc(&r->cflag, ns * 4));
-  HIPCHK(c, hipMalloc(&r->nbr, 6 * ns * 4));
-  HIPCHK(c, hipMalloc(&r->cnt, (size_t)NCOL * n * 4));
-  HIPCHK(c, hipMalloc(&r->off, (size_t)NCOL * n * 4));
-  HIPCHK(c, hipMalloc(&r->tsum, (size_t)NCOL * ntile * 4));
-  HIPCHK(c, hipMalloc(&r->info, (size_t)(1 + NCOL + FVG_REMESH_OWNERS) * 4));
+  HIPCHK(c, fv_dmalloc(c, &r->keys, (size_t)n * 8, true));
+  HIPCHK(c, fv_dmalloc(c, &r->hk, cap * 8, true));
+  HIPCHK(c, fv_dmalloc(c, &r->hv, cap * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->cflag, ns * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->nbr, 6 * ns * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->cnt, (size_t)NCOL * n * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->off, (size_t)NCOL * n * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->tsum, (size_t)NCOL * ntile * 4, true));
+  HIPCHK(c, fv_dmalloc(c, &r->info, (size_t)(1 + NCOL + FVG_REMESH_OWNERS) * 4, true));
   HIPCHK(c, hipMemcpyAsync(r->keys, keys.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_fill_u64v, dim3(1024), dim3(256), 0, st, r->hk, cap, KEY_EMPTY);
   hipLaunchKernelGGL(k_fill_i32v, dim3(1), dim3(256), 0, st, r->info, (size_t)1 + NCOL, 0);
@@ -609,9 +608,10 @@ int fvg_remesh_end(fvg_ctx *c, int32_t nlevel, const double *h, const double *di
       hipMemcpyAsync(&carry_err, r->info, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = set_err(c, FVG_ERR_HIP, "remesh: copy failed");
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = set_err(c, FVG_ERR_HIP, "remesh: kernels failed");
+  /* the old blocks go back to the cache only once nothing queued can still read them */
+  if (rc) (void)hipStreamSynchronize(st);
   void *old[] = {So, okeys, ohk, ohv};
-  for (void *p : old)
-    if (p) (void)hipFree(p);
+  for (void *p : old) fv_dfree(c, p);
   if (rc) {
     fv_remesh_release(c);
     return rc;
@@ -623,6 +623,7 @@ int fvg_remesh_end(fvg_ctx *c, int32_t nlevel, const double *h, const double *di
   r->keys = r->hk = nullptr;
   r->hv = nullptr;
   fv_remesh_release(c);
+  fv_dcache_trim(c);
   c->has_mesh = true;
   if (carry_err) return set_err(c, FVG_ERR_STATE, remesh_msg(carry_err));
   return FVG_OK;
