@@ -28,21 +28,83 @@
 
 /* ------------------------------------------------------------------ threads */
 
-/* [0, n) split into equal chunks over up to FVH_THREADS host threads (the caller takes
- * the first); fn(ctx, lo, hi) may touch only what belongs to its range.  The result
- * never depends on the split.  Plain pthreads: the process may already hold another
- * OpenMP runtime (torch's). */
+/* [0, n) split into chunks over up to FVH_THREADS host threads; fn(ctx, lo, hi) may touch
+ * only what belongs to its range, and the result never depends on the split.  Plain
+ * pthreads, as the process may already hold another OpenMP runtime (torch's).  The
+ * workers are started on first use and kept (starting and joining 15 threads costs about
+ * half a millisecond, and an adaptation makes about ten passes); one pass runs at a time,
+ * and fn must not itself call par_for. */
 #define FVH_THREADS 16
-typedef struct {
+
+static struct {
+  pthread_mutex_t mu;
+  pthread_cond_t go, idle;
+  int started;            /* workers running */
+  unsigned long gen;      /* pass number */
   void (*fn)(void *, int, int);
   void *ctx;
-  int lo, hi;
-} par_task;
+  int n, nchunk;
+  int next;               /* next chunk to hand out (atomic) */
+  int active;             /* threads inside the current pass (under mu) */
+  int left;               /* chunks of the current pass not yet done (under mu) */
+} pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER,
+          0, 0, NULL, NULL, 0, 0, 0, 0, 0};
+static pthread_mutex_t pool_user = PTHREAD_MUTEX_INITIALIZER;
 
-static void *par_run(void *a) {
-  par_task *t = (par_task *)a;
-  t->fn(t->ctx, t->lo, t->hi);
+static int pool_chunks(void (*fn)(void *, int, int), void *ctx, int n, int nchunk) {
+  int done = 0;
+  for (;;) {
+    const int k = __atomic_fetch_add(&pool.next, 1, __ATOMIC_RELAXED);
+    if (k >= nchunk) return done;
+    fn(ctx, (int)((int64_t)n * k / nchunk), (int)((int64_t)n * (k + 1) / nchunk));
+    ++done;
+  }
+}
+
+static void *pool_worker(void *unused) {
+  (void)unused;
+  unsigned long seen = 0;
+  pthread_mutex_lock(&pool.mu);
+  for (;;) {
+    while (pool.gen == seen) pthread_cond_wait(&pool.go, &pool.mu);
+    seen = pool.gen;
+    void (*fn)(void *, int, int) = pool.fn;
+    void *ctx = pool.ctx;
+    const int n = pool.n, nchunk = pool.nchunk;
+    ++pool.active;
+    pthread_mutex_unlock(&pool.mu);
+    const int done = pool_chunks(fn, ctx, n, nchunk);
+    pthread_mutex_lock(&pool.mu);
+    --pool.active;
+    pool.left -= done;
+    pthread_cond_broadcast(&pool.idle);
+  }
   return NULL;
+}
+
+/* a forked child has none of the parent's workers */
+static void pool_forked(void) {
+  pool.started = 0;
+  pool.active = 0;
+  pool.left = 0;
+}
+
+static void pool_start(int want) {
+  static int atfork = 0;
+  if (!atfork) {
+    atfork = 1;
+    (void)pthread_atfork(NULL, NULL, pool_forked);
+  }
+  while (pool.started < want) {
+    pthread_t t;
+    pthread_attr_t at;
+    if (pthread_attr_init(&at)) return;
+    (void)pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED);
+    const int rc = pthread_create(&t, &at, pool_worker, NULL);
+    pthread_attr_destroy(&at);
+    if (rc) return; /* fewer workers: the caller takes the chunks nobody takes */
+    ++pool.started;
+  }
 }
 
 static int host_threads(void) {
@@ -63,21 +125,30 @@ static void par_for(int n, int grain, void (*fn)(void *, int, int), void *ctx) {
     if (n > 0) fn(ctx, 0, n);
     return;
   }
-  pthread_t th[FVH_THREADS];
-  par_task task[FVH_THREADS];
-  int started[FVH_THREADS] = {0};
-  for (int k = 0; k < T; ++k) {
-    task[k].fn = fn;
-    task[k].ctx = ctx;
-    task[k].lo = (int)((int64_t)n * k / T);
-    task[k].hi = (int)((int64_t)n * (k + 1) / T);
-  }
-  for (int k = 1; k < T; ++k) started[k] = pthread_create(&th[k], NULL, par_run, &task[k]) == 0;
-  fn(ctx, task[0].lo, task[0].hi);
-  for (int k = 1; k < T; ++k) {
-    if (started[k]) pthread_join(th[k], NULL);
-    else fn(ctx, task[k].lo, task[k].hi);
-  }
+  /* a few chunks per thread, handed out in turn, even out uneven items */
+  int nchunk = n / grain;
+  if (nchunk > 4 * T) nchunk = 4 * T;
+  pthread_mutex_lock(&pool_user);
+  pthread_mutex_lock(&pool.mu);
+  pool_start(T - 1);
+  while (pool.active > 0) pthread_cond_wait(&pool.idle, &pool.mu);
+  pool.fn = fn;
+  pool.ctx = ctx;
+  pool.n = n;
+  pool.nchunk = nchunk;
+  __atomic_store_n(&pool.next, 0, __ATOMIC_RELAXED);
+  pool.left = nchunk;
+  ++pool.gen;
+  ++pool.active;
+  pthread_cond_broadcast(&pool.go);
+  pthread_mutex_unlock(&pool.mu);
+  const int done = pool_chunks(fn, ctx, n, nchunk);
+  pthread_mutex_lock(&pool.mu);
+  --pool.active;
+  pool.left -= done;
+  while (pool.left > 0) pthread_cond_wait(&pool.idle, &pool.mu);
+  pthread_mutex_unlock(&pool.mu);
+  pthread_mutex_unlock(&pool_user);
 }
 
 struct fvh_forest {
