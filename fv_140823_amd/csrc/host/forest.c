@@ -82,8 +82,13 @@ static void *pool_worker(void *unused) {
   return NULL;
 }
 
-/* a forked child has none of the parent's workers */
+/* A forked child has none of the parent's workers, and the pool's condition variables
+ * still count the parent's waiters: they start afresh. */
 static void pool_forked(void) {
+  pthread_mutex_init(&pool.mu, NULL);
+  pthread_mutex_init(&pool_user, NULL);
+  pthread_cond_init(&pool.go, NULL);
+  pthread_cond_init(&pool.idle, NULL);
   pool.started = 0;
   pool.active = 0;
   pool.left = 0;

@@ -183,3 +183,40 @@ def test_merge_into_parent_that_never_was_a_leaf():
     got = F.get_field(FIELD_SG)[:5]
     assert np.allclose(got, SG[:5].reshape(5, 3, 8, 8).mean(axis=3), rtol=1e-14, atol=0)
     F.close()
+
+
+def test_adaptation_in_a_forked_child():
+    """The host's worker threads do not survive a fork: a child that adapts a forest large
+    enough for several threads (after the parent has started its workers) must finish
+    with its own, and agree with the parent."""
+    import os
+    import time
+    from fv_140823_amd import configs
+
+    def run():
+        F, _, meta = configs.blast(5, 2)
+        xc, lev = F.leaf_info()
+        r = np.sqrt(((xc - 0.5) ** 2).sum(axis=1))
+        f = np.where((np.abs(r - 0.2) < 0.05) & (lev < meta["max_level"]), 1,
+                     np.where(r < 0.1, -1, 0)).astype(np.int8)
+        return F.adapt(f, meta["max_level"])
+
+    n_parent = run()
+    pid = os.fork()
+    if pid == 0:
+        code = 1
+        try:
+            code = 0 if run() == n_parent else 2
+        finally:
+            os._exit(code)
+    t0 = time.time()
+    while True:
+        done, status = os.waitpid(pid, os.WNOHANG)
+        if done:
+            break
+        if time.time() - t0 > 120:
+            os.kill(pid, 9)
+            os.waitpid(pid, 0)
+            raise AssertionError("the forked child did not finish its adaptation")
+        time.sleep(0.05)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0, status
