@@ -703,10 +703,12 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblk(n0), b1 = nblk(n1), b2 = nblk(n2);
   const int r0 = c->nreg[0], r1 = c->nreg[1], r2 = c->nreg[2];
-  const int nc = nblk(c->N);
+  /* grids padded to a multiple of the 8 XCDs, so that logical_block() gives each XCD a
+   * contiguous range (the kernels' bounds checks drop the padding) */
+  const int nc = (int)round_up(nblk(c->N), 8);
   const bool ns = c->prm.ns != 0;
   if (ns) fv_launch_grad(c, S); /* the momentum gradients of the stage's input, into c->SGm */
-  dim3 gf(b0 + b1 + b2), b(FV_BLOCK);
+  dim3 gf((int)round_up(b0 + b1 + b2, 8)), b(FV_BLOCK);
   if (c->rsize == 8) {
     const double *SGm = (const double *)c->SGm;
     double *lUL = (double *)a->lamUL, *lUR = (double *)a->lamUR, *lG = (double *)a->lamG;

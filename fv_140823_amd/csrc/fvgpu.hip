@@ -592,6 +592,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   const int lb = logical_block();
   const int t = threadIdx.x;
   const int i0 = lb * FV_BLOCK;
+  /* the grid is padded to a multiple of 8 blocks (gather_grid): the padding exits whole */
+  if (i0 >= M.N) return;
   const int i = i0 + t;
   const int nrow = min(FV_BLOCK, M.N - i0);
   const int nval = nrow * NEQ;
@@ -1283,6 +1285,11 @@ __global__ void k_transpose_state(const double *__restrict__ src, double *__rest
 /* ------------------------------------------------------------------ launches */
 
 static inline int nblocks(int n) { return (n + FV_BLOCK - 1) / FV_BLOCK; }
+/* The gather's grid, padded to a multiple of the 8 XCDs so that logical_block() hands each
+ * XCD one contiguous range of cells (with any other count it keeps the hardware's
+ * round-robin order, and the face fluxes shared by neighbouring blocks are fetched into
+ * several XCDs' L2). */
+static inline int gather_grid(int n) { return (int)round_up(nblocks(n), 8); }
 
 /* Every launcher takes the state it reads (S) explicitly: c->S in the time loop,
  * a segment slot when the adjoint recomputes stage states. */
@@ -1364,7 +1371,7 @@ template <int NEQ, typename R>
 static void launch_gather_t(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first,
                             int irk, double dt) {
   DevMesh M = dev_mesh(c);
-  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  dim3 g(gather_grid(c->N)), b(FV_BLOCK);
   const R *G = (const R *)c->G;
   if (write_rhs && !do_rk)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, sp.S,
@@ -1563,7 +1570,7 @@ void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int ir
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
                            bool first) {
   DevMesh M = dev_mesh(c);
-  dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  dim3 g(gather_grid(c->N)), b(FV_BLOCK);
   if (first)
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
                        (double *)nullptr, irk, dt, c->err, S, (float *)nullptr);
