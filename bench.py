@@ -147,9 +147,11 @@ def mesh_facts(F):
     Fb = int(((A["face_meta"] >> 6) & 3 != 0).sum())
     _, lev = F.leaf_info()
     rnd = lambda n: (n + 255) // 256 * 256
+    # the gather's grid is padded to a multiple of 8 blocks of 256 (one range per XCD)
+    rnd8 = lambda n: ((n + 255) // 256 + 7) // 8 * 8 * 256
     return dict(N=fl.ncell, F=fl.nface, Q=int(fl.mesh.nquad), Fb=Fb,
                 levels={int(l): int(c) for l, c in zip(*np.unique(lev, return_counts=True))},
-                grid_face=sum(rnd(int(n)) for n in fl.mesh.nface_axis), grid_cell=rnd(fl.ncell))
+                grid_face=sum(rnd(int(n)) for n in fl.mesh.nface_axis), grid_cell=rnd8(fl.ncell))
 
 
 def time_form(F, p, steps, warmup, dt):
