@@ -15,12 +15,14 @@
  * Periodic pairing of opposite sides is an extension (the reference has none).
  */
 #define _POSIX_C_SOURCE 200809L
+#define _DEFAULT_SOURCE
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 #include <unistd.h>
 
 #include "fvgpu.h"
@@ -175,14 +177,22 @@ struct fvh_forest {
   /* leaves in list order */
   BRANCH **leaves;
   int nleaves;
+  /* branch blocks: carved from large mappings (huge pages where the kernel gives them:
+   * a split's new blocks then cost few page faults), freed ones reused */
+  size_t bbytes;
+  char **slabs;
+  int nslab, cslab;
+  size_t slab_used;
+  void *freelist;
 };
 
 /* ------------------------------------------------------------------ allocation */
 
 static size_t align8(size_t x) { return (x + 7) & ~(size_t)7; }
 
-/* One block per branch: BRANCH + neighbour tables + CELL + LEAF arrays. */
-static BRANCH *branch_alloc(int neq) {
+/* One block per branch: BRANCH + neighbour tables + CELL + LEAF arrays, laid out over
+ * blk (zeroed, branch_bytes(neq) long); with blk null only the size is returned. */
+static BRANCH *branch_layout(char *blk, int neq, size_t *bytes) {
   size_t sz = 0;
   size_t o_br = sz; sz = align8(sz + sizeof(BRANCH));
   size_t o_ntr = sz; sz = align8(sz + 6 * sizeof(BRANCH **));
@@ -202,7 +212,8 @@ static BRANCH *branch_alloc(int neq) {
   size_t o_sg = sz; sz = align8(sz + (size_t)(neq + 1) * sizeof(double *));
   size_t o_sg_rows = sz; sz = align8(sz + (size_t)(neq + 1) * 3 * sizeof(double));
   size_t o_ff = sz; sz = align8(sz + 6 * sizeof(int));
-  char *blk = (char *)calloc(1, sz);
+  sz = (sz + 63) & ~(size_t)63;
+  if (bytes) *bytes = sz;
   if (!blk) return NULL;
   BRANCH *b = (BRANCH *)(blk + o_br);
   b->neigtr = (BRANCH ***)(blk + o_ntr);
@@ -239,6 +250,47 @@ static BRANCH *branch_alloc(int neq) {
   b->nsplit = 0;
   b->fv_index = -1;
   return b;
+}
+
+#define FVH_SLAB ((size_t)64 << 20)
+
+/* a block for one branch, not yet zeroed (sequential callers only) */
+static char *block_take(struct fvh_forest *fr) {
+  if (fr->freelist) {
+    char *b = (char *)fr->freelist;
+    fr->freelist = *(void **)b;
+    return b;
+  }
+  if (!fr->nslab || fr->slab_used + fr->bbytes > FVH_SLAB) {
+    if (fr->nslab == fr->cslab) {
+      int c = fr->cslab ? 2 * fr->cslab : 8;
+      char **q = (char **)realloc(fr->slabs, (size_t)c * sizeof(char *));
+      if (!q) return NULL;
+      fr->slabs = q;
+      fr->cslab = c;
+    }
+    void *m = mmap(NULL, FVH_SLAB, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (m == MAP_FAILED) return NULL;
+#ifdef MADV_HUGEPAGE
+    (void)madvise(m, FVH_SLAB, MADV_HUGEPAGE);
+#endif
+    fr->slabs[fr->nslab++] = (char *)m;
+    fr->slab_used = 0;
+  }
+  char *b = fr->slabs[fr->nslab - 1] + fr->slab_used;
+  fr->slab_used += fr->bbytes;
+  return b;
+}
+
+static void block_give(struct fvh_forest *fr, void *b) {
+  *(void **)b = fr->freelist;
+  fr->freelist = b;
+}
+
+/* a zeroed branch over block blk (touches only blk: may run on any thread) */
+static BRANCH *branch_make(const struct fvh_forest *fr, char *blk) {
+  memset(blk, 0, fr->bbytes);
+  return branch_layout(blk, fr->neq, NULL);
 }
 
 /* ------------------------------------------------------------------ hash */
@@ -433,10 +485,9 @@ static int64_t pow10_i64(int e) {
   return r;
 }
 
-static BRANCH *new_branch(const struct fvh_forest *fr, BRANCH *prnt, int level, const int ijk[3],
-                          int root) {
-  BRANCH *b = branch_alloc(fr->neq);
-  if (!b) return NULL;
+static BRANCH *new_branch(const struct fvh_forest *fr, char *blk, BRANCH *prnt, int level,
+                          const int ijk[3], int root) {
+  BRANCH *b = branch_make(fr, blk);
   b->level = level;
   b->prnt = prnt;
   b->root = root;
@@ -451,21 +502,14 @@ static BRANCH *new_branch(const struct fvh_forest *fr, BRANCH *prnt, int level, 
   return b;
 }
 
-/* The 8 kids of leaf b, allocated and initialised but not yet in the tree or the hash
- * (b->kids is filled, b->nkids stays 0).  Touches only b and the new blocks, so
- * different leaves may be done on different threads.  On failure nothing is left. */
-static int split_kids(const struct fvh_forest *fr, BRANCH *b) {
+/* The 8 kids of leaf b over the blocks blk[0..8), initialised but not yet in the tree
+ * or the hash (b->kids is filled, b->nkids stays 0).  Touches only b and the blocks, so
+ * different leaves may be done on different threads. */
+static void split_kids(const struct fvh_forest *fr, BRANCH *b, char *const *blk) {
   for (int c = 0; c < 8; ++c) {
     int ijk[3] = {2 * b->fv_ijk[0] + (c & 1), 2 * b->fv_ijk[1] + ((c >> 1) & 1),
                   2 * b->fv_ijk[2] + ((c >> 2) & 1)};
-    BRANCH *k = new_branch(fr, b, b->level + 1, ijk, b->root);
-    if (!k) {
-      for (int q = 0; q < c; ++q) {
-        free(b->kids[q]);
-        b->kids[q] = NULL;
-      }
-      return FVG_ERR_NOMEM;
-    }
+    BRANCH *k = new_branch(fr, blk[c], b, b->level + 1, ijk, b->root);
     for (int v = 0; v < fr->neq; ++v) {
       k->el->S[v] = b->el->S[v];
       k->el->SG[v][0] = b->el->SG[v][0];
@@ -478,6 +522,15 @@ static int split_kids(const struct fvh_forest *fr, BRANCH *b) {
     k->tag = (t >= 0 && t <= INT32_MAX) ? (int)t : -1;
     b->kids[c] = k;
   }
+}
+
+/* 8 blocks for one split; none taken on failure */
+static int take8(struct fvh_forest *fr, char **blk) {
+  for (int c = 0; c < 8; ++c)
+    if (!(blk[c] = block_take(fr))) {
+      while (c-- > 0) block_give(fr, blk[c]);
+      return FVG_ERR_NOMEM;
+    }
   return 0;
 }
 
@@ -488,7 +541,7 @@ static int split_link(struct fvh_forest *fr, BRANCH *b) {
     if (rc) {
       for (int q = 0; q < c; ++q) ht_remove(fr, b->kids[q]);
       for (int q = 0; q < 8; ++q) {
-        free(b->kids[q]);
+        block_give(fr, b->kids[q]);
         b->kids[q] = NULL;
       }
       return rc;
@@ -502,13 +555,18 @@ static int split_link(struct fvh_forest *fr, BRANCH *b) {
 /* a leaf's kids may already have been made (split_kids) and wait to be linked */
 static int split_branch(struct fvh_forest *fr, BRANCH *b) {
   if (b->nkids) return 0;
-  int rc = b->kids[0] ? 0 : split_kids(fr, b);
-  return rc ? rc : split_link(fr, b);
+  if (!b->kids[0]) {
+    char *blk[8];
+    int rc = take8(fr, blk);
+    if (rc) return rc;
+    split_kids(fr, b, blk);
+  }
+  return split_link(fr, b);
 }
 
-static void free_subtree(BRANCH *b) {
-  for (int c = 0; c < b->nkids; ++c) free_subtree(b->kids[c]);
-  free(b);
+static void free_subtree(struct fvh_forest *fr, BRANCH *b) {
+  for (int c = 0; c < b->nkids; ++c) free_subtree(fr, b->kids[c]);
+  block_give(fr, b);
 }
 
 /* merge.c:85-107: the parent's volume is computed before the kids' gradients are
@@ -533,7 +591,7 @@ static void merge_average(const struct fvh_forest *fr, BRANCH *b) {
 static void merge_unlink(struct fvh_forest *fr, BRANCH *b) {
   for (int c = 0; c < 8; ++c) {
     ht_remove(fr, b->kids[c]);
-    free_subtree(b->kids[c]);
+    free_subtree(fr, b->kids[c]);
     b->kids[c] = NULL;
   }
   b->nkids = 0;
@@ -850,8 +908,10 @@ int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3]
   fr->con.neq_temp = neq;
   fr->con.nprimitiv = neq;
   fr->con.level = level;
-  fr->glob = branch_alloc(neq);
-  if (!fr->glob) { free(fr); return FVG_ERR_NOMEM; }
+  branch_layout(NULL, neq, &fr->bbytes);
+  char *gb = block_take(fr);
+  if (!gb) { fvh_forest_destroy(fr); return FVG_ERR_NOMEM; }
+  fr->glob = branch_make(fr, gb);
   fr->glob->level = 0;
   fr->glob->root = -1;
   fr->glob->prnt = fr->glob;
@@ -865,8 +925,9 @@ int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3]
     for (int j = 0; j < nroot[1]; ++j)
       for (int i = 0; i < nroot[0]; ++i) {
         int ijk[3] = {i, j, k};
-        BRANCH *b = new_branch(fr, fr->glob, 1, ijk, r);
-        if (!b) { fvh_forest_destroy(fr); return FVG_ERR_NOMEM; }
+        char *m = block_take(fr);
+        if (!m) { fvh_forest_destroy(fr); return FVG_ERR_NOMEM; }
+        BRANCH *b = new_branch(fr, m, fr->glob, 1, ijk, r);
         fr->roots[r++] = b;
         int rc = ht_insert(fr, b);
         if (rc) { fvh_forest_destroy(fr); return rc; }
@@ -890,10 +951,10 @@ int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3]
 
 void fvh_forest_destroy(fvh_forest *fr) {
   if (!fr) return;
-  for (int r = 0; r < fr->nroots; ++r)
-    if (fr->roots[r]) free_subtree(fr->roots[r]);
+  /* every branch lives in the mappings */
+  for (int k = 0; k < fr->nslab; ++k) munmap(fr->slabs[k], FVH_SLAB);
+  free(fr->slabs);
   free(fr->roots);
-  free(fr->glob);
   free(fr->ht);
   free(fr->hk);
   free(fr->leaves);
@@ -1061,15 +1122,12 @@ static int split_queue(struct fvh_forest *fr, BRANCH *b, bvec *chg, bvec *work) 
 typedef struct {
   const struct fvh_forest *fr;
   BRANCH *const *leaves;
-  int rc;
+  char *const *blk; /* 8 per leaf */
 } kids_ctx;
 
 static void kids_part(void *v, int lo, int hi) {
   kids_ctx *k = (kids_ctx *)v;
-  for (int q = lo; q < hi; ++q) {
-    int rc = split_kids(k->fr, k->leaves[q]);
-    if (rc) __atomic_store_n(&k->rc, rc, __ATOMIC_RELAXED);
-  }
+  for (int q = lo; q < hi; ++q) split_kids(k->fr, k->leaves[q], k->blk + 8 * (size_t)q);
 }
 
 /* the per-leaf and per-candidate passes of an adaptation, each on the thread pool: every
@@ -1153,15 +1211,26 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
     size_t q = 0;
     for (int i = 0; i < N; ++i)
       if (lv[i]->split) sp[q++] = lv[i];
-    kids_ctx kc = {fr, sp, 0};
-    par_for((int)nflag, 256, kids_part, &kc);
-    rc = kc.rc;
+    /* the blocks are taken here, in order; the threads only fill them */
+    char **blk = (char **)malloc((8 * nflag + 1) * sizeof(char *));
+    size_t nb = 0;
+    if (!blk) rc = FVG_ERR_NOMEM;
+    for (; nb < 8 * nflag && !rc; ++nb)
+      if (!(blk[nb] = block_take(fr))) rc = FVG_ERR_NOMEM;
+    if (rc) {
+      for (size_t k = 0; blk && k + 1 < nb; ++k) block_give(fr, blk[k]);
+      nflag = 0; /* nothing made: nothing to link or drop */
+    } else {
+      kids_ctx kc = {fr, sp, blk};
+      par_for((int)nflag, 256, kids_part, &kc);
+    }
+    free(blk);
     for (size_t k = 0; k < nflag && !rc; ++k) rc = split_queue(fr, sp[k], &chg, &work);
     /* after a failure, kids made but not linked are dropped */
     for (size_t k = 0; k < nflag && rc; ++k)
       if (!sp[k]->nkids)
         for (int c = 0; c < 8; ++c) {
-          free(sp[k]->kids[c]);
+          block_give(fr, sp[k]->kids[c]);
           sp[k]->kids[c] = NULL;
         }
   }
