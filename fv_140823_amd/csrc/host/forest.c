@@ -13,6 +13,9 @@
  *   - split by injection of the parent state (split.c:492-502) and merge by the
  *     arithmetic mean of the kids (merge.c:92-107), with 2:1 face balance.
  * Periodic pairing of opposite sides is an extension (the reference has none).
+ * Adaptation is incremental (a (level, i, j, k) hash kept current, a worklist for the
+ * 2:1 balance, only changed leaves refreshed); its per-leaf passes run on a pool of
+ * host threads, and the branch blocks are carved from large mappings and reused.
  */
 #define _POSIX_C_SOURCE 200809L
 #define _DEFAULT_SOURCE
