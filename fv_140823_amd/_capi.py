@@ -23,7 +23,7 @@ FVG_ERR_IO = -7
 
 FVG_PLAIN, FVG_MIXED = 0, 1
 FVG_MINMOD, FVG_VANALBADA = 0, 1
-FIELD_S, FIELD_SN, FIELD_RHS, FIELD_SF, FIELD_SG, FIELD_FACEFLUX = range(6)
+FIELD_S, FIELD_SN, FIELD_RHS, FIELD_SF, FIELD_SG, FIELD_FACEFLUX, FIELD_STAGE_IN = range(7)
 
 ERR_NAMES = {
     FVG_ERR_ARG: "bad argument",

@@ -10,7 +10,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _capi
-from ._capi import (FIELD_FACEFLUX, FIELD_RHS, FIELD_S, FIELD_SF, FIELD_SG, FIELD_SN, FVG_MINMOD,
+from ._capi import (FIELD_FACEFLUX, FIELD_RHS, FIELD_S, FIELD_SF, FIELD_SG, FIELD_SN, FIELD_STAGE_IN, FVG_MINMOD,
                     FVG_MIXED, FVG_PLAIN, FVG_VANALBADA, FvgParams, check, lib)
 
 # reference bc order of the six sides (inputset.c:99-104)

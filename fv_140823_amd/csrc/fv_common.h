@@ -341,6 +341,9 @@ struct fvg_ctx {
    * Sx and swaps the three pointers at the end of each step (so S and SN keep
    * their meaning, but device addresses taken from them change) */
   double *Sx = nullptr;
+  /* after fvg_steps: the buffer holding the last stage's input (Sx, or SN for a
+   * one-stage method); null once anything else writes the state buffers */
+  double *stage_in = nullptr;
   void *SF = nullptr, *SGm = nullptr, *G = nullptr;
   int *err = nullptr;
   double *red = nullptr;

@@ -848,6 +848,7 @@ int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   HIPCHK(c, hipMemcpyAsync(c->S, seg_state(a, nck - 1, nsteps - nlast, 0), pl,
                            hipMemcpyDeviceToDevice, c->stream));
   c->sf_valid = false;
+  c->stage_in = nullptr;
   std::vector<double> js(2 * nsteps);
   HIPCHK(c, hipMemcpyAsync(js.data(), a->Jsteps, js.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -956,6 +956,7 @@ static void free_mesh(fvg_ctx *c) {
   c->SF = c->SGm = c->G = nullptr;
   c->Sf = nullptr;
   c->Sx = nullptr;
+  c->stage_in = nullptr;
   c->keys = c->hkeys = nullptr;
   c->hval = nullptr;
   c->hbits = 0;
@@ -1007,6 +1008,7 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, fv_dmalloc(c, &c->Sx, (size_t)neq * ns * 8, slack));
   if (rs == 4 && c->prm.model == 1) HIPCHK(c, fv_dmalloc(c, &c->Sf, (size_t)neq * ns * 4, slack));
   c->sf_valid = false;
+  c->stage_in = nullptr;
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemsetAsync(c->S, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SN, 0, (size_t)neq * ns * 8, st));
@@ -1208,6 +1210,7 @@ static size_t field_rows(const fvg_ctx *c, int which, int *len, void **dptr, boo
     case FVG_FIELD_SF: *dptr = c->SF; *len = c->ns; *isR = true; return 6 * c->neq;
     case FVG_FIELD_SG: *dptr = c->SG; *len = c->ns; return 3 * (c->neq + 1);
     case FVG_FIELD_FACEFLUX: *dptr = c->G; *len = c->nfp; *isR = true; return c->neq;
+    case FVG_FIELD_STAGE_IN: *dptr = c->stage_in; *len = c->ns; return c->neq;
     default: return 0;
   }
 }
@@ -1215,7 +1218,7 @@ static size_t field_rows(const fvg_ctx *c, int which, int *len, void **dptr, boo
 /* S, SN, RHS and the face flux are stored cell- (face-) major on the device */
 static bool cell_major(int which) {
   return which == FVG_FIELD_S || which == FVG_FIELD_SN || which == FVG_FIELD_RHS ||
-         which == FVG_FIELD_FACEFLUX;
+         which == FVG_FIELD_FACEFLUX || which == FVG_FIELD_STAGE_IN;
 }
 
 static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
@@ -1226,9 +1229,14 @@ static int copy_field(fvg_ctx *c, int which, double *host, bool to_dev) {
   bool isR;
   size_t rows = field_rows(c, which, &len, &d, &isR);
   if (!rows) return set_err(c, FVG_ERR_ARG, "unknown field");
+  if (which == FVG_FIELD_STAGE_IN) {
+    if (to_dev) return set_err(c, FVG_ERR_ARG, "the last stage's input is read only");
+    if (!d) return set_err(c, FVG_ERR_STATE, "no step has run since the state was last written");
+  }
   const int n = which == FVG_FIELD_FACEFLUX ? c->F : c->N;
   (void)hipSetDevice(c->device);
   if (to_dev && which == FVG_FIELD_S) c->sf_valid = false;
+  if (to_dev) c->stage_in = nullptr;
   const size_t es = isR ? c->rsize : 8;
   if (cell_major(which)) {
     /* host var-major [rows][n] <-> device [n][rows] */
@@ -1539,6 +1547,9 @@ static void time_step(fvg_ctx *c, int method, double dt, bool write_rhs_last, hi
   c->SN = X0;
   c->S = buf[(n - 1) & 1];
   c->Sx = buf[n & 1];
+  /* the last stage read S^n itself (one stage) or the previous stage's output, which
+   * the rotation leaves in Sx */
+  c->stage_in = n == 1 ? c->SN : c->Sx;
   c->sf_valid = fast; /* the last gather wrote the FP32 copy of the new S */
 }
 
@@ -1560,11 +1571,13 @@ void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs) {
   stage(c, StagePtrs{c->S, c->SN, c->S, nullptr}, irk, dt, first, write_rhs, nullptr, nullptr,
         nullptr, false);
   c->sf_valid = false;
+  c->stage_in = nullptr;
 }
 
 void fv_stage_ptrs(fvg_ctx *c, const double *S, double *SN, double *Sout, int irk, double dt) {
   stage(c, StagePtrs{S, SN, Sout, nullptr}, irk, dt, false, false, nullptr, nullptr, nullptr, false);
   if (Sout == c->S) c->sf_valid = false;
+  c->stage_in = nullptr;
 }
 
 void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, int irk, double dt,
@@ -1593,6 +1606,7 @@ static int state_device_copy(fvg_ctx *c, void *d, bool to_ctx) {
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
   (void)hipSetDevice(c->device);
   if (to_ctx) c->sf_valid = false;
+  if (to_ctx) c->stage_in = nullptr;
   if (to_ctx)
     hipLaunchKernelGGL(k_transpose_state, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
                        (const double *)d, c->S, c->N, c->neq, 1);
@@ -1698,6 +1712,7 @@ int fvg_rk(fvg_ctx *c, int irk, double dt) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   c->sf_valid = false;
+  c->stage_in = nullptr;
   if (c->neq == 5)
     hipLaunchKernelGGL(k_rk<5>, g, b, 0, c->stream, M, c->S, c->SN, c->RHS, irk, dt);
   else
@@ -1757,6 +1772,7 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     constexpr int kReps = 10;
     const bool fast = fast_mixed(c);
     if (fast) ensure_sf(c);
+    c->stage_in = nullptr; /* the batches below overwrite the spare buffer */
     bool ev_fail = false;
     auto batch = [&](auto &&launch) -> double {
       float ms = 0.f;

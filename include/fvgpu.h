@@ -50,7 +50,11 @@ enum {
   FVG_FIELD_RHS = 2,  /* [neq][ncell] f64, already multiplied by -Area, not divided by Vol */
   FVG_FIELD_SF = 3,   /* [6][neq][ncell] (f64 plain / f32 mixed on device; f64 on copy-out) */
   FVG_FIELD_SG = 4,   /* [neq+1][3][ncell] f64 (gradcalc output) */
-  FVG_FIELD_FACEFLUX = 5 /* [neq][nface] face flux of the owner side (f64 / f32) */
+  FVG_FIELD_FACEFLUX = 5, /* [neq][nface] face flux of the owner side (f64 / f32) */
+  /* [neq][ncell] f64, read only: after fvg_step / fvg_steps, the state the last stage of
+   * the last step read, i.e. the input whose residual FVG_FIELD_RHS holds.  FVG_ERR_STATE
+   * once anything else has written the state since. */
+  FVG_FIELD_STAGE_IN = 6
 };
 
 typedef struct fvg_params {
