@@ -1,16 +1,28 @@
 """Benchmark of the RHS + RK hot path on one MI355X.
 
-Workload (default): the 3D spherical blast with AMR (ideal gas, gamma 1.4, unit
-cube, base 32^3 refined two levels by a density-gradient criterion with 2:1
-balance, re-adapted every 10 steps).  The blast is evolved with host AMR for
---evolve steps (untimed) so the mesh has the size and hanging-face mix of the
-running case; then K SSP-RK3 steps are timed on that mesh with every input
-resident in HBM.  A step = RK(0) + 3 x (faceinterp, RHS, RK(k)); the metric is
-cell-updates/s = leaves x RK stages / second.
+Headline workload (`value`): the compressible Navier-Stokes Taylor-Green vortex on
+a uniform 128^3 periodic octree (2,097,152 leaves; Re 1600, Mach 0.1, MUSCL-minmod
++ HLLC, Green-Gauss gradients and viscous fluxes), the largest single-GPU
+configuration.  K SSP-RK3 steps are timed with every input resident in HBM; a step
+is RK(0) + 3 x (gradcalc, faceinterp, RHS, RK(k)) and the metric is cell-updates/s =
+leaves x RK stages / second.  The plain FP64 form gives `value`; the mixed
+FP32-flux / FP64-accumulate form is timed on the same state and reported beside it
+with its relative L1 error against the plain form after --acc-steps steps.
 
-The plain FP64 form gives `value`; the mixed FP32-flux / FP64-accumulate form is
-timed on the same state and reported beside it with its relative L1 error
-against the plain form after --acc-steps steps.
+Beside the headline, on the same run:
+  roofline      the RK stage's kernels against the HBM roofline: the reference data
+                flow's algorithmic bytes per cell-update (`achieved`, `frac`) and the
+                HBM bytes the kernels really move, from the committed rocprofv3
+                counters (`traffic`, `traffic_frac`), over the kernels' mean launch
+                durations (HIP events on the solver's stream);
+  hbm_copy      the rate of a plain streaming copy kernel (the achievable ceiling);
+  cpu_baseline  the oracle (the FP64 C restatement of the reference path) on the host
+                cores, on a bounded sample of the same mesh;
+  adjoint       the isentropic-vortex rollout (496,896 leaves, 50 steps): adjoint /
+                forward time with checkpoints every 10 steps and with the rollout
+                stored, and the worst relative difference from central differences;
+  extra         the 3D blast with AMR (base 32^3 + 2 levels, evolved 200 steps with
+                adaptation every 10), timed the same way on its final mesh.
 """
 from __future__ import annotations
 
@@ -24,25 +36,27 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["recon", "grad", "face", "gather_rk"]
+METRIC = "cell-updates/s, FP64 plain vs FP32-flux mixed @≤1e-6 rel L1; adjoint/fwd ratio"
 
 
 def _args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="blast", choices=["blast", "sod", "cavitation", "tgv", "vortex"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavitation", "sod", "blast", "vortex"])
+    ap.add_argument("--level", type=int, default=None,
+                    help="uniform level (tgv/cavitation/sod) or base level (blast/vortex)")
+    ap.add_argument("--extra", default="blast", help="comma-separated extra workloads ('' for none)")
     ap.add_argument("--evolve", type=int, default=200, help="untimed AMR steps before timing (blast)")
-    ap.add_argument("--host-flatten", action="store_true",
-                    help="blast evolution: flatten on the host after each adaptation (default: on the device)")
-    ap.add_argument("--level", type=int, default=None, help="uniform level (sod/cavitation/tgv) or base level (blast/vortex)")
     ap.add_argument("--refine", type=float, default=0.01, help="blast: split where h|grad rho|/rho exceeds this")
     ap.add_argument("--acc-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run 5 device copies of MIB MiB (calibrates rocprof byte counters)")
+                    help="only run 6 streaming copies of MIB MiB (calibrates rocprof byte counters)")
     return ap.parse_args()
 
 
@@ -74,61 +88,58 @@ def ref_bytes_per_cell(part, neq, f, w, ns_visc):
 
 
 def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage):
-    """Bytes a launch of this build's fused kernels must move at least: each
-    input read once, each output written once (MUSCL inside the face kernel)."""
-    N, F, Q = m["N"], m["F"], m["Q"]
+    """Bytes a launch of this build's fused kernels must move at least: each input
+    read once, each output written once (MUSCL inside the face kernel).  rs is the
+    byte width of the face flux (8 plain, 4 mixed)."""
+    N, F, Q, Fr = m["N"], m["F"], m["Q"], m["Freg"]
     meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
-    if kernel == "grad":
+    if kernel == "grad":                       # momentum rows of S -> 9 gradients
         return 8 * 3 * N + meta_c + 9 * rs * N
     if kernel == "face":
-        b = 8 * neq * N + meta_c + 12 * F + neq * rs * F
+        # S, mesh, face records (L, R, meta), far stencil cells of regular faces, flux
+        b = 8 * neq * N + meta_c + 12 * F + 8 * Fr + neq * rs * F
         if ns_visc:
             b += 9 * rs * N
         return b
     if kernel == "gather_rk":
         b = 4 * N + 24 * N + neq * rs * F + 8 * neq * N * (1 if first_stage else 2) + 8 * neq * N
-        if first_stage:
-            b += 8 * neq * N
         return b
     raise KeyError(kernel)
 
 
-def build_workload(name, evolve, quiet, level=None, refine=0.01, device_flatten=True):
+def build_workload(name, evolve, quiet, level=None, refine=0.01):
     import fv_140823_amd as fv
     from fv_140823_amd import configs, driver
     if name == "blast":
         F, p, meta = configs.blast(level or 5, 2)
-        g = fv.Solver.from_forest(F, p, device_flatten=device_flatten)
+        g = fv.Solver.from_forest(F, p, device_flatten=True)
         t0 = time.perf_counter()
         base = (level or 5) + 1
         # the density-gradient criterion on the device (the same flags as
         # configs.density_gradient_flags on the downloaded gradients)
         flags = dict(refine=refine, coarsen=refine / 5, base_level=base, kind=0)
         st = driver.evolve(F, g, evolve, meta["cfl"], 3, meta["adapt_every"], flags, meta["max_level"],
-                           log=lambda s: log(s, quiet), device_flatten=device_flatten)
+                           log=lambda s: log(s, quiet), device_flatten=True)
         # final adaptation so the timed mesh matches the state
-        rm = st["remesh"] + [driver.remesh(F, g, flags, meta["max_level"], device_flatten)]
+        rm = st["remesh"] + [driver.remesh(F, g, flags, meta["max_level"], True)]
         per = {k: round(1e3 * sum(r[k] for r in rm) / len(rm), 3)
                for k in ("download_s", "adapt_s", "flatten_upload_s")}
         meta = dict(meta, evolved_steps=evolve, evolved_time=st["time"],
-                    evolve_wall_s=time.perf_counter() - t0, remesh_count=len(rm),
+                    evolve_wall_s=round(time.perf_counter() - t0, 3), remesh_count=len(rm),
                     remesh_ms_mean={"flags": per["download_s"], "host_adapt": per["adapt_s"],
-                                    "flatten": per["flatten_upload_s"]},
-                    flatten="device" if device_flatten else "host")
+                                    "flatten": per["flatten_upload_s"]})
         F.set_state(g.get_state())
         g.close()
         return F, p, meta
     builder = getattr(configs, name)
-    F, p, meta = builder(level) if level else builder()
-    return F, p, meta
+    return builder(level) if level else builder()
 
 
 def measured_traffic(workload, kernel, grid=None):
-    """HBM bytes per launch of `kernel` on `workload` (the workload name, e.g.
-    "blast_32^3+2") from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    profiles/traffic.json, already scaled by the copy-kernel calibration; None if
-    that workload was not profiled or was profiled on a mesh of another size
-    (launch grid `grid`, threads)."""
+    """HBM bytes per launch of `kernel` on `workload` (its name, e.g. "tgv_128^3") from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/traffic.json,
+    already scaled by the copy-kernel calibration; None if that workload was not
+    profiled or was profiled on a mesh of another size (launch grid `grid`, threads)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -140,25 +151,32 @@ def measured_traffic(workload, kernel, grid=None):
     return int(k["read_bytes"] + k["write_bytes"])
 
 
-def mesh_facts(F):
-    import fv_140823_amd as fv
-    fl = fv.FlatMesh(F.run)
-    A = fl.arrays()
-    Fb = int(((A["face_meta"] >> 6) & 3 != 0).sum())
+def mesh_facts(flat, F):
+    A = flat.arrays()
     _, lev = F.leaf_info()
     rnd = lambda n: (n + 255) // 256 * 256
     # the gather's grid is padded to a multiple of 8 blocks of 256 (one range per XCD)
     rnd8 = lambda n: ((n + 255) // 256 + 7) // 8 * 8 * 256
-    return dict(N=fl.ncell, F=fl.nface, Q=int(fl.mesh.nquad), Fb=Fb,
+    return dict(N=flat.ncell, F=flat.nface, Q=int(flat.mesh.nquad),
+                Freg=int(sum(flat.mesh.nface_reg)), Fb=int((((A["face_meta"] >> 6) & 3) != 0).sum()),
                 levels={int(l): int(c) for l, c in zip(*np.unique(lev, return_counts=True))},
-                grid_face=sum(rnd(int(n)) for n in fl.mesh.nface_axis), grid_cell=rnd8(fl.ncell))
+                grid_face=sum(rnd(int(n)) for n in flat.mesh.nface_axis), grid_cell=rnd8(flat.ncell),
+                grid_grad=rnd(flat.ncell))
 
 
-def time_form(F, p, steps, warmup, dt):
-    import torch
-
+def _solver(flat, S0, p):
     import fv_140823_amd as fv
-    g = fv.Solver.from_forest(F, p)
+    g = fv.Solver(p)
+    g.set_mesh(flat)
+    g.set_state(S0)
+    return g
+
+
+def time_form(flat, S0, p, steps, warmup, dt):
+    """Wall time of `steps` steps (after `warmup`), and the mean duration of each stage
+    kernel from HIP events on the solver's stream (fvg_time_steps)."""
+    import torch
+    g = _solver(flat, S0, p)
     if warmup:
         g.steps(warmup, dt, 3)
     g.sync()
@@ -168,21 +186,18 @@ def time_form(F, p, steps, warmup, dt):
     g.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-kernel durations from HIP events on the context's stream, around batches
-    # of back-to-back launches of each stage kernel (fvg_time_steps)
-    ev_total, kms = g.time_steps(max(3, steps // 3), dt, 3)
+    _, kms = g.time_steps(max(3, steps // 3), dt, 3)
     g.close()
     return wall, kms
 
 
-def l1_rel_density(F, p_plain, p_mixed, steps, dt):
+def l1_errors(flat, F, S0, p_plain, p_mixed, steps, dt):
     """Relative L1 (volume-weighted) of density and pressure, mixed vs plain, after
     `steps` steps at the fixed dt."""
-    import fv_140823_amd as fv
     from fv_140823_amd.configs import pressure
     out = {}
     for tag, p in (("plain", p_plain), ("mixed", p_mixed)):
-        g = fv.Solver.from_forest(F, p)
+        g = _solver(flat, S0, p)
         g.steps(steps, dt, 3)
         out[tag] = g.get_state()
         g.close()
@@ -198,27 +213,33 @@ def l1_rel_density(F, p_plain, p_mixed, steps, dt):
 
 
 def _oracle_rate(F, p, dt, budget_s, threads):
-    """cell-updates/s of the oracle's RK stages (faceinterp, [gradcalc], RHS, RK)
-    on `threads` threads, over about budget_s seconds of stages."""
+    """cell-updates/s of the oracle's RK stages ([faceinterp], [gradcalc], RHS, RK) on
+    `threads` threads: one stage to size the sample, then as many stages as fit in
+    about budget_s seconds (at least the one already timed)."""
     import oracle
     oracle.set_threads(threads)
     N = F.nleaves
-    t0 = time.perf_counter()
-    oracle.faceinterp(F, p)
-    oracle.rhs(F, p)
-    one = time.perf_counter() - t0
-    stages = max(3, min(300, int(budget_s / max(one, 1e-4))))
-    oracle.rk(F, 0, dt)
-    t0 = time.perf_counter()
     irk = [1, 2, 3]
-    for k in range(stages):
+
+    def stage(k):
         if p.ischeme == 2:
             oracle.faceinterp(F, p)
         if p.ns:
             oracle.gradcalc(F, p)
         oracle.rhs(F, p)
         oracle.rk(F, irk[k % 3], dt)
-    wall = time.perf_counter() - t0
+
+    oracle.rk(F, 0, dt)
+    t0 = time.perf_counter()
+    stage(0)
+    one = time.perf_counter() - t0
+    extra = min(300, int(budget_s / max(one, 1e-4))) - 1
+    stages, wall = 1, one
+    if extra > 0:
+        t0 = time.perf_counter()
+        for k in range(1, 1 + extra):
+            stage(k)
+        stages, wall = extra, time.perf_counter() - t0
     oracle.set_threads(1)
     return N * stages / wall, stages, wall
 
@@ -229,135 +250,265 @@ def cpu_baseline(F, p, dt, budget_s, quiet):
     like the reference's one MPI rank per core) and, beside it, one core."""
     N = F.nleaves
     S0 = F.get_state()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     threads = max(1, min(threads, 64))
-    v1, st1, w1 = _oracle_rate(F, p, dt, budget_s / 2, 1)
+    vt, stt, wt = _oracle_rate(F, p, dt, budget_s * 0.6, threads)
     F.set_state(S0)
-    vt, stt, wt = _oracle_rate(F, p, dt, budget_s / 2, threads)
+    v1, st1, w1 = _oracle_rate(F, p, dt, budget_s * 0.4, 1)
     F.set_state(S0)
-    log(f"cpu oracle: 1 thread {st1} stages in {w1:.2f}s; {threads} threads {stt} stages in {wt:.2f}s", quiet)
-    return dict(value=vt, unit="cell-updates/s", cores=threads, kind="port",
-                value_1core=v1,
-                sample=f"{stt} RK stages (faceinterp+RHS+RK) of the oracle on the same {N}-leaf mesh on "
-                       f"{threads} threads ({st1} stages on 1 thread for value_1core)")
+    log(f"cpu oracle: {threads} threads {stt} stages in {wt:.2f}s; 1 thread {st1} stages in {w1:.2f}s", quiet)
+    return dict(value=round(vt, 1), unit="cell-updates/s", cores=threads, kind="port",
+                value_1core=round(v1, 1),
+                sample=f"{stt} RK stages ([faceinterp], [gradcalc], RHS, RK) of the oracle on the same "
+                       f"{N}-leaf mesh on {threads} threads, {wt:.1f} s ({st1} stages on 1 thread, "
+                       f"{w1:.1f} s, for value_1core)")
+
+
+def hbm_copy(quiet, mib=2048, reps=10):
+    """Rate of the library's streaming copy kernel over `mib` MiB (read + written)."""
+    import ctypes as C
+
+    import torch
+
+    import fv_140823_amd as fv
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ms = C.c_double()
+    rc = fv.lib().fvg_stream_copy(torch.cuda.current_device(), C.c_void_p(y.data_ptr()),
+                                  C.c_void_p(x.data_ptr()), n * 8, reps, C.byref(ms))
+    if rc:
+        raise RuntimeError(f"fvg_stream_copy returned {rc}")
+    ok = bool(torch.equal(x, y))
+    del x, y
+    gbs = 2 * n * 8 / (ms.value * 1e-3) / 1e9
+    log(f"stream copy {mib} MiB: {ms.value:.3f} ms, {gbs:.0f} GB/s", quiet)
+    return dict(bytes_per_copy=2 * n * 8, ms=round(ms.value, 4), gbs=round(gbs, 1),
+                frac=round(gbs / HBM_PEAK_GBS, 4), copy_exact=ok)
+
+
+def time_workload(F, p, meta, a, quiet, with_kernels=True):
+    """Plain and mixed timings, per-kernel durations, roofline and mixed-vs-plain error
+    of one workload."""
+    import fv_140823_amd as fv
+    flat = fv.FlatMesh(F.run)
+    mf = mesh_facts(flat, F)
+    log(f"{meta['name']}: {mf['N']} leaves, {mf['F']} faces, {mf['Q']} hanging quads", quiet)
+    neq = fv.neq_of(p.model)
+    S0 = F.get_state()
+    g = _solver(flat, S0, p)
+    dt = g.cfl_dt(meta["cfl"])
+    g.close()
+    pm = type(p).from_buffer_copy(p)
+    pm.precision = fv.FVG_MIXED
+    log(f"timing plain FP64: {a.steps} steps after {a.warmup} warm-up", quiet)
+    wall_p, k_p = time_form(flat, S0, p, a.steps, a.warmup, dt)
+    log("timing mixed", quiet)
+    wall_m, k_m = time_form(flat, S0, pm, a.steps, a.warmup, dt)
+    cu = mf["N"] * 3 * a.steps
+    val_p, val_m = cu / wall_p, cu / wall_m
+    log(f"plain {val_p:.4g} cell-upd/s, mixed {val_m:.4g} cell-upd/s", quiet)
+    acc = l1_errors(flat, F, S0, p, pm, a.acc_steps, dt)
+    log(f"mixed vs plain after {a.acc_steps} steps: {acc}", quiet)
+    fpc = mf["F"] / mf["N"]
+    fname = "k_face_fused" if p.ischeme == 2 else "k_face"
+    stage_kernels = ["grad", "face", "gather_rk"] if p.ns else ["face", "gather_rk"]
+    names = {"grad": ("k_grad", mf["grid_grad"]), "face": (fname, mf["grid_face"]),
+             "gather_rk": ("k_gather_rk", mf["grid_cell"])}
+    # the roofline of one RK stage of the plain form: its launches' mean durations
+    t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)
+    B = ref_bytes_per_cell("stage", neq, fpc, 8, p.ns)
+    achieved = B * mf["N"] / (t_stage * 1e-3) / 1e9
+    tr = [measured_traffic(meta["name"], *names[k]) for k in stage_kernels]
+    traffic = sum(tr) if all(t is not None for t in tr) else None
+    comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False) for k in stage_kernels)
+    roof = dict(bound="hbm", kernel="RK stage: " + " + ".join(names[k][0] for k in stage_kernels),
+                achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_gbs=round(traffic / (t_stage * 1e-3) / 1e9, 1) if traffic else None,
+                traffic_frac=round(traffic / (t_stage * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                alg_bytes_per_launch=int(B * mf["N"]), alg_bytes_per_cell_update=round(B, 1),
+                avg_launch_ms=round(t_stage, 5), compulsory_bytes_per_launch=int(comp),
+                compulsory_frac=round(comp / (t_stage * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                faces_per_cell=round(fpc, 4))
+    stage_roof = {}
+    for tag, val, w in (("plain", val_p, 8), ("mixed", val_m, 4)):
+        Bw = ref_bytes_per_cell("stage", neq, fpc, w, p.ns)
+        stage_roof[tag] = dict(bytes_per_cell_update=round(Bw, 1), achieved_gbs=round(Bw * val / 1e9, 1),
+                               frac=round(Bw * val / 1e9 / HBM_PEAK_GBS, 4))
+    kern = {}
+    for i, k in enumerate(KERNELS):
+        if k_p[i] <= 0 or k not in names:
+            continue
+        e = dict(name=names[k][0], plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
+        tb = measured_traffic(meta["name"], *names[k])
+        if tb is not None:
+            e["plain_traffic_bytes"] = tb
+            e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
+        for tag, ms, rs in (("plain", k_p[i], 8), ("mixed", k_m[i], 4)):
+            if ms:
+                cb = compulsory_bytes(k, mf, neq, rs, p.ns, False)
+                e[f"{tag}_compulsory_bytes"] = int(cb)
+                e[f"{tag}_compulsory_gbs"] = round(cb / (ms * 1e-3) / 1e9, 1)
+        kern[k] = e
+    cfg = {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"], "hanging_quads": mf["Q"],
+           "regular_faces": mf["Freg"], "levels": mf["levels"],
+           "model": {1: f"ideal gas gamma {p.gamma:g}", 2: "barotropic (model 2)",
+                     3: "two-phase barotropic (model 3)"}[p.model] + (" Navier-Stokes" if p.ns else ""),
+           "scheme": ("MUSCL-" + ("minmod" if p.limiter == 0 else "van Albada") if p.ischeme == 2
+                      else "first order") + " + HLLC(Davis), SSP-RK3", "dt": dt,
+           **{k: v for k, v in meta.items() if k != "name"}}
+    if p.ns:
+        cfg.update(mu=p.mu, viscous_work=bool(p.visc_work))
+    out = dict(value=round(val_p, 1), ms_per_step=round(wall_p / a.steps * 1e3, 4), config=cfg,
+               mixed={"value": round(val_m, 1), "ms_per_step": round(wall_m / a.steps * 1e3, 4),
+                      "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},
+               roofline=roof, stage_roofline=stage_roof)
+    if with_kernels:
+        out["kernels"] = kern
+    flat.close()
+    return out, dt
+
+
+def adjoint_block(quiet):
+    """The isentropic-vortex rollout: 64^3 base + two-level patch (496,896 leaves),
+    van Albada MUSCL, 50 SSP-RK3 steps, J = time-integrated probe pressure.  Forward =
+    the plain rollout (fvg_steps); adjoint = the reverse sweep (fvg_adj_backward into
+    HBM, recomputing checkpointed segments), after a forward that keeps checkpoints
+    every 10 steps, or every stage state (spacing chosen from free memory)."""
+    import torch
+
+    import fv_140823_amd as fv
+    from fv_140823_amd import configs
+    F, p, meta = configs.vortex()
+    xc, _ = F.leaf_info()
+    probe = ((np.abs(xc[:, 0] - 6.5) < 1.0) & (np.abs(xc[:, 1] - 6.5) < 1.0)).astype(np.uint8)
+    g = fv.Solver.from_forest(F, p)
+    g.adj_set_probe(probe)
+    S0 = F.get_state()
+    n = meta["steps"]
+    dt = g.cfl_dt(meta["cfl"])
+    g.adj_forward(2, 1, dt)  # warm-up
+    g.set_state(S0)
+    g.steps(2, dt, 3)
+    fwd = []
+    for _ in range(3):
+        g.set_state(S0)
+        g.sync()
+        t0 = time.perf_counter()
+        g.steps(n, dt, 3)
+        g.sync()
+        fwd.append(time.perf_counter() - t0)
+    t_fwd = min(fwd)
+    lam_dev = torch.empty((g.neq, g.ncell), dtype=torch.float64, device="cuda")
+    res, grads = {}, {}
+    for ck, key in ((meta["ckpt"], "ckpt10"), (0, "store")):
+        best = None
+        for _ in range(2):
+            g.set_state(S0)
+            g.sync()
+            t0 = time.perf_counter()
+            g.adj_forward(n, ck, dt)
+            t_f = time.perf_counter() - t0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.adj_backward_device(lam_dev.data_ptr())
+            t_b = time.perf_counter() - t0
+            best = (t_f, t_b) if best is None or t_b < best[1] else best
+        grads[key] = lam_dev.cpu().numpy()
+        res[key] = dict(spacing=g.adj_checkpoint_spacing, forward_with_checkpoints_ms=round(best[0] * 1e3, 3),
+                        adjoint_ms=round(best[1] * 1e3, 3), ratio=round(best[1] / t_fwd, 3))
+    lam = grads["ckpt10"]
+    eps = 1e-6
+    worst = 0.0
+    for k in range(3):
+        r = np.random.default_rng(500 + k)
+        c = np.array([r.uniform(5.5, 7.5), r.uniform(5.5, 7.5), r.uniform(0.0, 10.0)])
+        w = r.uniform(0.5, 1.5)
+        b = np.exp(-((xc - c) ** 2).sum(axis=1) / w ** 2)
+        v = np.zeros(S0.shape)
+        v[:4] = b * (r.choice([-1.0, 1.0], 4) * r.uniform(0.5, 1.5, 4))[:, None]
+        Js = []
+        for sgn in (1.0, -1.0):
+            g.set_state(S0 + sgn * eps * v)
+            g.adj_forward(n, meta["ckpt"], dt)
+            Js.append(g.adj_objective())
+        (ph, pl), (mh, ml) = Js
+        fd = ((ph - mh) + (pl - ml)) / (2 * eps)
+        worst = max(worst, abs(float((lam * v).sum()) - fd) / abs(fd))
+    g.close()
+    out = dict(workload=meta["name"], leaves=F.nleaves, steps=n, dt=dt, forward_ms=round(t_fwd * 1e3, 3),
+               ratio_ckpt10=res["ckpt10"]["ratio"], ratio_store=res["store"]["ratio"],
+               ckpt10=res["ckpt10"], store=res["store"],
+               store_grad_equal_ckpt10=bool(np.array_equal(grads["store"], grads["ckpt10"])),
+               fd_worst=float(f"{worst:.3e}"), fd_directions=3, fd_eps=eps)
+    log(f"adjoint: {out}", quiet)
+    return out
 
 
 def main():
     a = _args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the path has no "
-              "multi-GPU halo exchange; see the sharding notes in DESIGN.md", file=sys.stderr)
+              "multi-GPU halo exchange; the sharding is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     import torch
 
     import fv_140823_amd as fv
     if a.copy_calibration:
+        import ctypes as C
         n = a.copy_calibration * (1 << 20) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "bytes_read": n * 8,
-                          "bytes_written": n * 8}))
+        ms = C.c_double()
+        rc = fv.lib().fvg_stream_copy(torch.cuda.current_device(), C.c_void_p(y.data_ptr()),
+                                      C.c_void_p(x.data_ptr()), n * 8, 5, C.byref(ms))
+        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "copies": 6, "rc": rc,
+                          "bytes_read": n * 8, "bytes_written": n * 8, "ms": ms.value}))
         return
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    t_start = time.perf_counter()
     log(f"building workload {a.workload}", a.quiet)
-    F, p, meta = build_workload(a.workload, a.evolve, a.quiet, a.level, a.refine, not a.host_flatten)
-    mf = mesh_facts(F)
-    log(f"mesh: {mf}", a.quiet)
-    neq = fv.neq_of(p.model)
-    g = fv.Solver.from_forest(F, p)
-    dt = g.cfl_dt(meta["cfl"])
-    g.close()
-    pm = type(p).from_buffer_copy(p)
-    pm.precision = fv.FVG_MIXED
-    log(f"timing plain FP64: {a.steps} steps after {a.warmup} warm-up", a.quiet)
-    wall_p, k_p = time_form(F, p, a.steps, a.warmup, dt)
-    log(f"timing mixed: {a.steps} steps", a.quiet)
-    wall_m, k_m = time_form(F, pm, a.steps, a.warmup, dt)
-    stages = 3
-    cu = mf["N"] * stages * a.steps
-    val_p, val_m = cu / wall_p, cu / wall_m
-    log(f"plain {val_p:.4g} cell-upd/s, mixed {val_m:.4g} cell-upd/s", a.quiet)
-    # Roofline of one RK stage of the plain form (its two launches, face + gather)
-    # against the reference data flow's bytes per cell-update (B, ref_bytes_per_cell):
-    # achieved = B x leaves / (mean face + mean gather duration, HIP events);
-    # traffic = the two launches' HBM bytes from the committed rocprof counters.
-    fpc = mf["F"] / mf["N"]
-    stage_kernels = ["grad", "face", "gather_rk"] if p.ns else ["face", "gather_rk"]
-    t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)
-    B = ref_bytes_per_cell("stage", neq, fpc, 8, p.ns)
-    achieved = B * mf["N"] / (t_stage * 1e-3) / 1e9
-    fname = "k_face_fused" if p.ischeme == 2 else "k_face"
-    tf = measured_traffic(meta["name"], fname, mf["grid_face"])
-    tg = measured_traffic(meta["name"], "k_gather_rk", mf["grid_cell"])
-    traffic = tf + tg if (tf is not None and tg is not None and not p.ns) else None
-    comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False) for k in stage_kernels)
-    roof = dict(bound="hbm", kernel=f"RK stage: {fname} + k_gather_rk" + (" + k_grad" if p.ns else ""),
-                achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                traffic_gbs=round(traffic / (t_stage * 1e-3) / 1e9, 1) if traffic else None,
-                alg_bytes_per_launch=int(B * mf["N"]), alg_bytes_per_cell_update=round(B, 1),
-                avg_launch_ms=round(t_stage, 5), compulsory_bytes_per_launch=int(comp),
-                faces_per_cell=round(fpc, 4))
-    # the same against the wall clock of the timed steps, plain and mixed
-    stage_roof = {}
-    for tag, val, w in (("plain", val_p, 8), ("mixed", val_m, 4)):
-        Bw = ref_bytes_per_cell("stage", neq, fpc, w, p.ns)
-        stage_roof[tag] = dict(bytes_per_cell_update=round(Bw, 1), achieved_gbs=round(Bw * val / 1e9, 1),
-                               frac=round(Bw * val / 1e9 / HBM_PEAK_GBS, 4))
-    # per kernel: mean durations, the actual HBM rate (rocprof bytes / duration) and
-    # the rate on the kernel's compulsory bytes
-    kern = {}
-    for i in range(4):
-        if k_p[i] <= 0:
-            continue
-        k = KERNELS[i]
-        e = dict(plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
-        kn = {"face": (fname, mf["grid_face"]), "gather_rk": ("k_gather_rk", mf["grid_cell"])}.get(k)
-        tb = measured_traffic(meta["name"], *kn) if kn else None
-        if tb is not None:
-            e["plain_traffic_bytes"] = tb
-            e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
-        if k != "recon":
-            for tag, ms, rs in (("plain", k_p[i], 8), ("mixed", k_m[i], 4)):
-                if ms:
-                    e[f"{tag}_compulsory_gbs"] = round(compulsory_bytes(k, mf, neq, rs, p.ns, False) / (ms * 1e-3) / 1e9, 1)
-        kern[k] = e
-    acc = l1_rel_density(F, p, pm, a.acc_steps, dt)
-    log(f"mixed vs plain after {a.acc_steps} steps: {acc}", a.quiet)
+    F, p, meta = build_workload(a.workload, a.evolve, a.quiet, a.level, a.refine)
+    head, dt = time_workload(F, p, meta, a, a.quiet)
+    copy = hbm_copy(a.quiet)
+    extra = {}
+    for name in [x for x in a.extra.split(",") if x and x != a.workload]:
+        log(f"extra workload {name}", a.quiet)
+        Fx, px, mx = build_workload(name, a.evolve, a.quiet, None, a.refine)
+        extra[name], _ = time_workload(Fx, px, mx, a, a.quiet, with_kernels=False)
+        Fx.close()
+    adj = None if a.no_adjoint else adjoint_block(a.quiet)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(F, p, dt, a.cpu_budget, a.quiet)
     res = {
-        "metric": "cell-updates/s (RHS+RK, per RK stage), FP64 plain; FP32-flux mixed beside it",
-        "value": round(val_p, 1),
+        "metric": METRIC,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(wall_p / a.steps * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"],
-                   "hanging_quads": mf["Q"], "levels": mf["levels"],
-                   "model": {1: f"ideal gas gamma {p.gamma:g}", 2: "barotropic (model 2)",
-                             3: "two-phase barotropic (model 3)"}[p.model] + (" Navier-Stokes" if p.ns else ""),
-                   "scheme": ("MUSCL-" + ("minmod" if p.limiter == 0 else "van Albada") if p.ischeme == 2
-                              else "first order") + " + HLLC(Davis), SSP-RK3", "dt": dt,
-                   **{k: v for k, v in meta.items() if k != "name"}},
-        "mixed": {"value": round(val_m, 1), "ms_per_step": round(wall_m / a.steps * 1e3, 4),
-                  "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},
-        "kernels": kern,
-        "roofline": roof,
-        "stage_roofline": stage_roof,
+        "config": head["config"],
+        "mixed": head["mixed"],
+        "kernels": head["kernels"],
+        "roofline": head["roofline"],
+        "stage_roofline": head["stage_roofline"],
+        "hbm_copy": copy,
         "cpu_baseline": cpu,
+        "adjoint": adj,
+        "extra": extra,
+        "wall_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(res), flush=True)
 

@@ -170,6 +170,7 @@ _DECLS = [
                                    C.c_double, _dp, _dp, _dp]),
     ("fvg_limiter_axis_host", C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    ("fvg_stream_copy", C.c_int, [C.c_int, _vp, _vp, C.c_size_t, C.c_int, _dp]),
     ("fvg_set_state_device", C.c_int, [_vp, _vp]),
     ("fvg_get_state_device", C.c_int, [_vp, _vp]),
     ("fv_gpu_attach", C.c_int, [_vp, C.c_int, C.POINTER(FvgParams), C.c_int]),

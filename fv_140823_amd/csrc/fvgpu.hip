@@ -1887,3 +1887,60 @@ int fvg_sync(fvg_ctx *c) {
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------ bandwidth probe */
+
+/* A plain streaming copy: each thread moves four 16-byte words, all four loads issued
+ * before the stores, consecutive lanes on consecutive words (1 KiB per wave and load). */
+constexpr int kCopyBlock = 256, kCopyPerThread = 4;
+
+__global__ __launch_bounds__(kCopyBlock) void k_stream_copy(const double2 *__restrict__ s,
+                                                            double2 *__restrict__ d, size_t n) {
+  const size_t base = (size_t)blockIdx.x * (kCopyBlock * kCopyPerThread) + threadIdx.x;
+  double2 v[kCopyPerThread];
+#pragma unroll
+  for (int k = 0; k < kCopyPerThread; ++k) {
+    const size_t i = base + (size_t)k * kCopyBlock;
+    if (i < n) v[k] = s[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kCopyPerThread; ++k) {
+    const size_t i = base + (size_t)k * kCopyBlock;
+    if (i < n) d[i] = v[k];
+  }
+}
+
+extern "C" int fvg_stream_copy(int device, void *dst, const void *src, size_t bytes, int reps,
+                               double *ms) {
+  if (!dst || !src || !ms || reps < 1 || bytes == 0 || bytes % 16 ||
+      ((uintptr_t)dst | (uintptr_t)src) % 16)
+    return FVG_ERR_ARG;
+  const size_t n = bytes / 16;
+  const size_t per_block = (size_t)kCopyBlock * kCopyPerThread;
+  const size_t blocks = (n + per_block - 1) / per_block;
+  if (blocks > 0x7fffffffu) return FVG_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return FVG_ERR_HIP;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) return FVG_ERR_HIP;
+  int rc = FVG_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = FVG_ERR_HIP;
+  if (!rc) {
+    const dim3 g((unsigned)blocks), b(kCopyBlock);
+    hipLaunchKernelGGL(k_stream_copy, g, b, 0, st, (const double2 *)src, (double2 *)dst, n);
+    bool ok = hipEventRecord(e0, st) == hipSuccess;
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, g, b, 0, st, (const double2 *)src, (double2 *)dst, n);
+    ok = ok && hipGetLastError() == hipSuccess;
+    ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float t = 0.f;
+    ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+    if (ok) *ms = (double)t / reps;
+    else rc = FVG_ERR_HIP;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  return rc;
+}

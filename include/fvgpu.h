@@ -310,6 +310,12 @@ int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc
  * counterpart in the reference. */
 int fvg_limiter_axis_host(int limiter, double eps, int bp, int bm, double dP, double dM,
                           double ip, double im, double *out);
+/* Bandwidth probe (no counterpart in the reference): 1 + reps copies of `bytes` (a multiple
+ * of 16, 16-byte aligned device pointers on `device`) by a streaming kernel of 16-byte
+ * loads and stores on a stream of its own; *ms = the mean duration of one of the last
+ * reps copies (HIP events around them).  It gives the HBM rate a plain copy reaches,
+ * beside the 8 TB/s peak, and a kernel of known traffic for the rocprof byte counters. */
+int fvg_stream_copy(int device, void *dst, const void *src, size_t bytes, int reps, double *ms);
 /* device-to-device state copies on the context's stream (torch interop) */
 int fvg_set_state_device(fvg_ctx *ctx, const void *dS /* device [neq][ncell] f64 */);
 int fvg_get_state_device(fvg_ctx *ctx, void *dS);

@@ -256,3 +256,65 @@ def test_ns_adjoint_matches_central_fd():
         rel = abs(adj - fd) / abs(fd)
         print(f"  dir {k}: adjoint {adj:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
         assert rel <= 1e-6, (adj, fd)
+
+
+def test_vortex_workload_full_size():
+    """The adjoint rollout workload at its size: periodic 10^3 box, 64^3 base with a
+    two-level patch over the vortex core (496,896 leaves), van Albada MUSCL, 50 SSP-RK3
+    steps at CFL 0.5, checkpoints every 10 steps, objective = time-integrated pressure
+    over a probe box beside the core.  The gradient is checked against the tangent along
+    3 seeded random directions (<= 1e-10), against central differences along 3 seeded
+    smooth directions (<= 1e-6, eps 1e-6, double-double J), and the gradient with
+    checkpoints every 10 steps equals the one with the whole rollout stored, bit for bit."""
+    from fv_140823_amd import configs
+    F, p, meta = configs.vortex()
+    assert F.nleaves == 496896
+    print(f"case vortex full size: {F.nleaves} leaves, {meta['steps']} steps, ckpt {meta['ckpt']}",
+          flush=True)
+    xc, _ = F.leaf_info()
+    probe = ((np.abs(xc[:, 0] - 6.5) < 1.0) & (np.abs(xc[:, 1] - 6.5) < 1.0)).astype(np.uint8)
+    g = fv.Solver.from_forest(F, p)
+    g.adj_set_probe(probe)
+    S0 = F.get_state()
+    dt = g.cfl_dt(meta["cfl"])
+    n, ck = meta["steps"], meta["ckpt"]
+    J = g.adj_forward(n, ck, dt)
+    assert g.adj_checkpoint_spacing == ck
+    lam = g.adj_backward()
+    g.set_state(S0)
+    J0 = g.adj_forward(n, 0, dt)
+    lam0 = g.adj_backward()
+    print(f"  J {J:.15e}; spacing chosen from free memory {g.adj_checkpoint_spacing}", flush=True)
+    assert J0 == J
+    assert np.array_equal(lam0, lam)
+    rng = np.random.default_rng(50)
+    for k in range(3):
+        v = rng.standard_normal(S0.shape) * np.abs(S0).max(axis=1, keepdims=True) * 1e-2
+        g.set_state(S0)
+        J2, dJ = g.tlm(v, n, dt)
+        assert abs(J2 - J) <= 1e-13 * abs(J)
+        adj = float((lam * v).sum())
+        print(f"  tangent dir {k}: adjoint {adj:.15e} tangent {dJ:.15e}", flush=True)
+        assert abs(adj - dJ) <= 1e-10 * abs(dJ), (adj, dJ)
+    eps = 1e-6
+
+    def Jdd(S):
+        g.set_state(S)
+        g.adj_forward(n, ck, dt)
+        return g.adj_objective()
+
+    for k in range(3):
+        # smooth bumps centred near the probe box, so that they reach it within the rollout
+        r = np.random.default_rng(500 + k)
+        c = np.array([r.uniform(5.5, 7.5), r.uniform(5.5, 7.5), r.uniform(0.0, 10.0)])
+        w = r.uniform(0.5, 1.5)
+        b = np.exp(-((xc - c) ** 2).sum(axis=1) / w ** 2)
+        v = np.zeros(S0.shape)
+        v[:4] = b * (r.choice([-1.0, 1.0], 4) * r.uniform(0.5, 1.5, 4))[:, None]
+        (ph, pl), (mh, ml) = Jdd(S0 + eps * v), Jdd(S0 - eps * v)
+        fd = ((ph - mh) + (pl - ml)) / (2 * eps)
+        adj = float((lam * v).sum())
+        rel = abs(adj - fd) / abs(fd)
+        print(f"  fd dir {k}: adjoint {adj:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-6, (adj, fd)
+    g.close()
