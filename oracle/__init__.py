@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
         L.orc_cfl_dt.restype = C.c_int
         L.orc_face_flux.argtypes = [vp, dp, dp, dp, dp]
         L.orc_face_flux.restype = C.c_int
+        L.orc_star_states.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_star_states.restype = C.c_int
         L.orc_eos.argtypes = [vp, dp, dp]
         L.orc_eos.restype = C.c_int
         L.orc_limited_slope.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -134,6 +136,18 @@ def face_flux(p, UL, UR, R9=(1, 0, 0, 0, 1, 0, 0, 0, 1)) -> np.ndarray:
     out = np.zeros(n)
     _ck(lib().orc_face_flux(_p(p), _d(a), _d(b), _d(r), _d(out)), "orc_face_flux")
     return out
+
+
+def star_states(p, UL, UR, R9=(1, 0, 0, 0, 1, 0, 0, 0, 1)):
+    """(S_L, S*, S_R) and both HLLC star states U*_L, U*_R of a face (rows of R9 = n, t1,
+    t2), whichever region the face falls in."""
+    n = 5 if p.model != 2 else 4
+    a = np.ascontiguousarray(UL, np.float64)
+    b = np.ascontiguousarray(UR, np.float64)
+    r = np.ascontiguousarray(R9, np.float64)
+    sp, usl, usr = np.zeros(3), np.zeros(n), np.zeros(n)
+    _ck(lib().orc_star_states(_p(p), _d(a), _d(b), _d(r), _d(sp), _d(usl), _d(usr)), "orc_star_states")
+    return sp, usl, usr
 
 
 def eos(p, U) -> tuple[float, float]:

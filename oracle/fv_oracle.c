@@ -13,12 +13,17 @@
  * metis.h, parmetis.h and PETSc headers that are not installed), so this file
  * is pinned by what the mathematics fixes (tests/test_oracle_pins.py):
  *   flux_adv / HLLC        consistency F(U,U) = F(U)·n, the four-region choice,
- *                          Toro's S* and star state, brute force 2-leaf sums;
+ *                          Toro's integral consistency of both star states (every
+ *                          row, S* != u), brute force 2-leaf sums;
  *   EOS                    closed forms of the ideal gas and barotropic_eos,
  *                          the quadratic root of the two-phase mixture;
  *   minmod / van Albada    closed forms on 3-point data incl. the tie, the 1e-10
  *                          gate and the boundary 0/0 fall-through;
  *   distance               closed forms (h/4,h/2), (h/4,3h/8), (h/8,3h/8);
+ *   2:1 hanging mean       exact linear fields on the coarse side of every hanging
+ *                          face (mean of the 4 finer leaves at their centroid);
+ *   inlet ghost (bc 1)     a uniform inlet state is steady, supersonic inflow passes
+ *                          F(U_inlet), the MUSCL ghost enters the limiter by its sign;
  *   RHS                    free-stream preservation on uniform and 2:1 meshes,
  *                          discrete conservation, mirror symmetry;
  *   gradcalc               exact gradients of linear fields (interior cells);
@@ -479,19 +484,21 @@ static void orc_star(OCTX *o, const OSOL *s, const double R[3][3], double SS, do
   if (o->p->model == 3) o->Us_temp[4] = cs * s->ymass;
 }
 
-/* hllc_euler.c:3-90 with compute_wave_speeds.c:3-90 (Davis speeds) */
-static int orc_hllc(OCTX *o, BRANCH *crnt, int ifc) {
+/* rotated velocities and wave speeds of the face states sol / sol1
+ * (compute_wave_speeds.c:33-87: Davis S_L, S_R and the contact speed S*) */
+typedef struct OWAVES {
+  double uL, vL, wL, uR, vR, wR, SL, SS, SR;
+} OWAVES;
+
+static int orc_waves(const OCTX *o, const double R[3][3], OWAVES *w) {
   const OSOL *L = &o->sol, *Rs = &o->sol1;
-  const CELL *cl = crnt->cl;
-  const double R[3][3] = {{cl->nx[ifc], cl->ny[ifc], cl->nz[ifc]},
-                          {cl->nxt1[ifc], cl->nyt1[ifc], cl->nzt1[ifc]},
-                          {cl->nxt2[ifc], cl->nyt2[ifc], cl->nzt2[ifc]}};
-  double uL = L->u * R[0][0] + L->v * R[0][1] + L->w * R[0][2];
-  double vL = L->u * R[1][0] + L->v * R[1][1] + L->w * R[1][2];
-  double wL = L->u * R[2][0] + L->v * R[2][1] + L->w * R[2][2];
-  double uR = Rs->u * R[0][0] + Rs->v * R[0][1] + Rs->w * R[0][2];
-  double vR = Rs->u * R[1][0] + Rs->v * R[1][1] + Rs->w * R[1][2];
-  double wR = Rs->u * R[2][0] + Rs->v * R[2][1] + Rs->w * R[2][2];
+  w->uL = L->u * R[0][0] + L->v * R[0][1] + L->w * R[0][2];
+  w->vL = L->u * R[1][0] + L->v * R[1][1] + L->w * R[1][2];
+  w->wL = L->u * R[2][0] + L->v * R[2][1] + L->w * R[2][2];
+  w->uR = Rs->u * R[0][0] + Rs->v * R[0][1] + Rs->w * R[0][2];
+  w->vR = Rs->u * R[1][0] + Rs->v * R[1][1] + Rs->w * R[1][2];
+  w->wR = Rs->u * R[2][0] + Rs->v * R[2][1] + Rs->w * R[2][2];
+  const double uL = w->uL, uR = w->uR;
   double ruL = L->r * uL, ruR = Rs->r * uR;
   double cL = L->c, cR = Rs->c;
   double SR = (uL + cL) > (uR + cR) ? (uL + cL) : (uR + cR);
@@ -500,7 +507,23 @@ static int orc_hllc(OCTX *o, BRANCH *crnt, int ifc) {
   double s11R = orc_rotate_s11(Rs->p_hydro, R);
   double SS = ((L->r * sq(uL) + s11L) - (Rs->r * sq(uR) + s11R) - (SL * ruL) + (SR * ruR)) /
               (ruL - ruR - (SL * L->r) + (SR * Rs->r));
-  if (isnan(SS)) return ORC_ERR_NONFINITE;
+  w->SL = SL;
+  w->SS = SS;
+  w->SR = SR;
+  return isnan(SS) ? ORC_ERR_NONFINITE : ORC_OK;
+}
+
+/* hllc_euler.c:3-90 with compute_wave_speeds.c:3-90 (Davis speeds) */
+static int orc_hllc(OCTX *o, BRANCH *crnt, int ifc) {
+  const OSOL *L = &o->sol, *Rs = &o->sol1;
+  const CELL *cl = crnt->cl;
+  const double R[3][3] = {{cl->nx[ifc], cl->ny[ifc], cl->nz[ifc]},
+                          {cl->nxt1[ifc], cl->nyt1[ifc], cl->nzt1[ifc]},
+                          {cl->nxt2[ifc], cl->nyt2[ifc], cl->nzt2[ifc]}};
+  OWAVES wv;
+  if (orc_waves(o, R, &wv)) return ORC_ERR_NONFINITE;
+  const double uL = wv.uL, vL = wv.vL, wL = wv.wL, uR = wv.uR, vR = wv.vR, wR = wv.wR;
+  const double SL = wv.SL, SS = wv.SS, SR = wv.SR;
   const int neq = o->neq;
   for (int v = 0; v < neq; ++v) o->flux_HLLC[v] = 0.0;
   int flag = 0;
@@ -746,6 +769,32 @@ int orc_face_flux(const fvg_params *p, const double *UL, const double *UR, const
   if (rc) return rc;
   for (int v = 0; v < o.neq; ++v)
     out[v] = (o.flux_adv[v][0] * nx + o.flux_adv[v][1] * ny + o.flux_adv[v][2] * nz) + o.flux_HLLC[v];
+  return ORC_OK;
+}
+
+/* Both star states of star_region_euler.c:3-40 for the face states UL | UR (whichever
+ * region the face falls in), with the wave speeds (S_L, S*, S_R) they are built from;
+ * for the integral-consistency pin of the star region. */
+int orc_star_states(const fvg_params *p, const double *UL, const double *UR, const double *R9,
+                    double *speeds, double *UsL, double *UsR) {
+  OCTX o;
+  memset(&o, 0, sizeof(o));
+  o.p = p;
+  o.neq = orc_neq(p->model);
+  int rc = orc_fill(&o, &o.sol, UL, 1);
+  if (!rc) rc = orc_fill(&o, &o.sol1, UR, 1);
+  if (rc) return rc;
+  const double R[3][3] = {{R9[0], R9[1], R9[2]}, {R9[3], R9[4], R9[5]}, {R9[6], R9[7], R9[8]}};
+  OWAVES wv;
+  rc = orc_waves(&o, R, &wv);
+  if (rc) return rc;
+  speeds[0] = wv.SL;
+  speeds[1] = wv.SS;
+  speeds[2] = wv.SR;
+  orc_star(&o, &o.sol, R, wv.SS, wv.SL, wv.uL, wv.vL, wv.wL);
+  for (int v = 0; v < o.neq; ++v) UsL[v] = o.Us_temp[v];
+  orc_star(&o, &o.sol1, R, wv.SS, wv.SR, wv.uR, wv.vR, wv.wR);
+  for (int v = 0; v < o.neq; ++v) UsR[v] = o.Us_temp[v];
   return ORC_OK;
 }
 

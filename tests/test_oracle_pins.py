@@ -5,14 +5,18 @@ index, a wrong coefficient -- fails it:
 
   HLLC (hllc_euler.c, compute_wave_speeds.c, star_region_euler.c, flux_adv.c)
     consistency F(U,U) = F(U).n in any frame, rotational invariance, mirror
-    antisymmetry, supersonic upwinding, exact resolution of a stationary contact;
+    antisymmetry, supersonic upwinding, exact resolution of a stationary contact,
+    Toro's integral consistency of both star states (models 1-3, every row);
   EOS (facercnstr.c:30-35, barotropic_eos.c)
     ideal-gas closed form with p_inf, c^2 = dp/drho of each barotropic branch by
     finite differences, branch continuity, the two-phase root closes the volume
     fractions (alpha_gas + alpha_liquid = 1);
   limiter (reconctruct_V3.c)  minmod closed forms incl. tie, 1e-10 gate, 0/0 and
     inf fall-through; van Albada symmetry and limits;
-  faceinterp / distance       exact face values of a linear field (h/4, h/2);
+  faceinterp / distance       exact face values of a linear field (h/4, h/2), also on
+    the coarse side of every hanging face (the mean of the 4 finer leaves);
+  inlet (bc_inlet.c, asinvalues.c)  uniform inlet state steady, supersonic inflow
+    passes F(U_inlet), the MUSCL ghost enters the limiter by its sign;
   RHS (residual_faces_AMR.c)  free-stream preservation on uniform and 2:1 meshes,
     conservation on periodic meshes, x<->y mirror symmetry, brute-force face sum;
   gradcalc                    exact gradients of linear fields;
@@ -338,3 +342,164 @@ def test_oracle_threads_do_not_change_results():
         out.append(F.get_state())
     oracle.set_threads(1)
     assert np.array_equal(out[0], out[1])
+
+
+def _phys_flux_model(p, U, n):
+    """F(U).n of the model's conservation law, with the pressure of the model's EOS
+    (pinned above): mass, momentum, and energy (model 1) or mass fraction (model 3)."""
+    pr, _ = oracle.eos(p, U)
+    rho, m = U[0], np.asarray(U[1:4])
+    un = (m / rho) @ n
+    out = [rho * un, *(m * un + pr * n)]
+    if p.model == 1:
+        out.append(un * (U[4] + pr))
+    elif p.model == 3:
+        out.append(U[4] * un)
+    return np.array(out)
+
+
+def _rand_state_model(model):
+    if model == 1:
+        return _rand_state()
+    rho = RNG.uniform(960.0, 1000.0) if model == 2 else RNG.uniform(500.0, 990.0)
+    u = RNG.uniform(-3.0, 3.0, 3)
+    U = [rho, *(rho * u)]
+    if model == 3:
+        U.append(rho * RNG.uniform(1e-4, 5e-3))
+    return np.array(U)
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+@pytest.mark.parametrize("k", range(6))
+def test_hllc_star_states_integral_consistency(model, k):
+    """Toro's integral consistency of the HLLC star region (Riemann Solvers, eq. 10.24):
+    (S* - S_L) U*_L + (S_R - S*) U*_R = S_R U_R - S_L U_L - (F(U_R) - F(U_L)), every
+    row, with the physical fluxes taken from the conservation law.  It holds for the
+    star states of star_region_euler.c:19-37 with the S* of compute_wave_speeds.c:81-82
+    and fails for any other energy or mass-fraction row (a pin the contact and
+    upwinding checks cannot give: there S* = u and the correction terms vanish)."""
+    p = fv.params(model)
+    UL, UR = _rand_state_model(model), _rand_state_model(model)
+    R9 = _frame(RNG.standard_normal(3))
+    n = R9[:3]
+    (SL, SS, SR), UsL, UsR = oracle.star_states(p, UL, UR, R9)
+    assert SL < SS < SR
+    lhs = (SS - SL) * UsL + (SR - SS) * UsR
+    rhs = SR * UR - SL * UL - (_phys_flux_model(p, UR, n) - _phys_flux_model(p, UL, n))
+    scale = max(abs(SL), abs(SR)) * np.abs(np.stack([UL, UR])).max(axis=0)
+    assert np.all(np.abs(lhs - rhs) <= 1e-13 * scale), (lhs - rhs) / scale
+    # and the flux is the star-region flux of the side the face falls in
+    f = oracle.face_flux(p, UL, UR, R9)
+    if SL <= 0 <= SS:
+        ref = _phys_flux_model(p, UL, n) + SL * (UsL - UL)
+    elif SS <= 0 <= SR:
+        ref = _phys_flux_model(p, UR, n) + SR * (UsR - UR)
+    else:
+        ref = _phys_flux_model(p, UL if SL > 0 else UR, n)
+    assert np.allclose(f, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_faceinterp_two_to_one_coarse_side_exact():
+    """A linear field is reconstructed exactly at every face whose stencil holds no
+    coarser neighbour: same-level neighbours (distance.c:85-87) and the coarse side of
+    a hanging face, where the neighbour state is the mean of the 4 finer leaves
+    (facevalues_MUSCL.c:31-47, at their centroid) and the distances use the finer
+    leaves' parent (distance.c:90-115).  A wrong weight of one of the four finer
+    leaves, or a wrong factor, breaks it on the hanging faces; the field has lateral
+    gradients, so the four leaves differ."""
+    F = amr(3, 5)
+    A = fv.FlatMesh(F.run).arrays()
+    xc, lev = F.leaf_info()
+    S, g = _linear_field(xc)
+    F.set_state(S)
+    oracle.faceinterp(F, fv.params(1))
+    SF = F.get_field(fv.FIELD_SF)
+    cls = np.stack([(A["cflag"] >> (2 * f)) & 3 for f in range(6)])  # 0 same, 1 finer, 2 coarser, 3 boundary
+    h = 1.0 / 2.0 ** (lev - 1)
+    axis_sign = {0: (1, -1), 1: (0, 1), 2: (1, 1), 3: (0, -1), 4: (2, -1), 5: (2, 1)}
+    opp = {0: 2, 1: 3, 2: 0, 3: 1, 4: 5, 5: 4}
+    n_hanging = 0
+    for f, (a, sgn) in axis_sign.items():
+        ok = (cls[f] <= 1) & (cls[opp[f]] <= 1)
+        n_hanging += int((ok & (cls[f] == 1)).sum())
+        x = xc.copy()
+        x[:, a] += sgn * h / 2
+        expect = _linear_field(x)[0]
+        err = np.abs(SF[f][:, ok] - expect[:, ok]) / np.abs(expect[:, ok]).max(axis=1, keepdims=True)
+        assert err.max() <= 1e-14, (f, err.max())
+    assert n_hanging > 50
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+@pytest.mark.parametrize("ischeme", [1, 2])
+def test_inlet_equal_state_gives_zero_residual(model, ischeme):
+    """A moving uniform state equal to the inlet state (bc 1 on -x, bc_inlet.c:14-98,
+    asinvalues.c:3-66) is a steady solution: the residual vanishes to round-off in every
+    cell, the wall cells included (a ghost state differing from the inlet state in any
+    row, or an EOS of the ghost from another slot, leaves a residual there)."""
+    neq = fv.neq_of(model)
+    F = uniform(3, neq, bc=(1, 2, 2, 2, 2, 2))
+    U = _rand_state_model(model)
+    p = fv.params(model, ischeme=ischeme, inlet=U)
+    F.set_state(np.repeat(U[:, None], F.nleaves, axis=1))
+    r = oracle.full_rhs(F, p)
+    scale = np.abs(oracle.face_flux(p, U, U)).max() * (1.0 / 8) ** 2
+    assert np.abs(r).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+def test_inlet_supersonic_inflow_flux(model):
+    """Supersonic inflow through the -x inlet side (both states' u - c > 0): the
+    boundary face passes exactly F(U_inlet), so a wall cell of a uniform interior state
+    U_i gets RHS = A (F(U_inlet) - F(U_i)).e_x (first order; the other faces cancel),
+    and no other cell feels the inlet."""
+    neq = fv.neq_of(model)
+    F = uniform(3, neq, bc=(1, 2, 2, 2, 2, 2))
+    xc, _ = F.leaf_info()
+    if model == 1:
+        Ui = prim_to_cons(np.array(1.0), np.array(3.0), np.array(0.2), np.array(-0.1), np.array(1.0)).ravel()
+        Uin = prim_to_cons(np.array(1.3), np.array(3.5), np.array(-0.1), np.array(0.3), np.array(1.2)).ravel()
+    elif model == 2:  # cavitating branch: c = sqrt(cbar) / rho, a few cm/s
+        Ui = np.array([500.0, 500.0 * 2.0, 500.0 * 0.1, 0.0])
+        Uin = np.array([600.0, 600.0 * 2.5, 0.0, 600.0 * -0.2])
+    else:  # mixture: c = c_liq
+        Ui = np.array([900.0, 900.0 * 2000.0, 0.0, 900.0 * 10.0, 900.0 * 1e-3])
+        Uin = np.array([950.0, 950.0 * 2100.0, 950.0 * -5.0, 0.0, 950.0 * 2e-3])
+    p = fv.params(model, ischeme=1, inlet=Uin)
+    for U in (Ui, Uin):
+        pr, c = oracle.eos(p, U)
+        assert U[1] / U[0] - c > 0
+    F.set_state(np.repeat(Ui[:, None], F.nleaves, axis=1))
+    r = oracle.full_rhs(F, p)
+    ex = np.array([1.0, 0.0, 0.0])
+    A = (1.0 / 8) ** 2
+    wall = xc[:, 0] < 1.0 / 8
+    expect = A * (_phys_flux_model(p, Uin, ex) - _phys_flux_model(p, Ui, ex))
+    scale = np.abs(_phys_flux_model(p, Ui, ex)).max() * A
+    assert np.abs(r[:, wall] - expect[:, None]).max() <= 1e-13 * scale
+    assert np.abs(r[:, ~wall]).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("side", [-1.0, 1.0])
+def test_inlet_ghost_in_muscl_slope_enters_by_sign(side):
+    """At a domain side distance.c returns d_cc = 0 (distance.c:55-60), so the slope
+    through the inlet ghost (deltavalues.c:12-13) is -delta1/0 = +-inf, and minmod keeps
+    the finite interior slope when the ghost lies on the side a monotone profile puts it
+    (reconctruct_V3.c:22-26), or no slope otherwise: with S linear in x, the wall cells'
+    +x face state is then the exact face value, or the cell value.  A ghost taken from
+    the interior (delta1 = 0 -> NaN -> no slope) or with a flipped sign fails one case."""
+    F = uniform(3, 5, bc=(1, 2, 2, 2, 2, 2))
+    xc, _ = F.leaf_info()
+    slope = np.array([0.4, 0.3, 0.2, 0.1, 0.5])
+    base = np.array([1.0, 0.2, 0.1, 0.05, 2.5])
+    S = base[:, None] + slope[:, None] * xc[:, 0][None, :]
+    h = 1.0 / 8
+    U_wall = base + slope * h / 2
+    p = fv.params(1, inlet=U_wall + side * 0.1 * slope)
+    F.set_state(S)
+    oracle.faceinterp(F, p)
+    SF = F.get_field(fv.FIELD_SF)
+    wall = xc[:, 0] < h
+    exact = base + slope * h
+    expect = exact if side < 0 else U_wall
+    assert np.allclose(SF[1][:, wall], expect[:, None], rtol=1e-14, atol=1e-14)
