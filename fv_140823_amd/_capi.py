@@ -141,6 +141,8 @@ _DECLS = [
     ("fvg_adj_set_probe", C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     ("fvg_adapt_flags", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.POINTER(C.c_int8)]),
+    ("fvg_adapt_flags_static", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.POINTER(C.c_int8)]),
     ("fvg_pressure", C.c_int, [_vp, _dp]),
     ("fvg_remesh_begin", C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),

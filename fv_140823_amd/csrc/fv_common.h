@@ -358,6 +358,11 @@ struct fvg_ctx {
   float *Sf = nullptr;
   bool sf_valid = false;
 
+  /* device address of the static flag staging array of this context's GPU (taken once
+   * per device at fvg_create; fvg_adapt_flags_static) and the status of that lookup */
+  int8_t *flag_scratch = nullptr;
+  int flag_scratch_rc = 0;
+
   /* adjoint / tangent buffers (fvadj.hip), released with the mesh */
   struct fv_adj_state *adj = nullptr;
 

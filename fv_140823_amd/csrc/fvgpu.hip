@@ -20,6 +20,8 @@
  * The gather reads each cell's face fluxes in the reference's order
  * f = 0..5, k = 0..3 (no atomics, bitwise deterministic).
  */
+#include <mutex>
+
 #include "fv_common.h"
 
 /* ------------------------------------------------------------------ kernels */
@@ -1044,6 +1046,35 @@ int fv_upload_tables(fvg_ctx *c, int nl, const double *h, const double *dist, co
 
 extern "C" {
 
+/* Staging of the refinement flags of fvg_adapt_flags_static: a file-scope device array,
+ * one copy on every GPU the library runs on, held until the process ends, so that the
+ * entry point allocates nothing.  Host code never passes the symbol itself to a kernel
+ * or a copy (on the host it is a stand-in, not a device address): it uses the address
+ * hipGetSymbolAddress returns for the current device, looked up once per device. */
+static __device__ __align__(256) int8_t scratch[FVG_STATIC_FLAGS_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices];
+int8_t *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+}  // namespace
+
+/* the device address of `scratch` on `device`, which must be current */
+static hipError_t scratch_address(int device, int8_t **ptr) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void *a = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? (int8_t *)a : nullptr;
+    g_scratch_done[device] = true;
+  }
+  *ptr = g_scratch_ptr[device];
+  return g_scratch_err[device];
+}
+
 int fvg_create(int device, const fvg_params *p, fvg_ctx **out) {
   if (!out) return FVG_ERR_ARG;
   int rc = check_params(p);
@@ -1064,6 +1095,8 @@ int fvg_create(int device, const fvg_params *p, fvg_ctx **out) {
     delete c;
     return FVG_ERR_HIP;
   }
+  /* with the device current; a failure is reported by fvg_adapt_flags_static only */
+  c->flag_scratch_rc = (int)scratch_address(device, &c->flag_scratch);
   *out = c;
   return FVG_OK;
 }
@@ -1646,24 +1679,50 @@ int fvg_gradcalc(fvg_ctx *c) {
   return finish(c, "gradcalc");
 }
 
+/* gradcalc, then the flags of every cell into the device buffer d (>= N bytes), then
+ * to the host */
+static int adapt_flags_via(fvg_ctx *c, int kind, double refine, double coarsen, int base_level,
+                           int max_level, int8_t *d, int8_t *flags) {
+  dispatch_grad(c, c->S, true);
+  DevMesh M = dev_mesh(c);
+  hipLaunchKernelGGL(k_adapt_flags, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream, M,
+                     (const double *)c->S, c->neq, (const double *)c->SG, kind, refine, coarsen,
+                     base_level, max_level, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(flags, d, (size_t)c->N, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  return FVG_OK;
+}
+
 int fvg_adapt_flags(fvg_ctx *c, int kind, double refine, double coarsen, int base_level,
                     int max_level, int8_t *flags) {
   if (!c || !flags) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
   if (kind != 0 && kind != 1) return set_err(c, FVG_ERR_ARG, "kind must be 0 or 1");
   (void)hipSetDevice(c->device);
-  dispatch_grad(c, c->S, true);
   int8_t *d = nullptr;
   HIPCHK(c, hipMalloc((void **)&d, (size_t)c->ns));
-  DevMesh M = dev_mesh(c);
-  hipLaunchKernelGGL(k_adapt_flags, dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream, M,
-                     (const double *)c->S, c->neq, (const double *)c->SG, kind, refine, coarsen,
-                     base_level, max_level, d);
-  hipError_t e = hipMemcpyAsync(flags, d, (size_t)c->N, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  int rc = adapt_flags_via(c, kind, refine, coarsen, base_level, max_level, d, flags);
   (void)hipFree(d);
-  if (e != hipSuccess) return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
-  return finish(c, "adapt flags");
+  return rc;
+}
+
+int fvg_adapt_flags_static(fvg_ctx *c, int kind, double refine, double coarsen, int base_level,
+                           int max_level, int8_t *flags) {
+  if (!c || !flags) return FVG_ERR_ARG;
+  if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
+  if (kind != 0 && kind != 1) return set_err(c, FVG_ERR_ARG, "kind must be 0 or 1");
+  if (c->N > FVG_STATIC_FLAGS_MAX) {
+    char b[160];
+    snprintf(b, sizeof b, "%d cells exceed the static flag buffer of %d cells (FVG_STATIC_FLAGS_MAX)",
+             c->N, (int)FVG_STATIC_FLAGS_MAX);
+    return set_err(c, FVG_ERR_ARG, b);
+  }
+  if (c->flag_scratch_rc != (int)hipSuccess || !c->flag_scratch)
+    return set_err(c, FVG_ERR_HIP, "no device address for the static flag buffer");
+  (void)hipSetDevice(c->device);
+  return adapt_flags_via(c, kind, refine, coarsen, base_level, max_level, c->flag_scratch, flags);
 }
 
 int fvg_pressure(fvg_ctx *c, double *p) {

@@ -202,6 +202,16 @@ int fvg_gradcalc(fvg_ctx *ctx);              /* SG[neq+1][3] from S (gradcalc.c:
  * merge where it is under `coarsen` above base_level.  Leaves SG = gradcalc(S). */
 int fvg_adapt_flags(fvg_ctx *ctx, int kind, double refine, double coarsen, int base_level,
                     int max_level, int8_t *flags);
+/* The same flags without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving or AMR
+ * loops, harnesses that forbid hipMalloc).  The flags are staged in a static device array
+ * of FVG_STATIC_FLAGS_MAX bytes that each GPU the library runs on holds until the process
+ * ends; meshes with more cells get FVG_ERR_ARG.  Every context on one GPU shares that
+ * array, so calls that overlap on one GPU (several contexts, several host threads) race;
+ * each call overwrites the first ncell bytes and reads back only those. */
+#define FVG_STATIC_FLAGS_MAX (1 << 24)
+int fvg_adapt_flags_static(fvg_ctx *ctx, int kind, double refine, double coarsen, int base_level,
+                           int max_level, int8_t *flags);
 /* Cell pressure (host f64 [ncell]) from the device S, with the FP64 EOS of the gradient
  * kernel: the P column of the export (volmvalues.c:22-40, exportfield_linked_V3.c:390). */
 int fvg_pressure(fvg_ctx *ctx, double *p);
