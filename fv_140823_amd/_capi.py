@@ -181,7 +181,7 @@ _DECLS = [
     ("fv_gpu_upload", C.c_int, [_vp]),
     ("fv_gpu_download", C.c_int, [_vp]),
     ("fv_gpu_context", _vp, [_vp]),
-    ("fv_gpu_last_status", C.c_int, []),
+    ("fv_gpu_last_status", C.c_int, [_vp]),
     ("RHS", None, [_vp]),
     ("RK", None, [C.c_int, _vp]),
     ("faceinterp", None, [_vp]),

@@ -9,32 +9,41 @@
  * host calls fv_gpu_remesh().  In compat mode each call copies its outputs back
  * into the leaves (LEAF::RHS/S/SN/SF/SG), so reference-shaped callers work
  * unchanged; otherwise the state stays on the device until fv_gpu_download().
+ *
+ * The binding's state is per RUN, in run->info: the reference declares struct INFO
+ * (strdata.h:423) but never defines it and no reference file reads or writes that
+ * field, so this file defines it.  fv_gpu_attach() overwrites run->info without
+ * reading it (a RUN on the reference's stack holds garbage there) and
+ * fv_gpu_detach() clears it; the entry points on a RUN whose info is NULL report
+ * FVG_ERR_STATE.  Only the leaf arrays the reference owns (S, SN, RHS, SF, SG) are
+ * written; the leaves' structs are only read.
  * The reference reports errors by printf + exit(0) (residual_faces_AMR.c:186-194);
- * these wrappers keep the void signature and record the status instead
- * (fv_gpu_last_status).
+ * these wrappers keep the void signature and record the status in the binding
+ * instead (fv_gpu_last_status(run)).
  */
 #include <stdlib.h>
 #include <string.h>
 
 #include "fvgpu.h"
 
-#define FV_MAX_BIND 64
+#define FV_BINDING_MAGIC 0x46564742u /* "FVGB" */
 
-typedef struct {
+struct INFO {
+  unsigned magic;
   RUN *run;
   fvg_ctx *ctx;
   fvh_flat *flat;
   int compat;
   int device;
-} fv_binding;
+  int status; /* of the last RHS / RK / faceinterp / gradcalc call */
+};
 
-static fv_binding g_bind[FV_MAX_BIND];
-static int g_status = FVG_OK;
+typedef struct INFO fv_binding;
 
 static fv_binding *find(RUN *run) {
-  for (int i = 0; i < FV_MAX_BIND; ++i)
-    if (g_bind[i].run == run && run) return &g_bind[i];
-  return NULL;
+  if (!run || !run->info) return NULL;
+  fv_binding *b = run->info;
+  return (b->magic == FV_BINDING_MAGIC && b->run == run) ? b : NULL;
 }
 
 static int leaves_io(RUN *run, int which, double *buf, int to_leaves) {
@@ -108,26 +117,26 @@ static int push(fv_binding *b, int which) {
 }
 
 int fv_gpu_attach(RUN *run, int device, const fvg_params *p, int compat) {
-  if (!run || !p) return FVG_ERR_ARG;
-  if (find(run)) return FVG_ERR_STATE;
+  if (!run || !p || !run->con || !run->topo) return FVG_ERR_ARG;
   /* MUSCL on primitive variables: the reference's cons2primtv divides by a density
    * that facevalues_MUSCL never sets (cons2primtv.c:10-13), so there is no defined
    * result to reproduce */
-  if (run->con && run->con->iprimtv != 0) return FVG_ERR_UNSUPPORTED;
-  fv_binding *b = NULL;
-  for (int i = 0; i < FV_MAX_BIND; ++i)
-    if (!g_bind[i].run) { b = &g_bind[i]; break; }
+  if (run->con->iprimtv != 0) return FVG_ERR_UNSUPPORTED;
+  fv_binding *b = (fv_binding *)calloc(1, sizeof(*b));
   if (!b) return FVG_ERR_NOMEM;
-  memset(b, 0, sizeof(*b));
   int rc = fvg_create(device, p, &b->ctx);
-  if (rc) return rc;
+  if (rc) { free(b); return rc; }
+  b->magic = FV_BINDING_MAGIC;
   b->run = run;
   b->compat = compat;
   b->device = device;
+  run->info = b;
   rc = fv_gpu_remesh(run);
   if (rc) {
     fvg_destroy(b->ctx);
-    memset(b, 0, sizeof(*b));
+    b->magic = 0;
+    free(b);
+    run->info = NULL;
   }
   return rc;
 }
@@ -137,7 +146,9 @@ int fv_gpu_detach(RUN *run) {
   if (!b) return FVG_ERR_STATE;
   fvg_destroy(b->ctx);
   fvh_flat_destroy(b->flat);
-  memset(b, 0, sizeof(*b));
+  b->magic = 0;
+  free(b);
+  run->info = NULL;
   return FVG_OK;
 }
 
@@ -175,36 +186,39 @@ fvg_ctx *fv_gpu_context(RUN *run) {
   return b ? b->ctx : NULL;
 }
 
-int fv_gpu_last_status(void) { return g_status; }
+int fv_gpu_last_status(RUN *run) {
+  fv_binding *b = find(run);
+  return b ? b->status : FVG_ERR_STATE;
+}
 
 void RHS(struct RUN *run) {
   fv_binding *b = find(run);
-  if (!b) { g_status = FVG_ERR_STATE; return; }
+  if (!b) return;
   int rc = fvg_rhs(b->ctx);
   if (!rc && b->compat) rc = pull(b, FVG_FIELD_RHS);
-  g_status = rc;
+  b->status = rc;
 }
 
 void RK(int irk, RUN *run) {
   fv_binding *b = find(run);
-  if (!b) { g_status = FVG_ERR_STATE; return; }
+  if (!b) return;
   int rc = fvg_rk(b->ctx, irk, run->con->dt);
   if (!rc && b->compat) rc = pull(b, irk == 0 ? FVG_FIELD_SN : FVG_FIELD_S);
-  g_status = rc;
+  b->status = rc;
 }
 
 void faceinterp(RUN *run) {
   fv_binding *b = find(run);
-  if (!b) { g_status = FVG_ERR_STATE; return; }
+  if (!b) return;
   int rc = fvg_faceinterp(b->ctx);
   if (!rc && b->compat) rc = pull(b, FVG_FIELD_SF);
-  g_status = rc;
+  b->status = rc;
 }
 
 void gradcalc(struct RUN *run) {
   fv_binding *b = find(run);
-  if (!b) { g_status = FVG_ERR_STATE; return; }
+  if (!b) return;
   int rc = fvg_gradcalc(b->ctx);
   if (!rc && b->compat) rc = pull(b, FVG_FIELD_SG);
-  g_status = rc;
+  b->status = rc;
 }

@@ -24,6 +24,7 @@
 
 #include "fvgpu.h"
 #include "fv_tables.h"
+#include "ptrmap.h"
 
 /* Tecplot BRICK node order of the reference's hexahedron nodes (TPNDS[.][0], :83-90) */
 static const int TP_BRICK[8] = {0, 1, 3, 2, 4, 5, 7, 6};
@@ -46,8 +47,9 @@ int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p) {
   const int neq = run->con->neq;
   if (model < 1 || model > 3) return FVG_ERR_UNSUPPORTED;
   int N = 0;
-  for (BRANCH *b = run->topo->locl; b; b = b->lnxt) b->fv_index = N++;
+  for (BRANCH *b = run->topo->locl; b; b = b->lnxt) ++N;
   if (N == 0) return FVG_ERR_ARG;
+  fv_ptrmap pos = {NULL, NULL, 0};  /* leaf -> list position (the leaves are only read) */
   BRANCH **lv = (BRANCH **)malloc((size_t)N * sizeof(BRANCH *));
   int32_t *elnd = (int32_t *)malloc((size_t)8 * N * sizeof(int32_t));   /* (elem,node) -> link */
   int32_t *next = (int32_t *)malloc((size_t)8 * N * sizeof(int32_t));   /* member list links */
@@ -55,10 +57,16 @@ int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p) {
   int32_t *num = (int32_t *)malloc((size_t)8 * N * sizeof(int32_t));    /* link -> node number */
   FILE *fp = NULL;
   int rc = 0;
-  if (!lv || !elnd || !next || !lk || !num) { rc = FVG_ERR_NOMEM; goto done; }
+  if (!lv || !elnd || !next || !lk || !num || fv_ptrmap_init(&pos, (size_t)N)) {
+    rc = FVG_ERR_NOMEM;
+    goto done;
+  }
   {
     int i = 0;
-    for (BRANCH *b = run->topo->locl; b; b = b->lnxt) lv[i++] = b;
+    for (BRANCH *b = run->topo->locl; b; b = b->lnxt) {
+      fv_ptrmap_put(&pos, b, i);
+      lv[i++] = b;
+    }
   }
   for (size_t q = 0; q < (size_t)8 * N; ++q) elnd[q] = -1;
   int nlink = 0;
@@ -77,12 +85,13 @@ int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p) {
         const int32_t s1 = 8 * e + node1;
         const BRANCH *b2 = join ? b->neigtr[f][finer ? k : 0] : NULL;
         if (b2 == b) b2 = NULL;  /* a periodic axis one cell long */
-        const int node2 = b2 ? node_at(b2, &b->cl->nd[node1]) : -1;
+        const int32_t e2 = b2 ? fv_ptrmap_get(&pos, b2) : -1;
+        const int node2 = e2 >= 0 ? node_at(b2, &b->cl->nd[node1]) : -1;
         if (node2 < 0) {
           if (elnd[s1] == -1) OPEN(s1);
           continue;
         }
-        const int32_t s2 = 8 * b2->fv_index + node2;
+        const int32_t s2 = 8 * e2 + node2;
         const int32_t l1 = elnd[s1], l2 = elnd[s2];
         if (l1 == -1 && l2 == -1) {
           OPEN(s1);
@@ -156,5 +165,6 @@ int fvh_export_tecplot(RUN *run, const char *path, int model, const double *p) {
 done:
   if (fp && fclose(fp) != 0) rc = FVG_ERR_IO;
   free(lv); free(elnd); free(next); free(lk); free(num);
+  fv_ptrmap_free(&pos);
   return rc;
 }

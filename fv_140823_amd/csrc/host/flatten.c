@@ -31,6 +31,7 @@
 
 #include "fvgpu.h"
 #include "fv_tables.h"
+#include "ptrmap.h"
 
 #define FV_MAXLEV 32
 
@@ -155,19 +156,27 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
   if (!run || !run->topo || !run->con || !out) return FVG_ERR_ARG;
   const int neq = run->con->neq;
   int N = 0;
-  for (BRANCH *b = run->topo->locl; b; b = b->lnxt) b->fv_index = N++;
+  for (BRANCH *b = run->topo->locl; b; b = b->lnxt) ++N;
   if (N == 0) return FVG_ERR_ARG;
   fvh_flat *fl = (fvh_flat *)calloc(1, sizeof(*fl));
   if (!fl) return FVG_ERR_NOMEM;
+  /* leaf -> list position: the leaves' own structs are only read */
+  fv_ptrmap pos = {NULL, NULL, 0};
   BRANCH **lv = (BRANCH **)malloc((size_t)N * sizeof(BRANCH *));
   fl->cflag = (uint32_t *)calloc((size_t)N, sizeof(uint32_t));
   fl->nbr = (int32_t *)malloc((size_t)6 * N * sizeof(int32_t));
   fl->gslot = (int32_t *)malloc((size_t)6 * N * sizeof(int32_t));
   int rc = 0;
-  if (!lv || !fl->cflag || !fl->nbr || !fl->gslot) { rc = FVG_ERR_NOMEM; goto fail; }
+  if (!lv || !fl->cflag || !fl->nbr || !fl->gslot || fv_ptrmap_init(&pos, (size_t)N)) {
+    rc = FVG_ERR_NOMEM;
+    goto fail;
+  }
   {
     int i = 0;
-    for (BRANCH *b = run->topo->locl; b; b = b->lnxt) lv[i++] = b;
+    for (BRANCH *b = run->topo->locl; b; b = b->lnxt) {
+      fv_ptrmap_put(&pos, b, i);
+      lv[i++] = b;
+    }
   }
   /* ---- per-level geometry tables */
   unsigned char hav_h[FV_MAXLEV * 3] = {0}, hav_a[FV_MAXLEV * 3] = {0}, hav_v[FV_MAXLEV] = {0};
@@ -213,10 +222,10 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
         nb = nquad++;
       } else if (b->lfc_neigs[f] == 1 && n0->level == b->level) {
         cls = FV_CLS_SAME;
-        nb = n0->fv_index;
+        nb = fv_ptrmap_get(&pos, n0);
       } else if (b->lfc_neigs[f] == 1 && n0->level == b->level - 1) {
         cls = FV_CLS_COARSER;
-        nb = n0->fv_index;
+        nb = fv_ptrmap_get(&pos, n0);
       } else {
         rc = FVG_ERR_UNSUPPORTED;
         break;
@@ -241,7 +250,9 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
       for (int k = 0; k < 4; ++k) {
         BRANCH *kb = b->neigtr[f][k];
         if (!kb || kb->level != b->level + 1) { rc = FVG_ERR_UNSUPPORTED; break; }
-        fl->quad_nbr[4 * q + k] = kb->fv_index;
+        const int32_t kq = fv_ptrmap_get(&pos, kb);
+        if (kq < 0) { rc = FVG_ERR_UNSUPPORTED; break; }
+        fl->quad_nbr[4 * q + k] = kq;
       }
     }
   }
@@ -364,6 +375,7 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
   }
   free(lv);
   lv = NULL;
+  fv_ptrmap_free(&pos);
   fl->m.ncell = N;
   fl->m.nface = nface;
   fl->m.nface_axis[0] = (int32_t)nfa[0];
@@ -389,6 +401,7 @@ int fvh_flatten_run(RUN *run, fvh_flat **out) {
   return FVG_OK;
 fail:
   free(lv);
+  fv_ptrmap_free(&pos);
   fvh_flat_destroy(fl);
   return rc;
 }

@@ -170,6 +170,7 @@ struct fvh_forest {
   BRANCH **roots;
   int nroots;
   int nroot[3];
+  double x0[3], x1[3];   /* the box */
   int periodic[3];
   int bc[6];
   int neq;
@@ -191,13 +192,27 @@ struct fvh_forest {
 
 /* ------------------------------------------------------------------ allocation */
 
+/* A node of this forest: the reference's BRANCH, then what the forest keeps per node and
+ * the reference's struct has no field for: the node's integer address at its level (root
+ * grid index scaled by 2^(level-1)) and its position in the leaf list.  Only nodes made
+ * by this file are fvh_nodes; the flattener and the binding, which take any RUN, never
+ * use these fields. */
+typedef struct fvh_node {
+  BRANCH b;
+  int ijk[3];
+  int index;
+} fvh_node;
+
+#define IJK(p) (((fvh_node *)(p))->ijk)
+#define IDX(p) (((fvh_node *)(p))->index)
+
 static size_t align8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 /* One block per branch: BRANCH + neighbour tables + CELL + LEAF arrays, laid out over
  * blk (zeroed, branch_bytes(neq) long); with blk null only the size is returned. */
 static BRANCH *branch_layout(char *blk, int neq, size_t *bytes) {
   size_t sz = 0;
-  size_t o_br = sz; sz = align8(sz + sizeof(BRANCH));
+  size_t o_br = sz; sz = align8(sz + sizeof(fvh_node));
   size_t o_ntr = sz; sz = align8(sz + 6 * sizeof(BRANCH **));
   size_t o_ntr_rows = sz; sz = align8(sz + 6 * 4 * sizeof(BRANCH *));
   size_t o_ints = sz; sz = align8(sz + 4 * 6 * sizeof(int));
@@ -251,7 +266,7 @@ static BRANCH *branch_layout(char *blk, int neq, size_t *bytes) {
   b->nlnd = 8;
   b->nlfc = 6;
   b->nsplit = 0;
-  b->fv_index = -1;
+  IDX(b) = -1;
   return b;
 }
 
@@ -328,7 +343,7 @@ static int ht_insert(struct fvh_forest *fr, BRANCH *b) {
     int rc = ht_reserve(fr, fr->hcount + 1);
     if (rc) return rc;
   }
-  uint64_t k = hkey(b->level, b->fv_ijk);
+  uint64_t k = hkey(b->level, IJK(b));
   size_t p = hmix(k) & (fr->hcap - 1);
   while (fr->ht[p]) {
     if (fr->hk[p] == k) { fr->ht[p] = b; return 0; }
@@ -342,7 +357,7 @@ static int ht_insert(struct fvh_forest *fr, BRANCH *b) {
 static void ht_remove(struct fvh_forest *fr, const BRANCH *b) {
   if (!fr->hcap) return;
   const size_t mask = fr->hcap - 1;
-  uint64_t k = hkey(b->level, b->fv_ijk);
+  uint64_t k = hkey(b->level, IJK(b));
   size_t p = hmix(k) & mask;
   while (fr->ht[p] && fr->hk[p] != k) p = (p + 1) & mask;
   if (!fr->ht[p]) return;
@@ -378,14 +393,14 @@ static void set_nodes(const struct fvh_forest *fr, BRANCH *b) {
   double L[3];
   int N[3];
   for (int a = 0; a < 3; ++a) {
-    L[a] = fr->topo.fv_x1[a] - fr->topo.fv_x0[a];
+    L[a] = fr->x1[a] - fr->x0[a];
     N[a] = ncell_axis(fr, b->level, a);
   }
   for (int n = 0; n < 8; ++n) {
     int bit[3] = {n & 1, (n >> 1) & 1, (n >> 2) & 1};
     double x[3];
     for (int a = 0; a < 3; ++a)
-      x[a] = fr->topo.fv_x0[a] + ((double)(b->fv_ijk[a] + bit[a]) * L[a]) / (double)N[a];
+      x[a] = fr->x0[a] + ((double)(IJK(b)[a] + bit[a]) * L[a]) / (double)N[a];
     b->cl->nd[n].num = n;
     b->cl->nd[n].x = x[0];
     b->cl->nd[n].y = x[1];
@@ -494,7 +509,7 @@ static BRANCH *new_branch(const struct fvh_forest *fr, char *blk, BRANCH *prnt, 
   b->level = level;
   b->prnt = prnt;
   b->root = root;
-  for (int a = 0; a < 3; ++a) b->fv_ijk[a] = ijk[a];
+  for (int a = 0; a < 3; ++a) IJK(b)[a] = ijk[a];
   for (int f = 0; f < 6; ++f) {
     b->cl->fc[f].num = f;
     b->cl->fc[f].type = 0;
@@ -510,8 +525,8 @@ static BRANCH *new_branch(const struct fvh_forest *fr, char *blk, BRANCH *prnt, 
  * different leaves may be done on different threads. */
 static void split_kids(const struct fvh_forest *fr, BRANCH *b, char *const *blk) {
   for (int c = 0; c < 8; ++c) {
-    int ijk[3] = {2 * b->fv_ijk[0] + (c & 1), 2 * b->fv_ijk[1] + ((c >> 1) & 1),
-                  2 * b->fv_ijk[2] + ((c >> 2) & 1)};
+    int ijk[3] = {2 * IJK(b)[0] + (c & 1), 2 * IJK(b)[1] + ((c >> 1) & 1),
+                  2 * IJK(b)[2] + ((c >> 2) & 1)};
     BRANCH *k = new_branch(fr, blk[c], b, b->level + 1, ijk, b->root);
     for (int v = 0; v < fr->neq; ++v) {
       k->el->S[v] = b->el->S[v];
@@ -625,7 +640,7 @@ static int collect_leaves(BRANCH *b, BRANCH ***out, int *n, int *cap) {
 /* neighbour across face f of leaf b: fills neigtr/lfc_neigs/neigfc/nsfc and fc[f].bc */
 static int resolve_face(struct fvh_forest *fr, BRANCH *b, int f) {
   int a = FV_FACE_AXIS[f], s = FV_FACE_SIGN[f];
-  int n[3] = {b->fv_ijk[0], b->fv_ijk[1], b->fv_ijk[2]};
+  int n[3] = {IJK(b)[0], IJK(b)[1], IJK(b)[2]};
   int Na = ncell_axis(fr, b->level, a);
   n[a] += s;
   for (int k = 0; k < 4; ++k) b->neigtr[f][k] = NULL;
@@ -713,7 +728,7 @@ static void relist_link(void *v, int lo, int hi) {
   relist_ctx *r = (relist_ctx *)v;
   for (int i = lo; i < hi; ++i) {
     BRANCH *b = r->lv[i];
-    b->fv_index = i;
+    IDX(b) = i;
     b->lprv = i ? r->lv[i - 1] : r->glob;
     b->lnxt = (i + 1 < r->n) ? r->lv[i + 1] : NULL;
   }
@@ -822,7 +837,7 @@ static int rebuild(struct fvh_forest *fr) {
 static int across(const struct fvh_forest *fr, const BRANCH *b, int f, int n[3]) {
   int a = FV_FACE_AXIS[f];
   int Na = ncell_axis(fr, b->level, a);
-  for (int d = 0; d < 3; ++d) n[d] = b->fv_ijk[d];
+  for (int d = 0; d < 3; ++d) n[d] = IJK(b)[d];
   n[a] += FV_FACE_SIGN[f];
   if (n[a] < 0 || n[a] >= Na) {
     if (!fr->periodic[a]) return 0;
@@ -832,14 +847,14 @@ static int across(const struct fvh_forest *fr, const BRANCH *b, int f, int n[3])
 }
 
 static void mark_subtree(const BRANCH *b, uint8_t *mark) {
-  if (!b->nkids) { mark[b->fv_index] = 1; return; }
+  if (!b->nkids) { mark[IDX(b)] = 1; return; }
   for (int c = 0; c < 8; ++c) mark_subtree(b->kids[c], mark);
 }
 
 /* leaves of b's subtree that touch the cell across face f of b's same-level
  * neighbour: the kids on the side of b facing that cell, recursively */
 static void mark_face(const BRANCH *b, int f, uint8_t *mark) {
-  if (!b->nkids) { mark[b->fv_index] = 1; return; }
+  if (!b->nkids) { mark[IDX(b)] = 1; return; }
   int a = FV_FACE_AXIS[f];
   for (int k = 0; k < 4; ++k) mark_face(b->kids[FV_FACE_NODES[f][k] ^ (1 << a)], f, mark);
 }
@@ -866,7 +881,7 @@ static int rebuild_changed(struct fvh_forest *fr, const bvec *chg) {
       } else if (p->level > 1) {
         int m[3] = {n[0] >> 1, n[1] >> 1, n[2] >> 1};
         const BRANCH *cb = ht_find(fr, p->level - 1, m);
-        if (cb && !cb->nkids) mark[cb->fv_index] = 1;
+        if (cb && !cb->nkids) mark[IDX(cb)] = 1;
       }
     }
   }
@@ -898,10 +913,8 @@ int fvh_forest_create(const int nroot[3], const double x0[3], const double x1[3]
   for (int a = 0; a < 3; ++a) {
     fr->nroot[a] = nroot[a];
     fr->periodic[a] = periodic ? (periodic[a] != 0) : 0;
-    fr->topo.fv_nroot[a] = nroot[a];
-    fr->topo.fv_x0[a] = x0[a];
-    fr->topo.fv_x1[a] = x1[a];
-    fr->topo.fv_period[a] = fr->periodic[a] ? (x1[a] - x0[a]) : 0.0;
+    fr->x0[a] = x0[a];
+    fr->x1[a] = x1[a];
   }
   for (int s = 0; s < 6; ++s) fr->bc[s] = bc[s];
   fr->run.con = &fr->con;
@@ -972,8 +985,8 @@ int fvh_forest_geometry(fvh_forest *fr, int nroot[3], double x0[3], double x1[3]
   if (!fr) return FVG_ERR_ARG;
   for (int a = 0; a < 3; ++a) {
     if (nroot) nroot[a] = fr->nroot[a];
-    if (x0) x0[a] = fr->topo.fv_x0[a];
-    if (x1) x1[a] = fr->topo.fv_x1[a];
+    if (x0) x0[a] = fr->x0[a];
+    if (x1) x1[a] = fr->x1[a];
     if (periodic) periodic[a] = fr->periodic[a];
   }
   for (int s = 0; s < 6 && bc; ++s) bc[s] = fr->bc[s];
@@ -999,9 +1012,9 @@ static void keys_part(void *v, int lo, int hi) {
   for (int i = lo; i < hi; ++i) {
     const BRANCH *b = k->lv[i];
     k->lev[i] = b->level;
-    k->ijk[3 * i] = b->fv_ijk[0];
-    k->ijk[3 * i + 1] = b->fv_ijk[1];
-    k->ijk[3 * i + 2] = b->fv_ijk[2];
+    k->ijk[3 * i] = IJK(b)[0];
+    k->ijk[3 * i + 1] = IJK(b)[1];
+    k->ijk[3 * i + 2] = IJK(b)[2];
   }
 }
 
@@ -1296,8 +1309,8 @@ int fvh_forest_save(fvh_forest *fr, const char *path, int32_t tstep, double time
   for (int a = 0; a < 3; ++a) {
     h.nroot[a] = fr->nroot[a];
     h.periodic[a] = fr->periodic[a];
-    h.x0[a] = fr->topo.fv_x0[a];
-    h.x1[a] = fr->topo.fv_x1[a];
+    h.x0[a] = fr->x0[a];
+    h.x1[a] = fr->x1[a];
   }
   for (int s = 0; s < 6; ++s) h.bc[s] = fr->bc[s];
   h.tstep = tstep;
@@ -1428,7 +1441,7 @@ int fvh_forest_load_text(fvh_forest *fr, const char *path, int32_t *tstep, doubl
     uint8_t *seen = (uint8_t *)calloc((size_t)nl, 1);
     if (!seen) rc = FVG_ERR_NOMEM;
     for (size_t q = 0; q < got.n && !rc; ++q) {
-      int i = got.p[q]->fv_index;
+      int i = IDX(got.p[q]);
       if (got.p[q]->nkids || i < 0 || i >= nl || seen[i]) rc = FVG_ERR_ARG;
       else seen[i] = 1;
     }

@@ -334,14 +334,18 @@ int fvg_get_state_device(fvg_ctx *ctx, void *dS);
 /* Flatten `run`, create a context on `device` and upload S; afterwards RHS(run), RK(irk,run),
  * faceinterp(run) and gradcalc(run) run on the GPU.  compat != 0 copies every output back
  * into the leaf arrays after each call (reference-shaped callers); compat == 0 keeps the
- * state on the device until fv_gpu_download(run). */
+ * state on the device until fv_gpu_download(run).  `run` may come from the reference's
+ * host code (its structs have the reference's layout, fv_strdata.h): the binding keeps
+ * its state in run->info, a field the reference declares and never uses; attach
+ * overwrites it without reading it, detach clears it, and the four entry points on a RUN
+ * whose info is NULL do nothing but report FVG_ERR_STATE. */
 int fv_gpu_attach(RUN *run, int device, const fvg_params *p, int compat);
 int fv_gpu_detach(RUN *run);
 int fv_gpu_remesh(RUN *run);      /* after a mesh adaptation: re-flatten and upload S */
 int fv_gpu_upload(RUN *run);      /* leaves -> device S */
 int fv_gpu_download(RUN *run);    /* device S, SN, RHS -> leaves */
 fvg_ctx *fv_gpu_context(RUN *run);
-int fv_gpu_last_status(void);     /* status of the last RHS/RK/faceinterp/gradcalc call */
+int fv_gpu_last_status(RUN *run); /* status of the last RHS/RK/faceinterp/gradcalc call on run */
 
 #ifdef __cplusplus
 }

@@ -32,12 +32,14 @@ def lib() -> C.CDLL:
             raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
         L = C.CDLL(os.fspath(LIB_PATH))
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
-        for name in ("orc_faceinterp", "orc_gradcalc", "orc_rhs"):
+        for name in ("orc_gradcalc", "orc_rhs"):
             getattr(L, name).argtypes = [vp, vp]
             getattr(L, name).restype = C.c_int
+        L.orc_faceinterp.argtypes = [vp, vp, dp]
+        L.orc_faceinterp.restype = C.c_int
         L.orc_rk.argtypes = [vp, C.c_int, C.c_double]
         L.orc_rk.restype = C.c_int
-        L.orc_step.argtypes = [vp, vp, C.c_int, C.c_double]
+        L.orc_step.argtypes = [vp, vp, dp, C.c_int, C.c_double]
         L.orc_step.restype = C.c_int
         L.orc_totals.argtypes = [vp, dp]
         L.orc_totals.restype = C.c_int
@@ -54,6 +56,14 @@ def lib() -> C.CDLL:
         L.orc_limited_slope.restype = C.c_double
         L.orc_last_bad_cell.argtypes = []
         L.orc_last_bad_cell.restype = C.c_int
+        L.orc_copy_run.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.orc_copy_run.restype = C.c_int
+        L.orc_copy_guards.argtypes = [vp]
+        L.orc_copy_guards.restype = C.c_int
+        L.orc_copy_free.argtypes = [vp]
+        L.orc_copy_free.restype = None
+        L.orc_run_field.argtypes = [vp, C.c_int, dp, C.c_int]
+        L.orc_run_field.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
         _lib = L
@@ -83,8 +93,15 @@ def _d(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _period(forest) -> np.ndarray:
+    """Length of each periodic axis of the forest's box (0 where not periodic)."""
+    g = forest.geometry()
+    return np.array([(b - a) if per else 0.0 for a, b, per in zip(g["x0"], g["x1"], g["periodic"])])
+
+
 def faceinterp(forest, p) -> None:
-    _ck(lib().orc_faceinterp(forest.run, _p(p)), "orc_faceinterp")
+    per = _period(forest)
+    _ck(lib().orc_faceinterp(forest.run, _p(p), _d(per)), "orc_faceinterp")
 
 
 def gradcalc(forest, p) -> None:
@@ -100,7 +117,8 @@ def rk(forest, irk: int, dt: float) -> None:
 
 
 def step(forest, p, method: int, dt: float) -> None:
-    _ck(lib().orc_step(forest.run, _p(p), int(method), float(dt)), "orc_step")
+    per = _period(forest)
+    _ck(lib().orc_step(forest.run, _p(p), _d(per), int(method), float(dt)), "orc_step")
 
 
 def totals(forest) -> np.ndarray:
@@ -161,3 +179,38 @@ def limited_slope(limiter: int, eps: float, boundary_face: bool, delta0: float, 
                   dp1cc: float, dm1cc: float) -> float:
     return lib().orc_limited_slope(int(limiter), float(eps), int(bool(boundary_face)), float(delta0),
                                    float(delta1), float(dp1cc), float(dm1cc))
+
+
+class ReferenceRun:
+    """A reference-shaped deep copy of a forest's RUN (oracle/orc_runcopy.c): every struct
+    and array in an allocation of exactly its reference size followed by guard bytes, the
+    RUN's other fields garbage.  `run` is its RUN*; guards() counts allocations whose guard
+    bytes were overwritten."""
+
+    def __init__(self, forest):
+        self.neq = forest.neq
+        self.n = forest.nleaves
+        h, r = C.c_void_p(), C.c_void_p()
+        _ck(lib().orc_copy_run(forest.run, C.byref(h), C.byref(r)), "orc_copy_run")
+        self._h, self.run = h, r
+
+    def guards(self) -> int:
+        return lib().orc_copy_guards(self._h)
+
+    def get_field(self, which: int) -> np.ndarray:
+        shape = (6, self.neq, self.n) if which == 3 else (self.neq, self.n)
+        a = np.empty(shape)
+        _ck(lib().orc_run_field(self.run, int(which), _d(a), 0), "orc_run_field")
+        return a
+
+    def set_field(self, which: int, a) -> None:
+        a = np.ascontiguousarray(a, np.float64)
+        _ck(lib().orc_run_field(self.run, int(which), _d(a), 1), "orc_run_field")
+
+    def close(self) -> None:
+        if self._h:
+            lib().orc_copy_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -340,8 +340,10 @@ static void orc_reconstruct(OCTX *o, BRANCH *crnt, int f, double dp1cf, double d
 
 static const int OFCOPP[6] = {2, 3, 0, 1, 5, 4};
 
-/* faceinterp.c:21-61 */
-int orc_faceinterp(RUN *run, const fvg_params *p) {
+/* faceinterp.c:21-61; period[a] > 0 is the length of a periodic axis (an extension: the
+ * distance to a neighbour across the periodic side is measured to its image), NULL or 0
+ * for none */
+int orc_faceinterp(RUN *run, const fvg_params *p, const double *period) {
   int n, err = 0, bad = -1;
   BRANCH **L = leaf_array(run, &n);
   if (!L) return ORC_ERR_ARG;
@@ -351,7 +353,7 @@ int orc_faceinterp(RUN *run, const fvg_params *p) {
     memset(&o, 0, sizeof(o));
     o.p = p;
     o.neq = run->con->neq;
-    o.period = run->topo->fv_period;
+    o.period = period;
 #pragma omp for schedule(static)
     for (int q = 0; q < n; ++q) {
       BRANCH *c = L[q];
@@ -362,7 +364,7 @@ int orc_faceinterp(RUN *run, const fvg_params *p) {
         orc_distance(&o, c, fo, &m1cf, &m1cc);
         int rc = orc_deltavalues(&o, c, o.delta1, fo);
         if (!rc) rc = orc_deltavalues(&o, c, o.delta0, f);
-        if (rc) { note_failure(rc, c->fv_index, &err, &bad); break; }
+        if (rc) { note_failure(rc, q, &err, &bad); break; }
         orc_reconstruct(&o, c, f, p1cf, p1cc, m1cf, m1cc);
       }
     }
@@ -403,7 +405,7 @@ int orc_gradcalc(RUN *run, const fvg_params *p) {
             if (!rc) rc = orc_bc(&o, b->cl->fc[f].bc, &o.sol1);
           }
         }
-        if (rc) { note_failure(rc, b->fv_index, &err, &bad); failed = 1; break; }
+        if (rc) { note_failure(rc, iq, &err, &bad); failed = 1; break; }
         CELL *cl = b->cl;
         for (int v = 0; v < neq; ++v) {
           double m = 0.5 * (o.sol1.vec[v] + o.sol.vec[v]);
@@ -652,7 +654,7 @@ int orc_rhs(RUN *run, const fvg_params *p) {
 #pragma omp for schedule(static)
     for (int q = 0; q < n; ++q) {
       int r = orc_residual(&o, L[q]);
-      if (r) note_failure(r, L[q]->fv_index, &err, &bad);
+      if (r) note_failure(r, q, &err, &bad);
     }
   }
   free(L);
@@ -700,13 +702,13 @@ int orc_rk(RUN *run, int irk, double dt) {
 }
 
 /* the natural reading of the undefined advancetimeexplicit (loop.c:48) */
-int orc_step(RUN *run, const fvg_params *p, int method, double dt) {
+int orc_step(RUN *run, const fvg_params *p, const double *period, int method, double dt) {
   static const int st3[3] = {1, 2, 3}, st2[2] = {1, 4}, st1[1] = {1};
   const int *st = method == 3 ? st3 : (method == 2 ? st2 : st1);
   int ns = method == 3 ? 3 : (method == 2 ? 2 : 1);
   int rc = orc_rk(run, 0, dt);
   for (int k = 0; k < ns && !rc; ++k) {
-    if (p->ischeme == 2) rc = orc_faceinterp(run, p);
+    if (p->ischeme == 2) rc = orc_faceinterp(run, p, period);
     if (!rc && p->ns == 1) rc = orc_gradcalc(run, p);
     if (!rc) rc = orc_rhs(run, p);
     if (!rc) rc = orc_rk(run, st[k], dt);
@@ -732,14 +734,15 @@ int orc_cfl_dt(RUN *run, const fvg_params *p, double cfl, double *dt) {
   o.p = p;
   o.neq = run->con->neq;
   double m = 0.0;
-  for (BRANCH *c = run->topo->locl; c; c = c->lnxt) {
+  int q = 0;
+  for (BRANCH *c = run->topo->locl; c; c = c->lnxt, ++q) {
     int rc = orc_fill(&o, &o.sol, c->el->S, 0);
     if (rc) return rc;
     const ND *n = c->cl->nd;
     double hx = n[1].x - n[0].x, hy = n[2].y - n[0].y, hz = n[4].z - n[0].z;
     double r = (fabs(o.sol.u) + o.sol.c) / hx + (fabs(o.sol.v) + o.sol.c) / hy +
                (fabs(o.sol.w) + o.sol.c) / hz;
-    if (!(r == r)) { g_last_bad = c->fv_index; return ORC_ERR_NONFINITE; }
+    if (!(r == r)) { g_last_bad = q; return ORC_ERR_NONFINITE; }
     if (r > m) m = r;
   }
   *dt = cfl / m;

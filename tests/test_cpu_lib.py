@@ -195,29 +195,29 @@ def test_unsupported_boundary_rejected():
         fv.FlatMesh(F.run)
 
 
-class _CON(C.Structure):
-    """the leading fields of CON (fv_strdata.h), up to iprimtv"""
-    _fields_ = [("level", C.c_int), ("ns", C.c_int), ("turb", C.c_int), ("i_case", C.c_int),
-                ("nprimitiv", C.c_int), ("ymin", C.c_double), ("model", C.c_int), ("multimat", C.c_int),
-                ("nstep", C.c_int), ("dt", C.c_double), ("eos", C.c_int), ("verbose", C.c_int),
-                ("halfdt", C.c_int), ("limiter", C.c_int), ("istep", C.c_int), ("tstep", C.c_int),
-                ("time", C.c_double), ("rank", C.c_int), ("size", C.c_int), ("neq", C.c_int),
-                ("neq_temp", C.c_int), ("ischeme", C.c_int), ("iprimtv", C.c_int), ("method", C.c_int)]
+def _layout(struct, field):
+    """Byte offset of a field of the reference's struct layout (tests/golden fixture)."""
+    import json
+    import pathlib
+    d = json.load(open(pathlib.Path(__file__).resolve().parent / "golden" / "strdata_layout.json"))
+    return d["layout"][f"{struct}.{field}"][0]
 
 
-class _RUN(C.Structure):
-    _fields_ = [("con", C.POINTER(_CON)), ("par", C.c_void_p), ("topo", C.c_void_p)]
+def _con_int(run_ptr, field):
+    """A C.c_int view of an int field of the RUN's CON, addressed by the reference layout."""
+    con = C.c_void_p.from_address(run_ptr.value + _layout("RUN", "con")).value
+    return C.c_int.from_address(con + _layout("CON", field))
 
 
 def test_primitive_reconstruction_refused():
     """iprimtv = 1 is refused before any device call: the reference's cons2primtv reads
     a density its loader never set, so there is no defined result to match."""
     F = uniform(2, 5)
-    run = C.cast(F.run, C.POINTER(_RUN)).contents
-    assert run.con.contents.neq == 5 and run.con.contents.iprimtv == 0
-    run.con.contents.iprimtv = 1
+    neq, iprimtv = _con_int(F.run, "neq"), _con_int(F.run, "iprimtv")
+    assert neq.value == 5 and iprimtv.value == 0
+    iprimtv.value = 1
     try:
         rc = fv.lib().fv_gpu_attach(F.run, 0, C.byref(fv.params(1)), 1)
         assert rc == _capi.FVG_ERR_UNSUPPORTED
     finally:
-        run.con.contents.iprimtv = 0
+        iprimtv.value = 0

@@ -200,13 +200,45 @@ def test_run_binding_compat():
         L.RK(0, run)
         L.faceinterp(run)
         L.RHS(run)
-        assert L.fv_gpu_last_status() == 0
+        assert L.fv_gpu_last_status(run) == 0
         got = F.get_field(fv.FIELD_RHS)
         F.set_state(S0)
         ref = oracle.full_rhs(F, p)
         assert np.all(rel_linf(got, ref, 5) <= TOL)
     finally:
         L.fv_gpu_detach(run)
+
+
+def test_run_binding_reference_shaped_run():
+    """RHS(run) on a RUN laid out as the reference's host code builds it: every struct a
+    malloc of exactly its reference size with guard bytes behind (oracle.ReferenceRun),
+    the RUN's other fields garbage (including info, which fv_gpu_attach overwrites
+    without reading).  The residual lands in the copy's leaves, equal to the oracle's,
+    and no byte past any reference struct is written."""
+    print("case RUN binding, reference-shaped RUN", flush=True)
+    F = amr(3, 5)
+    xc, _ = F.leaf_info()
+    F.set_state(state(1, xc, seed=19))
+    p = fv.params(1)
+    ref = oracle.full_rhs(F, p)
+    R = oracle.ReferenceRun(F)
+    L = fv.lib()
+    assert L.fv_gpu_attach(R.run, 0, C.byref(p), 1) == 0
+    try:
+        L.RK(0, R.run)
+        L.faceinterp(R.run)
+        L.RHS(R.run)
+        assert L.fv_gpu_last_status(R.run) == 0
+        got = R.get_field(fv.FIELD_RHS)
+        assert np.all(rel_linf(got, ref, 5) <= TOL)
+        sf = R.get_field(fv.FIELD_SF)
+        sf_ref = F.get_field(fv.FIELD_SF)
+        assert np.all(rel_linf(sf.reshape(30, -1), sf_ref.reshape(30, -1), 5) <= TOL)
+    finally:
+        assert L.fv_gpu_detach(R.run) == 0
+    assert L.fv_gpu_last_status(R.run) != 0  # detached: info cleared
+    assert R.guards() == 0
+    R.close()
 
 
 def test_nonfinite_reported():
