@@ -19,8 +19,10 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #define _DEFAULT_SOURCE
+#define _GNU_SOURCE
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -117,10 +119,13 @@ static void pool_start(int want) {
   }
 }
 
+/* the CPUs this process may run on (its affinity mask, which cgroup CPU sets narrow),
+ * at most FVH_THREADS */
 static int host_threads(void) {
   static int n = 0;
   if (!n) {
-    long c = sysconf(_SC_NPROCESSORS_ONLN);
+    cpu_set_t cs;
+    long c = sched_getaffinity(0, sizeof(cs), &cs) == 0 ? CPU_COUNT(&cs) : sysconf(_SC_NPROCESSORS_ONLN);
     n = c < 1 ? 1 : (c > FVH_THREADS ? FVH_THREADS : (int)c);
   }
   return n;
@@ -1257,7 +1262,9 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
     if (!c->nkids) rc = split_queue(fr, c, &chg, &work);
   }
   free(work.p);
-  if (rc) { free(cand); free(chg.p); return rc; }
+  /* after a failure some splits may be linked already: relist the leaves so that the
+   * forest stays consistent (balanced or not) for the caller's next step */
+  if (rc) { free(cand); free(chg.p); (void)rebuild(fr); return rc; }
   /* decide every merge on the same (post-split) tree, then apply: the decisions and the
    * means on the thread pool, the unlinking in order */
   int nm = 0;
@@ -1280,7 +1287,7 @@ int fvh_forest_adapt(fvh_forest *fr, const int8_t *flag, int max_level) {
   /* fr->leaves still lists freed kids of merged parents: rebuild before any use */
   if (!rc) rc = rebuild_changed(fr, &chg);
   free(chg.p);
-  if (rc) return rc;
+  if (rc) { (void)rebuild(fr); return rc; }
   pc.lv = fr->leaves;
   par_for(fr->nleaves, 8192, clear_part, &pc);
   return fr->nleaves;
