@@ -1,9 +1,9 @@
 """Benchmark of the RHS + RK hot path on one MI355X.
 
 Headline workload (`value`): the compressible Navier-Stokes Taylor-Green vortex on
-a uniform 128^3 periodic octree (2,097,152 leaves; Re 1600, Mach 0.1, MUSCL-minmod
-+ HLLC, Green-Gauss gradients and viscous fluxes), the largest single-GPU
-configuration.  K SSP-RK3 steps are timed with every input resident in HBM; a step
+a uniform 128^3 periodic octree (2,097,152 leaves; Re 1600, Mach 0.1, Prandtl 0.71,
+MUSCL-minmod + HLLC, Green-Gauss gradients, viscous fluxes and heat conduction), the
+largest single-GPU configuration.  K SSP-RK3 steps are timed with every input resident in HBM; a step
 is RK(0) + 3 x (gradcalc, faceinterp, RHS, RK(k)) and the metric is cell-updates/s =
 leaves x RK stages / second.  The plain FP64 form gives `value`; the mixed
 FP32-flux / FP64-accumulate form is timed on the same state and reported beside it
@@ -360,7 +360,7 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
                       else "first order") + " + HLLC(Davis), SSP-RK3", "dt": dt,
            **{k: v for k, v in meta.items() if k != "name"}}
     if p.ns:
-        cfg.update(mu=p.mu, viscous_work=bool(p.visc_work))
+        cfg.update(mu=p.mu, viscous_work=bool(p.visc_work), prandtl=p.prandtl or None)
     out = dict(value=round(val_p, 1), ms_per_step=round(wall_p / a.steps * 1e3, 4), config=cfg,
                mixed={"value": round(val_m, 1), "ms_per_step": round(wall_m / a.steps * 1e3, 4),
                       "speedup_vs_plain": round(val_m / val_p, 3), "l1_rel_err_vs_plain": acc},

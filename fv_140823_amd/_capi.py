@@ -23,6 +23,7 @@ FVG_ERR_IO = -7
 
 FVG_PLAIN, FVG_MIXED = 0, 1
 FVG_MINMOD, FVG_VANALBADA = 0, 1
+FVG_WAVES_DAVIS, FVG_WAVES_PVRS = 0, 1
 FIELD_S, FIELD_SN, FIELD_RHS, FIELD_SF, FIELD_SG, FIELD_FACEFLUX, FIELD_STAGE_IN = range(7)
 
 ERR_NAMES = {
@@ -61,6 +62,9 @@ class FvgParams(C.Structure):
         ("mix_cmix", C.c_double),
         ("inlet", C.c_double * 5),
         ("vanalbada_eps", C.c_double),
+        ("iprimtv", C.c_int),
+        ("wave_speeds", C.c_int),
+        ("prandtl", C.c_double),
     ]
 
 

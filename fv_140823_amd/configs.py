@@ -9,7 +9,7 @@ e = (p + gamma pinf) / (rho (gamma - 1))).
   sod        uniform 2^L cube, Sod tube along x, transmissive sides
   blast      base 2^L cube + 2 AMR levels around an overpressured sphere
   cavitation [-1,1]^3, model 2 barotropic liquid with a vapour bubble
-  tgv        periodic [0,2 pi]^3 Taylor-Green vortex, Navier-Stokes
+  tgv        periodic [0,2 pi]^3 Taylor-Green vortex, Navier-Stokes, Prandtl 0.71
   vortex     periodic 10^3 isentropic vortex, base level + finer core patch
 """
 from __future__ import annotations
@@ -132,7 +132,10 @@ def cavitation(level: int = 7, radius: float = 0.2, ratio: float = 100.0):
     return F, p, dict(name=f"cavitation_{2**level}^3", steps=500, cfl=0.5, method=3)
 
 
-def tgv(level: int = 7, Re: float = 1600.0, Mach: float = 0.1, gamma: float = 1.4):
+def tgv(level: int = 7, Re: float = 1600.0, Mach: float = 0.1, gamma: float = 1.4,
+        prandtl: float = 0.71):
+    """Taylor-Green vortex, Navier-Stokes with viscous work and heat conduction at
+    Prandtl `prandtl` (0: none, as flux_viscous.c)."""
     L = 2.0 * math.pi
     F = Forest((1, 1, 1), (0, 0, 0), (L, L, L), level, (0, 0, 0, 0, 0, 0), (1, 1, 1), 5)
     xc, _ = F.leaf_info()
@@ -144,7 +147,7 @@ def tgv(level: int = 7, Re: float = 1600.0, Mach: float = 0.1, gamma: float = 1.
     p = p0 + (1.0 / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
     rho = np.ones_like(u)
     F.set_state(prim_to_cons(rho, u, v, w, p, gamma))
-    return F, params(1, ischeme=2, ns=1, mu=1.0 / Re, visc_work=1, gamma=gamma), dict(
+    return F, params(1, ischeme=2, ns=1, mu=1.0 / Re, visc_work=1, gamma=gamma, prandtl=prandtl), dict(
         name=f"tgv_{2**level}^3", steps=1000, cfl=0.5, method=3)
 
 

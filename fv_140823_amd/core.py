@@ -34,6 +34,8 @@ def params(model: int = 1, **kw) -> FvgParams:
             v = {"plain": FVG_PLAIN, "mixed": FVG_MIXED}[v]
         if k == "limiter" and isinstance(v, str):
             v = {"minmod": FVG_MINMOD, "vanalbada": FVG_VANALBADA}[v]
+        if k == "wave_speeds" and isinstance(v, str):
+            v = {"davis": 0, "pvrs": 1}[v]
         if k == "inlet":
             arr = (C.c_double * 5)(*([float(x) for x in v] + [0.0] * (5 - len(v))))
             p.inlet = arr

@@ -258,6 +258,104 @@ __device__ __forceinline__ void recon_regular(const DevMesh &M, const DevParams 
   }
 }
 
+/* ---- MUSCL on primitive variables (option iprimtv = 1): the reference's path of
+ * deltavalues.c:16-19 (deltas of cons2primtv of both states, the finer neighbours'
+ * conserved states averaged first as facevalues_MUSCL.c:31-47 does), the limiter on
+ * them (reconctruct_V3.c:11-41) and prim2conc.c of the face value.  cons2primtv here
+ * takes the density of the state itself and the kinetic energy from its velocities; the
+ * pair keeps the reference's p = rho (gamma - 1) e - p_inf and e = (p + p_inf) /
+ * (rho (gamma - 1)), which invert each other. */
+template <int NEQ>
+__device__ __forceinline__ void cons_to_prim(const DevParams &P, const double *U, double *W) {
+  const double r = U[0];
+  W[0] = r;
+  W[1] = U[1] / r;
+  W[2] = U[2] / r;
+  W[3] = U[3] / r;
+  if constexpr (NEQ == 5) {
+    if (P.model == 1) {
+      const double ei = (U[4] / r) - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+      W[4] = r * (P.gamma - 1.0) * ei - P.pinf;
+    } else {
+      W[4] = U[4] / r;
+    }
+  }
+}
+
+template <int NEQ>
+__device__ __forceinline__ void prim_to_cons(const DevParams &P, const double *W, double *U) {
+  const double r = W[0];
+  U[0] = r;
+  U[1] = r * W[1];
+  U[2] = r * W[2];
+  U[3] = r * W[3];
+  if constexpr (NEQ == 5) {
+    if (P.model == 1) {
+      const double ei = (W[4] + P.pinf) / (r * (P.gamma - 1.0));
+      U[4] = r * (ei + 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
+    } else {
+      U[4] = r * W[4];
+    }
+  }
+}
+
+/* recon_face on primitive variables */
+template <int NEQ, typename T>
+__device__ __forceinline__ void recon_face_prim(const DevMesh &M, const DevParams &P,
+                                                const double *__restrict__ S, int c, int f,
+                                                double *out) {
+  const uint32_t cf = M.cflag[c];
+  const int fo = f ^ (f < 4 ? 2 : 1);
+  double U[NEQ], U0[NEQ], U1[NEQ], W[NEQ], W0[NEQ], W1[NEQ], Wf[NEQ];
+  load_S<NEQ>(S, M.ns, c, U);
+  nb_state<NEQ>(M, P, S, c, cf, f, U, U0);
+  nb_state<NEQ>(M, P, S, c, cf, fo, U, U1);
+  cons_to_prim<NEQ>(P, U, W);
+  cons_to_prim<NEQ>(P, U0, W0);
+  cons_to_prim<NEQ>(P, U1, W1);
+  double pcf, pcc, mcf, mcc, pic, mic;
+  face_dist(M, cf, f, pcf, pcc, pic);
+  face_dist(M, cf, fo, mcf, mcc, mic);
+  const bool bnd = ((cf >> (2 * f)) & 3) == CLS_BOUNDARY;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const T g = limited<T>(P.limiter, P.va_eps, bnd, (T)(W0[v] - W[v]), (T)(W1[v] - W[v]), (T)pic,
+                           (T)mic);
+    Wf[v] = W[v] + (double)((T)pcf * g);
+  }
+  prim_to_cons<NEQ>(P, Wf, out);
+}
+
+/* recon_regular on primitive variables */
+template <int NEQ, typename T>
+__device__ __forceinline__ void recon_regular_prim(const DevMesh &M, const DevParams &P,
+                                                   const double *__restrict__ S, int j, int iL,
+                                                   int iR, int lev, int a, double *UL, double *UR) {
+  const int k = (lev * 3 + a) * 3;
+  const double pcf = M.dist[2 * k], ic = M.idist[k];
+  double A0[NEQ], A1[NEQ], B0[NEQ], B1[NEQ], WL[NEQ], WR[NEQ];
+  load_S<NEQ>(S, M.ns, M.fLL[j], A0);
+  load_S<NEQ>(S, M.ns, iL, A1);
+  load_S<NEQ>(S, M.ns, iR, B0);
+  load_S<NEQ>(S, M.ns, M.fRR[j], B1);
+  double PA0[NEQ], PA1[NEQ], PB0[NEQ], PB1[NEQ];
+  cons_to_prim<NEQ>(P, A0, PA0);
+  cons_to_prim<NEQ>(P, A1, PA1);
+  cons_to_prim<NEQ>(P, B0, PB0);
+  cons_to_prim<NEQ>(P, B1, PB1);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(PB0[v] - PA1[v]), (T)(PA0[v] - PA1[v]),
+                            (T)ic, (T)ic);
+    const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(PA1[v] - PB0[v]), (T)(PB1[v] - PB0[v]),
+                            (T)ic, (T)ic);
+    WL[v] = PA1[v] + (double)((T)pcf * gL);
+    WR[v] = PB0[v] + (double)((T)pcf * gR);
+  }
+  prim_to_cons<NEQ>(P, WL, UL);
+  prim_to_cons<NEQ>(P, WR, UR);
+}
+
 /* ---- FP32 reconstruction of the mixed form (ideal gas): the gather keeps an
  * FP32 copy Sf of the cell-major state beside the FP64 one, and the face kernel
  * reconstructs from it (half the bytes of every gathered state). */

@@ -53,6 +53,10 @@ struct DevParams {
   double mix_RG, mix_Tref, mix_rsat, mix_psat, mix_cliq, mix_cmix;
   double inlet[5];
   double va_eps;
+  /* options (fvg_params: iprimtv, wave_speeds, prandtl); read only by the kernels'
+   * option variants */
+  int iprimtv, pvrs;
+  double kheat; /* mu gamma / ((gamma - 1) Pr), 0 without heat conduction */
 };
 
 template <int MODEL> struct ModelTraits;
@@ -65,6 +69,7 @@ template <typename T> struct FState {
   T r, u[3], e, p, c, y;
   T q[5]; /* conserved vector (SOL::vec) */
   T g[9]; /* gradients of rho u, rho v, rho w (SG rows 1..3) */
+  T gt[3]; /* gradient of p / rho (heat conduction option) */
 };
 
 /* EOS of a conserved vector, always in FP64 (returns false on a failed root) */
@@ -150,9 +155,38 @@ FV_HD __forceinline__ bool make_state(const DevParams &P, const double *U, bool 
   return ok;
 }
 
+/* Option bits of the kernels' option variants (fvg_params iprimtv, wave_speeds, prandtl):
+ * a variant compiles only the code of its bits, and checks the runtime switches of those
+ * bits, so that one instantiation with all bits serves every combination. */
+enum { OPT_PRIM = 1, OPT_PVRS = 2, OPT_HEAT = 4, OPT_ALL = 7 };
+
+/* PVRS wave speeds (Toro, Riemann Solvers, 3rd ed., 10.5.2): the star pressure of the
+ * linearised Riemann problem, p* = max(0, (pL + pR)/2 - (uR - uL) rho_ avg a_avg / 2)
+ * with arithmetic means of density and sound speed, then S_L = uL - aL qL and
+ * S_R = uR + aR qR, q_K = 1 where p* <= p_K (rarefaction) and
+ * sqrt(1 + (gamma + 1) / (2 gamma) (p* / p_K - 1)) otherwise (the Rankine-Hugoniot shock
+ * speed).  Stiffened gas: every pressure is taken as p + p_inf.  uL, uR are the normal
+ * velocities. */
+template <typename T>
+FV_HD __forceinline__ void pvrs_speeds(const DevParams &P, const FState<T> &L, const FState<T> &R,
+                                       T uL, T uR, T &SL, T &SR) {
+  const T g = (T)P.gamma, pinf = (T)P.pinf;
+  const T pL = L.p + pinf, pR = R.p + pinf;
+  const T rb = (T)0.5 * (L.r + R.r), ab = (T)0.5 * (L.c + R.c);
+  T ps = (T)0.5 * (pL + pR) - (T)0.5 * (uR - uL) * rb * ab;
+  ps = ps > (T)0 ? ps : (T)0;
+  const T k = qdiv(g + (T)1, (T)2 * g);
+  const T qL = ps <= pL ? (T)1 : qsqrt((T)1 + k * (qdiv(ps, pL) - (T)1));
+  const T qR = ps <= pR ? (T)1 : qsqrt((T)1 + k * (qdiv(ps, pR) - (T)1));
+  SL = uL - L.c * qL;
+  SR = uR + R.c * qR;
+}
+
 /* Davis-HLLC flux of (L, R) across a face of normal s*e_a, plus the viscous flux;
- * returns out[v] = (F_adv - F_visc).n + HLLC correction (before the -Area factor). */
-template <int MODEL, bool NS, int a, typename T>
+ * returns out[v] = (F_adv - F_visc).n + HLLC correction (before the -Area factor).
+ * OPTS: option bits; OPT_PVRS honours P.pvrs (PVRS wave speeds, model 1), OPT_HEAT
+ * P.kheat (heat conduction, model 1 with NS). */
+template <int MODEL, bool NS, int a, typename T, int OPTS = 0>
 FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
                                           const FState<T> &R, T s, T *out) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
@@ -161,6 +195,9 @@ FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
   T ruL = L.r * uL, ruR = R.r * uR;
   T SR = (uL + L.c) > (uR + R.c) ? (uL + L.c) : (uR + R.c);
   T SL = (uL - L.c) < (uR - R.c) ? (uL - L.c) : (uR - R.c);
+  if constexpr ((OPTS & OPT_PVRS) && MODEL == 1) {
+    if (P.pvrs) pvrs_speeds<T>(P, L, R, uL, uR, SL, SR);
+  }
   T SS = qdiv(((L.r * (uL * uL) + L.p) - (R.r * (uR * uR) + R.p) - (SL * ruL) + (SR * ruR)),
               (ruL - ruR - (SL * L.r) + (SR * R.r)));
   int region = 0;
@@ -212,6 +249,11 @@ FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
      * the viscous work from it too */
     if constexpr (MODEL == 1 || MODEL == 3) {
       if (P.visc_work) F[4] = F[4] - (uf[0] * tau[a][0] + uf[1] * tau[a][1] + uf[2] * tau[a][2]);
+    }
+    /* heat conduction (option): the energy flux loses q_a = -kheat d(p/rho)/dx_a, with the
+     * face mean of the two cells' gradients of p/rho */
+    if constexpr ((OPTS & OPT_HEAT) && MODEL == 1) {
+      if (P.kheat != 0.0) F[4] = F[4] - (T)P.kheat * ((T)0.5 * (L.gt[a] + R.gt[a]));
     }
   }
   T corr[NEQ];

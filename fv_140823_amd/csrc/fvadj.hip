@@ -656,6 +656,9 @@ static int adj_check(fvg_ctx *c) {
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");
   if (c->prm.model != 1)
     return set_err(c, FVG_ERR_UNSUPPORTED, "adjoint/tangent: model 1 (ideal/stiffened gas) only");
+  if (c->prm.iprimtv || c->prm.wave_speeds != FVG_WAVES_DAVIS || c->prm.prandtl > 0.0)
+    return set_err(c, FVG_ERR_UNSUPPORTED,
+                   "adjoint/tangent: conserved-variable MUSCL, Davis speeds, no heat conduction");
   return FVG_OK;
 }
 

@@ -26,13 +26,24 @@
 
 /* ------------------------------------------------------------------ kernels */
 
-template <int MODEL, typename R>
+template <int MODEL, typename R, bool PRIM = false>
 __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
                                                     const double *__restrict__ S,
                                                     R *__restrict__ SF) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const int i = logical_block() * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
+  if constexpr (PRIM) {
+    /* MUSCL on primitive variables (option): face by face */
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      double out[NEQ];
+      recon_face_prim<NEQ, double>(M, P, S, i, f, out);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) SF[((size_t)f * NEQ + v) * M.ns + i] = (R)out[v];
+    }
+    return;
+  }
   const uint32_t cf = M.cflag[i];
   double U[NEQ];
   load_S<NEQ>(S, M.ns, i, U);
@@ -61,13 +72,25 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
   }
 }
 
+/* theta = p / rho of an ideal / stiffened-gas state, (gamma - 1) e_int - gamma p_inf / rho
+ * (the p of facevalues.c:40 divided by rho), without the sound speed */
+__device__ __forceinline__ double theta_ideal(const DevParams &P, const double *U) {
+  const double ir = 1.0 / U[0];
+  const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  const double ei = (U[4] * ir) - 0.5 * (u * u + v * v + w * w);
+  return (P.gamma - 1.0) * ei - P.gamma * P.pinf * ir;
+}
+
 /* Green-Gauss gradients (gradcalc.c:23-42).  FULL: all neq+1 rows (incl. p) to
  * SG f64 [neq+1][3][ns]; otherwise the momentum rows 1..3 to SGm [9][ns]. */
-template <int MODEL, bool FULL, typename R>
+template <int MODEL, bool FULL, typename R, bool HEAT = false>
 __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
                                                    const double *__restrict__ S,
                                                    double *__restrict__ SG, R *__restrict__ SGm) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  /* HEAT (heat conduction option, model 1): also the Green-Gauss gradient of p / rho,
+   * from the same face means, into SGm rows 9..11 */
+  static_assert(!HEAT || (MODEL == 1 && !FULL), "heat conduction: model 1, momentum rows");
   const int i = logical_block() * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
   const uint32_t cf = M.cflag[i];
@@ -75,12 +98,13 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
   const double V = M.vol[L];
   double U[NEQ];
   load_S<NEQ>(S, M.ns, i, U);
-  double pown = 0.0;
+  double pown = 0.0, town = 0.0;
   if constexpr (FULL) {
     double r, u[3], e, p, c, y;
     eos64<MODEL>(P, U, false, r, u, e, p, c, y);
     pown = p;
   }
+  if constexpr (HEAT) town = theta_ideal(P, U);
   constexpr int V0 = FULL ? 0 : 1;
   constexpr int V1 = FULL ? NEQ : 4;
   /* the six neighbours and the row of each single neighbour are loaded before any
@@ -95,12 +119,19 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
     const int cls = (cf >> (2 * f)) & 3;
     load_S<NEQ>(S, M.ns, (cls == CLS_SAME || cls == CLS_COARSER) ? nbs[f] : i, Ur[f]);
   }
+  /* theta of the single neighbours, formed right away so that only the momentum rows
+   * of Ur stay live */
+  double thr[6];
+  if constexpr (HEAT) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) thr[f] = theta_ideal(P, Ur[f]);
+  }
   /* per axis d, faces of that axis in increasing face id */
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int fa = d == 0 ? 1 : (d == 1 ? 0 : 4);
     const int fb = d == 0 ? 3 : (d == 1 ? 2 : 5);
-    double acc[NEQ + 1];
+    double acc[NEQ + 1], acct = 0.0;
 #pragma unroll
     for (int v = 0; v <= NEQ; ++v) acc[v] = 0.0;
 #pragma unroll
@@ -133,8 +164,15 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
           eos64<MODEL>(P, Un, false, r, u, e, p, c, y);
           acc[NEQ] += 0.5 * (p + pown) * nd * A / V;
         }
+        if constexpr (HEAT) {
+          const double thn = cls == CLS_FINER ? theta_ideal(P, Un)
+                             : (cls == CLS_BOUNDARY ? (-1 - nb == 1 ? theta_ideal(P, Un) : town)
+                                                    : thr[f]);
+          acct += 0.5 * (thn + town) * nd * A / V;
+        }
       }
     }
+    if constexpr (HEAT) SGm[((size_t)9 + d) * M.ns + i] = (R)acct;
     if constexpr (FULL) {
 #pragma unroll
       for (int v = 0; v <= NEQ; ++v) SG[((size_t)v * 3 + d) * M.ns + i] = acc[v];
@@ -161,9 +199,22 @@ __device__ __forceinline__ void store_rows(R *__restrict__ G, const R *gbuf, int
   for (int k = threadIdx.x; k < nrow * NEQ; k += FV_BLOCK) g[k] = gbuf[k];
 }
 
+/* the gradients of p / rho of both cells (SGm rows 9..11; the kernels with the
+ * heat-conduction bit read them, and use them where P.kheat != 0) */
+template <typename T, typename R>
+__device__ __forceinline__ void load_heat_grad(const DevMesh &M, const R *__restrict__ SGm,
+                                               int iL, int iR, FState<T> &L, FState<T> &Rs) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    L.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iL];
+    Rs.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iR];
+  }
+}
+
 /* one face, both sides (residual_faces_AMR.c:95-150 for the owner side), with
- * the axis a of the face as a compile-time constant */
-template <int MODEL, bool NS, int a, typename T, typename R>
+ * the axis a of the face as a compile-time constant.  OPT: the option variant (PVRS
+ * wave speeds, heat conduction; the primitive reconstruction is k_recon's) */
+template <int MODEL, bool NS, int a, typename T, typename R, int OPT = 0>
 __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, int j,
                                           const double *__restrict__ S, const R *__restrict__ SF,
                                           const R *__restrict__ SGm, R *dst,
@@ -199,9 +250,10 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
       L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
       Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
     }
+    if constexpr ((OPT & OPT_HEAT) != 0) load_heat_grad<T, R>(M, SGm, iL, iR, L, Rs);
   }
   T out[NEQ];
-  ok = hllc_face<MODEL, NS, a, T>(P, L, Rs, s, out) && ok;
+  ok = hllc_face<MODEL, NS, a, T, OPT>(P, L, Rs, s, out) && ok;
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -216,7 +268,7 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
  * face (interior, stencil face_LL, face_L, face_R, face_RR of one level along
  * the axis): the same arithmetic as recon_face with the neighbours and the
  * same-level distances known, without the class and neighbour lookups. */
-template <int MODEL, bool NS, int a, typename T, typename R, bool REG>
+template <int MODEL, bool NS, int a, typename T, typename R, bool REG, int OPT = 0>
 __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParams &P, int j,
                                                 const double *__restrict__ S,
                                                 const R *__restrict__ SGm, R *dst,
@@ -228,12 +280,16 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
             lev = (meta >> 8) & 0xff;
   const T s = (T)c_face_sign[fl];
   double UL[NEQ], UR[NEQ];
+  const bool prim = (OPT & OPT_PRIM) && P.iprimtv;
   if constexpr (REG) {
-    recon_regular<NEQ, T>(M, P, S, j, iL, iR, lev, a, UL, UR);
+    if (prim) recon_regular_prim<NEQ, T>(M, P, S, j, iL, iR, lev, a, UL, UR);
+    else recon_regular<NEQ, T>(M, P, S, j, iL, iR, lev, a, UL, UR);
   } else {
-    recon_face<NEQ, T>(M, P, S, iL, fl, UL);
+    if (prim) recon_face_prim<NEQ, T>(M, P, S, iL, fl, UL);
+    else recon_face<NEQ, T>(M, P, S, iL, fl, UL);
     if (bc == 0) {
-      recon_face<NEQ, T>(M, P, S, iR, fr, UR);
+      if (prim) recon_face_prim<NEQ, T>(M, P, S, iR, fr, UR);
+      else recon_face<NEQ, T>(M, P, S, iR, fr, UR);
     } else if (bc == 1) {
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) UR[v] = P.inlet[v];
@@ -251,20 +307,21 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
       L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
       Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
     }
+    if constexpr ((OPT & OPT_HEAT) != 0) load_heat_grad<T, R>(M, SGm, iL, iR, L, Rs);
   }
   T out[NEQ];
-  ok = hllc_face<MODEL, NS, a, T>(P, L, Rs, s, out) && ok;
+  ok = hllc_face<MODEL, NS, a, T, OPT>(P, L, Rs, s, out) && ok;
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) dst[v] = (R)(-(A) * out[v]);
   if (!ok) atomicMin(err, iL);
 }
 
-template <int MODEL, bool NS, typename T, typename R>
+template <int MODEL, bool NS, typename T, typename R, int OPT = 0>
 /* occupancy targets: 6 waves per SIMD for the Euler flux (78-80 VGPRs, no
  * spill); the viscous variants need 96-98 VGPRs and ask for fewer waves instead
- * of spilling */
-__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k_face_fused(
+ * of spilling.  OPT: option bits (primitive MUSCL, PVRS, heat conduction). */
+__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : (OPT ? 4 : 6)) void k_face_fused(
     DevMesh M, DevParams P, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
     int *__restrict__ err) {
@@ -283,18 +340,18 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
   if (ax == 0) {
     base = 0;
     n = n0;
-    if (q < r0) face_fused_body<MODEL, NS, 0, T, R, true>(M, P, q, S, SGm, dst, err);
-    else if (q < n0) face_fused_body<MODEL, NS, 0, T, R, false>(M, P, q, S, SGm, dst, err);
+    if (q < r0) face_fused_body<MODEL, NS, 0, T, R, true, OPT>(M, P, q, S, SGm, dst, err);
+    else if (q < n0) face_fused_body<MODEL, NS, 0, T, R, false, OPT>(M, P, q, S, SGm, dst, err);
   } else if (ax == 1) {
     base = n0;
     n = n1;
-    if (q < r1) face_fused_body<MODEL, NS, 1, T, R, true>(M, P, n0 + q, S, SGm, dst, err);
-    else if (q < n1) face_fused_body<MODEL, NS, 1, T, R, false>(M, P, n0 + q, S, SGm, dst, err);
+    if (q < r1) face_fused_body<MODEL, NS, 1, T, R, true, OPT>(M, P, n0 + q, S, SGm, dst, err);
+    else if (q < n1) face_fused_body<MODEL, NS, 1, T, R, false, OPT>(M, P, n0 + q, S, SGm, dst, err);
   } else {
     base = n0 + n1;
     n = n2;
-    if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true>(M, P, n0 + n1 + q, S, SGm, dst, err);
-    else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false>(M, P, n0 + n1 + q, S, SGm, dst, err);
+    if (q < r2) face_fused_body<MODEL, NS, 2, T, R, true, OPT>(M, P, n0 + n1 + q, S, SGm, dst, err);
+    else if (q < n2) face_fused_body<MODEL, NS, 2, T, R, false, OPT>(M, P, n0 + n1 + q, S, SGm, dst, err);
   }
   if constexpr (STAGE) {
     __syncthreads();
@@ -309,7 +366,7 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : 6) void k
  * difference of large terms) is left to face_rebuild_block, which rebuilds it
  * from the FP64 cell states with the FP64 inversion, exactly as the FP64-read
  * mixed kernel computes it.  Returns false for such a face. */
-template <bool NS, int a, bool REG>
+template <bool NS, int a, bool REG, bool HEAT = false>
 __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams &P, int j,
                                                const float *__restrict__ Sf,
                                                const float *__restrict__ SGm,
@@ -360,9 +417,10 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
       L.g[k] = SGm[(size_t)k * M.ns + iL];
       Rs.g[k] = SGm[(size_t)k * M.ns + iR];
     }
+    if constexpr (HEAT) load_heat_grad<float, float>(M, SGm, iL, iR, L, Rs);
   }
   float out[5];
-  const bool ok = hllc_face<1, NS, a, float>(P, L, Rs, s, out);
+  const bool ok = hllc_face<1, NS, a, float, HEAT ? OPT_HEAT : 0>(P, L, Rs, s, out);
   const float A = (float)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < 5; ++v) dst[v] = -(A)*out[v];
@@ -408,7 +466,7 @@ __device__ __forceinline__ double recon_face_var(const DevMesh &M, const DevPara
  * one thread per (face, side, variable) stages the FP64 face states in LDS, then
  * one thread per face inverts them and evaluates the flux.  Written lean so the
  * kernel's register budget stays close to that of its main path. */
-template <bool NS>
+template <bool NS, bool HEAT = false>
 __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
                                    const double *__restrict__ S, const float *__restrict__ SGm,
                                    float *__restrict__ G, int *__restrict__ err, const int *list,
@@ -445,12 +503,14 @@ __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
           L.g[k] = SGm[(size_t)k * M.ns + iL];
           Rs.g[k] = SGm[(size_t)k * M.ns + iR];
         }
+        if constexpr (HEAT) load_heat_grad<float, float>(M, SGm, iL, iR, L, Rs);
       }
       const float s = (float)c_face_sign[fl];
       float out[5];
-      if (a == 0) ok = hllc_face<1, NS, 0, float>(P, L, Rs, s, out) && ok;
-      else if (a == 1) ok = hllc_face<1, NS, 1, float>(P, L, Rs, s, out) && ok;
-      else ok = hllc_face<1, NS, 2, float>(P, L, Rs, s, out) && ok;
+      constexpr int O = HEAT ? OPT_HEAT : 0;
+      if (a == 0) ok = hllc_face<1, NS, 0, float, O>(P, L, Rs, s, out) && ok;
+      else if (a == 1) ok = hllc_face<1, NS, 1, float, O>(P, L, Rs, s, out) && ok;
+      else ok = hllc_face<1, NS, 2, float, O>(P, L, Rs, s, out) && ok;
       const float A = (float)M.area[3 * lev + a];
       for (int v = 0; v < 5; ++v) G[(size_t)j * 5 + v] = -(A)*out[v];
       if (!ok) atomicMin(err, iL);
@@ -462,7 +522,7 @@ __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
 /* The faces of a block whose FP32 inversion was rejected are collected in LDS
  * and rebuilt by the block itself (no other block, no extra launch); blocks with
  * no such face skip it after one block-wide vote. */
-template <bool NS>
+template <bool NS, bool HEAT = false>
 __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, DevParams P, int n0, int n1, int n2,
                                                                    int r0, int r1, int r2,
                                                                    const double *__restrict__ S,
@@ -484,19 +544,19 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
   const int j = ax == 0 ? q : (ax == 1 ? n0 + q : n0 + n1 + q);
   float *dst = G + (size_t)j * 5;
   if (ax == 0) {
-    if (q < r0) done = face_fast_body<NS, 0, true>(M, P, j, Sf, SGm, dst, err);
-    else if (q < n0) done = face_fast_body<NS, 0, false>(M, P, j, Sf, SGm, dst, err);
+    if (q < r0) done = face_fast_body<NS, 0, true, HEAT>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n0) done = face_fast_body<NS, 0, false, HEAT>(M, P, j, Sf, SGm, dst, err);
   } else if (ax == 1) {
-    if (q < r1) done = face_fast_body<NS, 1, true>(M, P, j, Sf, SGm, dst, err);
-    else if (q < n1) done = face_fast_body<NS, 1, false>(M, P, j, Sf, SGm, dst, err);
+    if (q < r1) done = face_fast_body<NS, 1, true, HEAT>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n1) done = face_fast_body<NS, 1, false, HEAT>(M, P, j, Sf, SGm, dst, err);
   } else {
-    if (q < r2) done = face_fast_body<NS, 2, true>(M, P, j, Sf, SGm, dst, err);
-    else if (q < n2) done = face_fast_body<NS, 2, false>(M, P, j, Sf, SGm, dst, err);
+    if (q < r2) done = face_fast_body<NS, 2, true, HEAT>(M, P, j, Sf, SGm, dst, err);
+    else if (q < n2) done = face_fast_body<NS, 2, false, HEAT>(M, P, j, Sf, SGm, dst, err);
   }
   if (!__syncthreads_or(!done)) return;
   if (!done) list[atomicAdd(&nlist, 1)] = j;
   __syncthreads();
-  face_rebuild_block<NS>(M, P, S, SGm, G, err, list, nlist);
+  face_rebuild_block<NS, HEAT>(M, P, S, SGm, G, err, list, nlist);
 }
 
 /* Refinement flags from the density gradient SG[0][.] (gradcalc rows, [neq+1][3][ns]):
@@ -542,7 +602,7 @@ __global__ void k_to_f32(const double *__restrict__ S, float *__restrict__ Sf, i
 
 /* Faces are grouped by axis; blocks [0, b0) take x faces, [b0, b0+b1) y faces,
  * the rest z faces, so the axis is uniform within a block. */
-template <int MODEL, bool NS, typename T, typename R>
+template <int MODEL, bool NS, typename T, typename R, int OPT = 0>
 __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b0, int b1,
                                                    int n0, int n1, int n2,
                                                    const double *__restrict__ S,
@@ -561,11 +621,11 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
   const int n = ax == 0 ? n0 : (ax == 1 ? n1 : n2);
   R *dst = STAGE ? gbuf + t * NEQ : G + (size_t)(base + q) * NEQ;
   if (ax == 0) {
-    if (q < n0) face_body<MODEL, NS, 0, T, R>(M, P, q, S, SF, SGm, dst, err);
+    if (q < n0) face_body<MODEL, NS, 0, T, R, OPT>(M, P, q, S, SF, SGm, dst, err);
   } else if (ax == 1) {
-    if (q < n1) face_body<MODEL, NS, 1, T, R>(M, P, n0 + q, S, SF, SGm, dst, err);
+    if (q < n1) face_body<MODEL, NS, 1, T, R, OPT>(M, P, n0 + q, S, SF, SGm, dst, err);
   } else {
-    if (q < n2) face_body<MODEL, NS, 2, T, R>(M, P, n0 + n1 + q, S, SF, SGm, dst, err);
+    if (q < n2) face_body<MODEL, NS, 2, T, R, OPT>(M, P, n0 + n1 + q, S, SF, SGm, dst, err);
   }
   if constexpr (STAGE) {
     __syncthreads();
@@ -874,7 +934,18 @@ static void to_dev_params(const fvg_params *p, DevParams *d) {
   d->mix_cmix = p->mix_cmix;
   for (int v = 0; v < 5; ++v) d->inlet[v] = p->inlet[v];
   d->va_eps = p->vanalbada_eps;
+  d->iprimtv = p->iprimtv;
+  d->pvrs = p->wave_speeds == FVG_WAVES_PVRS;
+  d->kheat = p->prandtl > 0.0 ? p->mu * p->gamma / ((p->gamma - 1.0) * p->prandtl) : 0.0;
 }
+
+/* the options beyond the reference's behaviour as OPT_* bits: the kernels' option
+ * variants run where any is set */
+static int opt_mask(const fvg_params &p) {
+  return (p.iprimtv ? OPT_PRIM : 0) | (p.wave_speeds != FVG_WAVES_DAVIS ? OPT_PVRS : 0) |
+         (p.prandtl > 0.0 ? OPT_HEAT : 0);
+}
+static bool has_opts(const fvg_params &p) { return opt_mask(p) != 0; }
 
 static int check_params(const fvg_params *p) {
   if (!p) return FVG_ERR_ARG;
@@ -883,6 +954,12 @@ static int check_params(const fvg_params *p) {
   if (p->limiter != FVG_MINMOD && p->limiter != FVG_VANALBADA) return FVG_ERR_UNSUPPORTED;
   if (p->precision != FVG_PLAIN && p->precision != FVG_MIXED) return FVG_ERR_UNSUPPORTED;
   if (p->ns && p->model == 2) return FVG_ERR_UNSUPPORTED; /* viscous path needs 5 rows of SG */
+  if (p->iprimtv != 0 && p->iprimtv != 1) return FVG_ERR_ARG;
+  if (p->wave_speeds != FVG_WAVES_DAVIS && p->wave_speeds != FVG_WAVES_PVRS) return FVG_ERR_ARG;
+  /* the PVRS estimate and the heat flux are written for the ideal / stiffened gas */
+  if (p->wave_speeds == FVG_WAVES_PVRS && p->model != 1) return FVG_ERR_UNSUPPORTED;
+  if (!(p->prandtl >= 0.0)) return FVG_ERR_ARG;
+  if (p->prandtl > 0.0 && (p->model != 1 || !p->ns)) return FVG_ERR_UNSUPPORTED;
   return FVG_OK;
 }
 
@@ -1005,7 +1082,8 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, fv_dmalloc(c, &c->RHS, (size_t)neq * ns * 8, slack));
   HIPCHK(c, fv_dmalloc(c, &c->SG, (size_t)(neq + 1) * 3 * ns * 8, slack));
   HIPCHK(c, fv_dmalloc(c, &c->SF, (size_t)6 * neq * ns * rs, slack));
-  HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)9 * ns * rs, slack));
+  /* 9 momentum gradients, then 3 of p / rho (heat conduction option) */
+  HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)12 * ns * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->G, (size_t)neq * c->nfp * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->Sx, (size_t)neq * ns * 8, slack));
   if (rs == 4 && c->prm.model == 1) HIPCHK(c, fv_dmalloc(c, &c->Sf, (size_t)neq * ns * 4, slack));
@@ -1018,7 +1096,7 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
-  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)9 * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)12 * ns * rs, st));
   HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
   fv_dcache_trim(c);
   return FVG_OK;
@@ -1338,10 +1416,18 @@ template <int MODEL>
 static void launch_recon(fvg_ctx *c, const double *S) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
-  if (c->rsize == 8)
-    hipLaunchKernelGGL((k_recon<MODEL, double>), g, b, 0, c->stream, M, c->dp, S, (double *)c->SF);
-  else
-    hipLaunchKernelGGL((k_recon<MODEL, float>), g, b, 0, c->stream, M, c->dp, S, (float *)c->SF);
+  const bool prim = c->prm.iprimtv != 0;
+  if (c->rsize == 8) {
+    if (prim)
+      hipLaunchKernelGGL((k_recon<MODEL, double, true>), g, b, 0, c->stream, M, c->dp, S, (double *)c->SF);
+    else
+      hipLaunchKernelGGL((k_recon<MODEL, double>), g, b, 0, c->stream, M, c->dp, S, (double *)c->SF);
+  } else {
+    if (prim)
+      hipLaunchKernelGGL((k_recon<MODEL, float, true>), g, b, 0, c->stream, M, c->dp, S, (float *)c->SF);
+    else
+      hipLaunchKernelGGL((k_recon<MODEL, float>), g, b, 0, c->stream, M, c->dp, S, (float *)c->SF);
+  }
 }
 
 template <int MODEL>
@@ -1351,7 +1437,16 @@ static void launch_grad(fvg_ctx *c, const double *S, bool full) {
   if (full)
     hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)nullptr);
-  else if (c->rsize == 8)
+  else if (MODEL == 1 && c->dp.kheat != 0.0) {
+    if constexpr (MODEL == 1) {
+      if (c->rsize == 8)
+        hipLaunchKernelGGL((k_grad<MODEL, false, double, true>), g, b, 0, c->stream, M, c->dp, S,
+                           c->SG, (double *)c->SGm);
+      else
+        hipLaunchKernelGGL((k_grad<MODEL, false, float, true>), g, b, 0, c->stream, M, c->dp, S,
+                           c->SG, (float *)c->SGm);
+    }
+  } else if (c->rsize == 8)
     hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)c->SGm);
   else
@@ -1359,34 +1454,63 @@ static void launch_grad(fvg_ctx *c, const double *S, bool full) {
                        (float *)c->SGm);
 }
 
+/* The option variant a parameter set runs: none, the heat-only one (the viscous
+ * Taylor-Green configuration: no other option code compiled in), or the one with all
+ * option bits (their runtime switches pick the path). */
 template <int MODEL, bool NS>
-static void launch_face_m(fvg_ctx *c, const double *S) {
+static int opt_variant(const fvg_ctx *c) {
+  const int m = opt_mask(c->prm);
+  if (m == 0) return 0;
+  if (m == OPT_HEAT && MODEL == 1 && NS) return OPT_HEAT;
+  return OPT_ALL;
+}
+
+template <int MODEL, bool NS, typename R, int OPT>
+static void launch_face_t(fvg_ctx *c, const double *S) {
   DevMesh M = dev_mesh(c);
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
-  if (c->rsize == 8)
-    hipLaunchKernelGGL((k_face<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
-                       n1, n2, S, (const double *)c->SF, (const double *)c->SGm, (double *)c->G,
-                       c->err);
-  else
-    hipLaunchKernelGGL((k_face<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp, b0, b1, n0,
-                       n1, n2, S, (const float *)c->SF, (const float *)c->SGm, (float *)c->G,
-                       c->err);
+  hipLaunchKernelGGL((k_face<MODEL, NS, R, R, OPT>), g, b, 0, c->stream, M, c->dp, b0, b1, n0, n1,
+                     n2, S, (const R *)c->SF, (const R *)c->SGm, (R *)c->G, c->err);
+}
+
+template <int MODEL, bool NS, typename R>
+static void launch_face_r(fvg_ctx *c, const double *S) {
+  const int v = opt_variant<MODEL, NS>(c);
+  if (v == 0) launch_face_t<MODEL, NS, R, 0>(c, S);
+  else if (v == OPT_HEAT) launch_face_t<MODEL, NS, R, OPT_HEAT>(c, S);
+  else launch_face_t<MODEL, NS, R, OPT_ALL>(c, S);
+}
+
+template <int MODEL, bool NS>
+static void launch_face_m(fvg_ctx *c, const double *S) {
+  if (c->rsize == 8) launch_face_r<MODEL, NS, double>(c, S);
+  else launch_face_r<MODEL, NS, float>(c, S);
+}
+
+template <int MODEL, bool NS, typename R, int OPT>
+static void launch_face_fused_t(fvg_ctx *c, const double *S) {
+  DevMesh M = dev_mesh(c);
+  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
+  const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
+  dim3 g(b0 + b1 + b2), b(FV_BLOCK);
+  hipLaunchKernelGGL((k_face_fused<MODEL, NS, R, R, OPT>), g, b, 0, c->stream, M, c->dp, n0, n1, n2,
+                     c->nreg[0], c->nreg[1], c->nreg[2], S, (const R *)c->SGm, (R *)c->G, c->err);
+}
+
+template <int MODEL, bool NS, typename R>
+static void launch_face_fused_r(fvg_ctx *c, const double *S) {
+  const int v = opt_variant<MODEL, NS>(c);
+  if (v == 0) launch_face_fused_t<MODEL, NS, R, 0>(c, S);
+  else if (v == OPT_HEAT) launch_face_fused_t<MODEL, NS, R, OPT_HEAT>(c, S);
+  else launch_face_fused_t<MODEL, NS, R, OPT_ALL>(c, S);
 }
 
 template <int MODEL, bool NS>
 static void launch_face_fused_m(fvg_ctx *c, const double *S) {
-  DevMesh M = dev_mesh(c);
-  const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
-  const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
-  dim3 g(b0 + b1 + b2), b(FV_BLOCK);
-  if (c->rsize == 8)
-    hipLaunchKernelGGL((k_face_fused<MODEL, NS, double, double>), g, b, 0, c->stream, M, c->dp,
-                       n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const double *)c->SGm, (double *)c->G, c->err);
-  else
-    hipLaunchKernelGGL((k_face_fused<MODEL, NS, float, float>), g, b, 0, c->stream, M, c->dp,
-                       n0, n1, n2, c->nreg[0], c->nreg[1], c->nreg[2], S, (const float *)c->SGm, (float *)c->G, c->err);
+  if (c->rsize == 8) launch_face_fused_r<MODEL, NS, double>(c, S);
+  else launch_face_fused_r<MODEL, NS, float>(c, S);
 }
 
 template <int MODEL>
@@ -1504,7 +1628,11 @@ static void launch_face_fast(fvg_ctx *c, const double *S) {
   const int n0 = c->nfa[0], n1 = c->nfa[1], n2 = c->nfa[2];
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
-  if (c->prm.ns)
+  if (c->prm.ns && c->dp.kheat != 0.0)
+    hipLaunchKernelGGL((k_face_fast<true, true>), g, b, 0, c->stream, M, c->dp, n0, n1, n2,
+                       c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
+                       (const float *)c->SGm, (float *)c->G, c->err);
+  else if (c->prm.ns)
     hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
                        (const float *)c->SGm, (float *)c->G, c->err);
@@ -1533,7 +1661,10 @@ static const int *stage_list(int method, int *n) {
 }
 
 static bool fast_mixed(const fvg_ctx *c) {
-  return c->Sf && c->prm.model == 1 && c->prm.ischeme == 2;
+  /* heat conduction has its own variant of the FP32-read kernel; the other options run
+   * the FP64-read kernels' option variant instead */
+  const int m = opt_mask(c->prm);
+  return c->Sf && c->prm.model == 1 && c->prm.ischeme == 2 && (m == 0 || m == OPT_HEAT);
 }
 
 /* one stage's kernels on the state sp.S -> sp.Sout; ev (optional) receives

@@ -118,13 +118,16 @@ static int push(fv_binding *b, int which) {
 
 int fv_gpu_attach(RUN *run, int device, const fvg_params *p, int compat) {
   if (!run || !p || !run->con || !run->topo) return FVG_ERR_ARG;
-  /* MUSCL on primitive variables: the reference's cons2primtv divides by a density
-   * that facevalues_MUSCL never sets (cons2primtv.c:10-13), so there is no defined
-   * result to reproduce */
-  if (run->con->iprimtv != 0) return FVG_ERR_UNSUPPORTED;
+  /* con->iprimtv selects the MUSCL variables as in the reference (deltavalues.c:16).
+   * With 1 the primitive path runs in its defined form: the reference's cons2primtv
+   * divides by a density that facevalues_MUSCL never sets (cons2primtv.c:10-13), so its
+   * own output is not a function of its inputs */
+  if (run->con->iprimtv != 0 && run->con->iprimtv != 1) return FVG_ERR_UNSUPPORTED;
+  fvg_params q = *p;
+  q.iprimtv = run->con->iprimtv;
   fv_binding *b = (fv_binding *)calloc(1, sizeof(*b));
   if (!b) return FVG_ERR_NOMEM;
-  int rc = fvg_create(device, p, &b->ctx);
+  int rc = fvg_create(device, &q, &b->ctx);
   if (rc) { free(b); return rc; }
   b->magic = FV_BINDING_MAGIC;
   b->run = run;

@@ -43,6 +43,9 @@ enum {
 enum { FVG_PLAIN = 0, FVG_MIXED = 1 };
 /* limiters: minmod is the reference's hard-wired one (reconctruct_V3.c:19-30) */
 enum { FVG_MINMOD = 0, FVG_VANALBADA = 1 };
+/* wave speeds of the HLLC solver: Davis, the reference's (compute_wave_speeds.c:53-54), or
+ * the primitive-variable (PVRS) pressure estimate (Toro, Riemann Solvers, 3rd ed., 10.5.2) */
+enum { FVG_WAVES_DAVIS = 0, FVG_WAVES_PVRS = 1 };
 /* fields for fvg_get_field / fvg_device_ptr */
 enum {
   FVG_FIELD_S = 0,    /* [neq][ncell] f64 */
@@ -76,6 +79,16 @@ typedef struct fvg_params {
   double mix_RG, mix_Tref, mix_rsat, mix_psat, mix_cliq, mix_cmix;
   double inlet[5];        /* conserved ghost state of bc type 1 (bc_inlet.c) */
   double vanalbada_eps;   /* epsilon of the smooth van Albada limiter */
+  /* Options beyond the reference's behaviour; 0 keeps it. */
+  int iprimtv;      /* 1: MUSCL on the primitive variables (rho, u, v, w, p | Y) of the reference's
+                       iprimtv = 1 path (deltavalues.c:16-19, reconctruct_V3.c:37-51, prim2conc.c),
+                       with the state's own density and the kinetic energy from its velocities
+                       (cons2primtv.c:10-18 reads a stale density and squares the momenta) */
+  int wave_speeds;  /* FVG_WAVES_DAVIS | FVG_WAVES_PVRS (model 1) */
+  double prandtl;   /* > 0: heat conduction in the energy row of the viscous flux (model 1, ns),
+                       q = -mu gamma / ((gamma - 1) Pr) grad(p / rho), the gradient a Green-Gauss one
+                       of p / rho averaged over the face's two cells; 0: none (flux_viscous.c has
+                       no heat flux) */
 } fvg_params;
 
 /* Flattened mesh (all arrays host memory, read by fvg_set_mesh). */

@@ -58,8 +58,10 @@
 typedef struct OSOL {
   double r, u, v, w, e, e_internal, p_hydro, c;
   double vec[8];
+  double wec[8];  /* primitive variables (iprimtv option) */
   double ymass;
   double ux, uy, uz, vx, vy, vz, wx, wy, wz;
+  double tx, ty, tz; /* gradient of p / rho (heat conduction option) */
 } OSOL;
 
 typedef struct OCTX {
@@ -191,23 +193,41 @@ static void orc_load_grads(OSOL *s, const LEAF *el) {
   s->wx = el->SG[3][0]; s->wy = el->SG[3][1]; s->wz = el->SG[3][2];
 }
 
+static int orc_theta_grad(OCTX *o, BRANCH *b, double g[3]);
+static int orc_heat_on(const OCTX *o);
+
+/* the gradient of p / rho of leaf b into s (heat conduction option) */
+static int orc_load_theta(OCTX *o, BRANCH *b, OSOL *s) {
+  if (!orc_heat_on(o)) return ORC_OK;
+  double g[3];
+  int rc = orc_theta_grad(o, b, g);
+  s->tx = g[0];
+  s->ty = g[1];
+  s->tz = g[2];
+  return rc;
+}
+
 /* facevalues.c:3-82 */
 static int orc_facevalues(OCTX *o, BRANCH *crnt, int in, int ifc, int iside, OSOL *s) {
-  LEAF *el = iside == 0 ? crnt->el : crnt->neigtr[ifc][in]->el;
+  BRANCH *b = iside == 0 ? crnt : crnt->neigtr[ifc][in];
+  LEAF *el = b->el;
   int rc = orc_fill(o, s, el->S, 0);
   if (o->p->ns == 1) orc_load_grads(s, el);
+  if (!rc) rc = orc_load_theta(o, b, s);
   return rc;
 }
 
 /* facercnstr.c:3-89 */
 static int orc_facercnstr(OCTX *o, BRANCH *crnt, int in, int ifc, int iside, OSOL *s) {
-  LEAF *el = crnt->el;
+  BRANCH *b = crnt;
   if (iside == 1) {
-    el = crnt->neigtr[ifc][in]->el;
+    b = crnt->neigtr[ifc][in];
     ifc = crnt->neigfc[ifc];
   }
+  LEAF *el = b->el;
   int rc = orc_fill(o, s, el->SF[ifc], 1);
   if (o->p->ns == 1) orc_load_grads(s, el);
+  if (!rc) rc = orc_load_theta(o, b, s);
   return rc;
 }
 
@@ -226,6 +246,76 @@ static int orc_bc(OCTX *o, int bc, OSOL *s1) {
       return ORC_ERR_UNSUPPORTED; /* bc_reflective / bc_wall are undefined (bc.c:38,42) */
   }
 }
+
+/* cons2primtv.c:3-25 in its defined form (option iprimtv = 1): the density of the state
+ * itself (the reference divides by a stale sol->r) and the kinetic energy from the
+ * velocities (cons2primtv.c:17 squares the momenta) */
+static void orc_cons2prim(const OCTX *o, const double *U, double *W) {
+  const fvg_params *p = o->p;
+  double r = U[0];
+  W[0] = r;
+  W[1] = U[1] / r;
+  W[2] = U[2] / r;
+  W[3] = U[3] / r;
+  if (p->model == 1) {
+    double ei = (U[4] / r) - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    W[4] = r * (p->gamma - 1.0) * ei - p->pinf;
+  } else if (p->model == 3) {
+    W[4] = U[4] / r;
+  }
+}
+
+/* prim2conc.c:3-23 */
+static void orc_prim2cons(const OCTX *o, const double *W, double *U) {
+  const fvg_params *p = o->p;
+  double r = W[0];
+  U[0] = r;
+  U[1] = r * W[1];
+  U[2] = r * W[2];
+  U[3] = r * W[3];
+  if (p->model == 1) {
+    double ei = (W[4] + p->pinf) / (r * (p->gamma - 1.0));
+    U[4] = r * (ei + 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
+  } else if (p->model == 3) {
+    U[4] = r * W[4];
+  }
+}
+
+/* Green-Gauss gradient of theta = p / rho of leaf b (option prandtl > 0): the loop of
+ * gradcalc.c:23-40 over the faces and sub-faces of b, face value (theta_b + theta_nb)/2,
+ * theta_nb of the neighbour's cell state or of the ghost (bc 2: b's own, bc 1: the inlet
+ * state) */
+static int orc_theta_grad(OCTX *o, BRANCH *b, double g[3]) {
+  OSOL t;
+  int rc = orc_fill(o, &t, b->el->S, 0);
+  if (rc) return rc;
+  const double th = t.p_hydro / t.r;
+  const CELL *cl = b->cl;
+  g[0] = g[1] = g[2] = 0.0;
+  for (int f = 0; f < b->nlfc; ++f)
+    for (int k = 0; k < b->nsfc[f]; ++k) {
+      double thn = th;
+      const int bc = cl->fc[f].bc;
+      if (bc == 0) {
+        rc = orc_fill(o, &t, b->neigtr[f][k]->el->S, 0);
+        if (rc) return rc;
+        thn = t.p_hydro / t.r;
+      } else if (bc == 1) {
+        double U[8];
+        for (int v = 0; v < o->neq; ++v) U[v] = o->p->inlet[v];
+        rc = orc_fill(o, &t, U, 0);
+        if (rc) return rc;
+        thn = t.p_hydro / t.r;
+      }
+      double m = 0.5 * (thn + th);
+      g[0] += m * cl->nx[f] * cl->Area[f][k] / cl->Vol;
+      g[1] += m * cl->ny[f] * cl->Area[f][k] / cl->Vol;
+      g[2] += m * cl->nz[f] * cl->Area[f][k] / cl->Vol;
+    }
+  return ORC_OK;
+}
+
+static int orc_heat_on(const OCTX *o) { return o->p->model == 1 && o->p->ns == 1 && o->p->prandtl > 0.0; }
 
 /* ------------------------------------------------------------------ MUSCL */
 
@@ -304,14 +394,24 @@ static int orc_deltavalues(OCTX *o, BRANCH *crnt, double *delta, int f) {
       return ORC_ERR_UNSUPPORTED;
     }
   }
+  if (o->p->iprimtv == 1) {
+    /* deltavalues.c:16-19: the deltas of the primitive variables */
+    orc_cons2prim(o, s->vec, s->wec);
+    orc_cons2prim(o, s1->vec, s1->wec);
+    for (int v = 0; v < neq; ++v) delta[v] = s1->wec[v] - s->wec[v];
+    return ORC_OK;
+  }
   for (int v = 0; v < neq; ++v) delta[v] = s1->vec[v] - s->vec[v];
   return ORC_OK;
 }
 
-/* reconctruct_V3.c:3-57 (iprimtv == 0); limiter FVG_VANALBADA is an extension */
+/* reconctruct_V3.c:3-57; limiter FVG_VANALBADA is an extension.  iprimtv = 1: the
+ * primitive face value (reconctruct_V3.c:37-38) through prim2conc (:47-51) */
 static void orc_reconstruct(OCTX *o, BRANCH *crnt, int f, double dp1cf, double dp1cc,
                             double dm1cf, double dm1cc) {
   (void)dm1cf;
+  const int prim = o->p->iprimtv == 1;
+  double Wf[8];
   for (int v = 0; v < o->neq; ++v) {
     double grad = 0.0;
     if (crnt->cl->fc[f].bc == 0) {
@@ -334,8 +434,10 @@ static void orc_reconstruct(OCTX *o, BRANCH *crnt, int f, double dp1cf, double d
         grad = (a * (b * b + e) + b * (a * a + e)) / (a * a + b * b + 2.0 * e);
       }
     }
-    crnt->el->SF[f][v] = o->sol.vec[v] + dp1cf * grad;
+    if (prim) Wf[v] = o->sol.wec[v] + dp1cf * grad;
+    else crnt->el->SF[f][v] = o->sol.vec[v] + dp1cf * grad;
   }
+  if (prim) orc_prim2cons(o, Wf, crnt->el->SF[f]);
 }
 
 static const int OFCOPP[6] = {2, 3, 0, 1, 5, 4};
@@ -505,6 +607,17 @@ static int orc_waves(const OCTX *o, const double R[3][3], OWAVES *w) {
   double cL = L->c, cR = Rs->c;
   double SR = (uL + cL) > (uR + cR) ? (uL + cL) : (uR + cR);
   double SL = (uL - cL) < (uR - cR) ? (uL - cL) : (uR - cR);
+  if (o->p->wave_speeds == FVG_WAVES_PVRS) {
+    /* option: PVRS speeds (Toro, Riemann Solvers, 3rd ed., 10.5.2), pressures as p + p_inf */
+    const double g = o->p->gamma, pL = L->p_hydro + o->p->pinf, pR = Rs->p_hydro + o->p->pinf;
+    double ps = 0.5 * (pL + pR) - 0.5 * (uR - uL) * (0.5 * (L->r + Rs->r)) * (0.5 * (cL + cR));
+    if (ps < 0.0) ps = 0.0;
+    const double k = (g + 1.0) / (2.0 * g);
+    const double qL = ps <= pL ? 1.0 : sqrt(1.0 + k * (ps / pL - 1.0));
+    const double qR = ps <= pR ? 1.0 : sqrt(1.0 + k * (ps / pR - 1.0));
+    SL = uL - cL * qL;
+    SR = uR + cR * qR;
+  }
   double s11L = orc_rotate_s11(L->p_hydro, R);
   double s11R = orc_rotate_s11(Rs->p_hydro, R);
   double SS = ((L->r * sq(uL) + s11L) - (Rs->r * sq(uR) + s11R) - (SL * ruL) + (SR * ruR)) /
@@ -581,6 +694,16 @@ static void orc_flux_viscous(OCTX *o) {
     F[4][0] = uf * txx + vf * txy + wf * txz;
     F[4][1] = uf * tyx + vf * tyy + wf * tyz;
     F[4][2] = uf * tzx + vf * tzy + wf * tzz;
+  }
+  if (orc_heat_on(o)) {
+    /* option: -q = mu gamma / ((gamma - 1) Pr) grad(p / rho), face mean of the two cells' */
+    const fvg_params *p = o->p;
+    const double kh = p->mu * p->gamma / ((p->gamma - 1.0) * p->prandtl);
+    const double w0 = p->visc_work ? F[4][0] : 0.0, w1 = p->visc_work ? F[4][1] : 0.0,
+                 w2 = p->visc_work ? F[4][2] : 0.0;
+    F[4][0] = w0 + kh * (0.5 * (s->tx + s1->tx));
+    F[4][1] = w1 + kh * (0.5 * (s->ty + s1->ty));
+    F[4][2] = w2 + kh * (0.5 * (s->tz + s1->tz));
   }
 }
 

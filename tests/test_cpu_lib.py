@@ -209,13 +209,13 @@ def _con_int(run_ptr, field):
     return C.c_int.from_address(con + _layout("CON", field))
 
 
-def test_primitive_reconstruction_refused():
-    """iprimtv = 1 is refused before any device call: the reference's cons2primtv reads
-    a density its loader never set, so there is no defined result to match."""
+def test_undefined_primitive_switch_refused():
+    """con->iprimtv selects conserved (0) or primitive (1) MUSCL; any other value is
+    refused before any device call."""
     F = uniform(2, 5)
     neq, iprimtv = _con_int(F.run, "neq"), _con_int(F.run, "iprimtv")
     assert neq.value == 5 and iprimtv.value == 0
-    iprimtv.value = 1
+    iprimtv.value = 2
     try:
         rc = fv.lib().fv_gpu_attach(F.run, 0, C.byref(fv.params(1)), 1)
         assert rc == _capi.FVG_ERR_UNSUPPORTED

@@ -80,6 +80,83 @@ def test_rhs_parity(model, ischeme, ns, mesh):
         assert np.all(e <= TOL), e
 
 
+OPTION_CASES = [
+    # (model, ischeme, ns, mesh, options): the faceinterp() / RHS() path (k_recon, k_face)
+    (1, 2, 0, "amr", dict(iprimtv=1)),
+    (1, 2, 0, "periodic_amr", dict(iprimtv=1, limiter="vanalbada")),
+    (3, 2, 0, "amr", dict(iprimtv=1)),
+    (2, 2, 0, "amr", dict(iprimtv=1)),
+    (1, 2, 0, "amr", dict(wave_speeds="pvrs")),
+    (1, 1, 0, "uniform", dict(wave_speeds="pvrs")),
+    (1, 2, 1, "periodic_amr", dict(prandtl=0.71)),
+    (1, 2, 1, "amr", dict(prandtl=0.71, wave_speeds="pvrs", iprimtv=1)),
+]
+
+
+@pytest.mark.parametrize("model,ischeme,ns,mesh,opts", OPTION_CASES)
+def test_rhs_parity_options(model, ischeme, ns, mesh, opts):
+    print(f"case options model={model} ischeme={ischeme} ns={ns} mesh={mesh} {opts}", flush=True)
+    neq = fv.neq_of(model)
+    F = _mesh(mesh, neq)
+    xc, _ = F.leaf_info()
+    F.set_state(state(model, xc, seed=model * 10 + ischeme + 7))
+    p = fv.params(model, ischeme=ischeme, ns=ns, mu=0.05 if ns else 0.0, visc_work=ns, **opts)
+    ref = oracle.full_rhs(F, p)
+    g = fv.Solver.from_forest(F, p)
+    got = g.full_rhs()
+    err = rel_linf(got, ref, neq)
+    print("rel Linf RHS", err, flush=True)
+    assert np.all(err <= _tol(model, neq)), err
+    if ischeme == 2:
+        sf_ref = F.get_field(fv.FIELD_SF)
+        sf = g.get_field(fv.FIELD_SF)
+        e = rel_linf(sf.reshape(6 * neq, -1), sf_ref.reshape(6 * neq, -1), neq)
+        assert np.all(e <= TOL), e
+    g.close()
+
+
+def test_run_binding_primitive_switch():
+    """con->iprimtv = 1 in the RUN selects the primitive-variable MUSCL through the
+    binding (fvg_params.iprimtv is taken from the RUN)."""
+    print("case RUN binding iprimtv", flush=True)
+    from test_cpu_lib import _con_int
+    F = amr(3, 5)
+    xc, _ = F.leaf_info()
+    F.set_state(state(1, xc, seed=23))
+    ref = oracle.full_rhs(F, fv.params(1, iprimtv=1))
+    L = fv.lib()
+    flag = _con_int(F.run, "iprimtv")
+    flag.value = 1
+    try:
+        assert L.fv_gpu_attach(F.run, 0, C.byref(fv.params(1)), 1) == 0
+        L.faceinterp(F.run)
+        L.RHS(F.run)
+        assert L.fv_gpu_last_status(F.run) == 0
+        got = F.get_field(fv.FIELD_RHS)
+        assert np.all(rel_linf(got, ref, 5) <= TOL)
+    finally:
+        L.fv_gpu_detach(F.run)
+        flag.value = 0
+
+
+def test_options_refused_where_undefined():
+    """PVRS and heat conduction are written for the ideal gas (heat conduction with the
+    viscous flux); the adjoint covers the reference's scheme only."""
+    for model, kw in ((2, dict(wave_speeds="pvrs")), (3, dict(wave_speeds="pvrs")),
+                      (1, dict(prandtl=0.71)), (3, dict(ns=1, prandtl=0.71))):
+        with pytest.raises(fv.FvError):
+            fv.Solver(fv.params(model, **kw))
+    F = uniform(3, 5, periodic=(1, 1, 1), bc=(0,) * 6)
+    xc, _ = F.leaf_info()
+    F.set_state(state(1, xc, seed=3, jump=False))
+    g = fv.Solver.from_forest(F, fv.params(1, wave_speeds="pvrs"))
+    with pytest.raises(fv.FvError):
+        g.adj_set_probe(np.ones(F.nleaves, np.uint8))
+    with pytest.raises(fv.FvError):
+        g.adj_forward(2, 1, 1e-3)
+    g.close()
+
+
 @pytest.mark.parametrize("irk", [0, 1, 2, 3, 4])
 def test_rk_stage_formula(irk):
     print(f"case rk irk={irk}", flush=True)

@@ -25,7 +25,9 @@ the stage formula with fused multiply-adds) give residuals that differ by a few
 The cases cover every face-kernel instantiation the workloads time: models 1/2/3,
 minmod and van Albada, Euler and Navier-Stokes (viscous work on), uniform, 2:1,
 periodic and periodic 2:1 meshes, an inlet side (bc 1), first order, and the
-three stage sequences (forward Euler, SSP-RK2 = irk 1,4, SSP-RK3 = irk 1,2,3).
+three stage sequences (forward Euler, SSP-RK2 = irk 1,4, SSP-RK3 = irk 1,2,3); and
+the kernels' option variants: MUSCL on primitive variables, PVRS wave speeds and heat
+conduction, alone and together.
 """
 import math
 
@@ -124,6 +126,18 @@ CASES = {
     "m3_amr_rk3": (3, 2, 0, "minmod", "amr", 3, {}),
     "m3_ns_amr_rk3": (3, 2, 1, "minmod", "amr", 3, {}),
     "m3_inlet_uniform_rk3": (3, 2, 0, "minmod", "uniform", 3, {"inlet": True}),
+    # options beyond the reference (fvg_params iprimtv, wave_speeds, prandtl)
+    "m1_pvrs_amr_rk3": (1, 2, 0, "minmod", "amr", 3, {"pvrs": True}),
+    "m1_pvrs_first_order_amr_rk3": (1, 1, 0, "minmod", "amr", 3, {"pvrs": True}),
+    "m1_iprimtv_amr_rk3": (1, 2, 0, "minmod", "amr", 3, {"iprimtv": True}),
+    "m1_iprimtv_vanalbada_periodic_amr_rk2": (1, 2, 0, "vanalbada", "periodic_amr", 2, {"iprimtv": True}),
+    "m1_iprimtv_inlet_uniform_rk3": (1, 2, 0, "minmod", "uniform", 3, {"iprimtv": True, "inlet": True}),
+    "m1_ns_heat_periodic_amr_rk3": (1, 2, 1, "minmod", "periodic_amr", 3, {"prandtl": 0.71}),
+    "m1_ns_heat_inlet_uniform_rk3": (1, 2, 1, "minmod", "uniform", 3, {"prandtl": 0.71, "inlet": True}),
+    "m1_ns_all_options_amr_rk3": (1, 2, 1, "vanalbada", "amr", 3,
+                                  {"prandtl": 0.71, "pvrs": True, "iprimtv": True}),
+    "m2_c3_iprimtv_amr_rk3": (2, 2, 0, "minmod", "amr", 3, {"c3": True, "iprimtv": True}),
+    "m3_iprimtv_amr_rk3": (3, 2, 0, "minmod", "amr", 3, {"iprimtv": True}),
 }
 
 
@@ -137,6 +151,12 @@ def _case(name):
     if opt.get("inlet"):
         bc = (1, 2, 2, 2, 2, 2)
         kw["inlet"] = INLET[model]
+    if opt.get("pvrs"):
+        kw["wave_speeds"] = "pvrs"
+    if opt.get("iprimtv"):
+        kw["iprimtv"] = 1
+    if opt.get("prandtl"):
+        kw["prandtl"] = opt["prandtl"]
     p = barotropic_cavitation_params(**kw) if opt.get("c3") else fv.params(model, **kw)
     F = _mesh(mesh, neq, bc)
     xc, _ = F.leaf_info()

@@ -503,3 +503,153 @@ def test_inlet_ghost_in_muscl_slope_enters_by_sign(side):
     exact = base + slope * h
     expect = exact if side < 0 else U_wall
     assert np.allclose(SF[1][:, wall], expect[:, None], rtol=1e-14, atol=1e-14)
+
+
+# ---------------------------------------------------------------- options beyond the reference
+
+def _exact_star_pressure(WL, WR, g=1.4):
+    """Star pressure of the exact Riemann problem of the ideal gas (Toro, Riemann Solvers,
+    ch. 4: f_L(p) + f_R(p) + u_R - u_L = 0, shock branch (p - p_K) sqrt(A_K / (p + B_K)),
+    rarefaction branch 2 a_K / (g - 1) ((p / p_K)^((g - 1) / (2 g)) - 1)), by Newton
+    iteration from the arithmetic mean."""
+    def f(p, W):
+        r, u, pk = W
+        a = math.sqrt(g * pk / r)
+        if p > pk:
+            A, B = 2.0 / ((g + 1) * r), (g - 1) / (g + 1) * pk
+            q = math.sqrt(A / (p + B))
+            return (p - pk) * q, q * (1.0 - 0.5 * (p - pk) / (B + p))
+        return (2 * a / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1),
+                1.0 / (r * a) * (p / pk) ** (-(g + 1) / (2 * g)))
+    p = 0.5 * (WL[2] + WR[2])
+    for _ in range(100):
+        fl, dl = f(p, WL)
+        fr, dr = f(p, WR)
+        dp = (fl + fr + WR[1] - WL[1]) / (dl + dr)
+        p -= dp
+        if abs(dp) < 1e-15 * p:
+            break
+    return p
+
+
+def _exact_speeds(WL, WR, g=1.4):
+    """S_L, S_R of the exact solution's outer waves as the HLLC estimates read them:
+    the shock speed u_K -+ a_K sqrt((g + 1)/(2 g) p*/p_K + (g - 1)/(2 g)) where p* > p_K,
+    the rarefaction head u_K -+ a_K otherwise."""
+    ps = _exact_star_pressure(WL, WR, g)
+    out = []
+    for W, sgn in ((WL, -1.0), (WR, 1.0)):
+        r, u, pk = W
+        a = math.sqrt(g * pk / r)
+        q = math.sqrt((g + 1) / (2 * g) * ps / pk + (g - 1) / (2 * g)) if ps > pk else 1.0
+        out.append(u + sgn * a * q)
+    return out
+
+
+def _u1d(W, g=1.4):
+    r, u, p = W
+    return prim_to_cons(np.array(r), np.array(u), np.array(0.0), np.array(0.0), np.array(p), g).ravel()
+
+
+def test_pvrs_speeds_limits():
+    """PVRS wave speeds (option): equal states give u -+ a, and two rarefactions (states
+    moving apart, p* below both pressures) give u_L - a_L and u_R + a_R exactly."""
+    p = fv.params(1, wave_speeds="pvrs")
+    R9 = _frame([1.0, 0, 0])
+    W = (1.3, 0.4, 2.0)
+    a = math.sqrt(1.4 * W[2] / W[0])
+    (SL, SS, SR), _, _ = oracle.star_states(p, _u1d(W), _u1d(W), R9)
+    assert SL == pytest.approx(W[1] - a, rel=1e-15) and SR == pytest.approx(W[1] + a, rel=1e-15)
+    assert SS == pytest.approx(W[1], rel=1e-14)
+    WL, WR = (1.0, -0.5, 1.0), (0.8, 0.6, 0.9)
+    (SL, SS, SR), _, _ = oracle.star_states(p, _u1d(WL), _u1d(WR), R9)
+    aL, aR = math.sqrt(1.4 * WL[2] / WL[0]), math.sqrt(1.4 * WR[2] / WR[0])
+    assert SL == pytest.approx(WL[1] - aL, rel=1e-15) and SR == pytest.approx(WR[1] + aR, rel=1e-15)
+    # the Davis speeds of the same states differ (min/max of u -+ a)
+    (dL, _, dR), _, _ = oracle.star_states(fv.params(1), _u1d(WL), _u1d(WR), R9)
+    assert dL == min(WL[1] - aL, WR[1] - aR) and dR == max(WL[1] + aL, WR[1] + aR)
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_pvrs_speeds_second_order_in_the_acoustic_limit(k):
+    """In the acoustic limit the linearised star pressure of PVRS is the exact one to
+    second order in the jump, so the PVRS wave speeds approach the exact outer-wave
+    speeds (shock speed from the Rankine-Hugoniot relation, or rarefaction head) as
+    eps^2: halving the jump divides the difference by about 4.  A wrong coefficient of
+    the shock factor (e.g. (g + 1)/2 * g) or of the pressure estimate leaves an O(eps)
+    difference."""
+    rng = np.random.default_rng(300 + k)
+    W0 = np.array([rng.uniform(0.5, 2.0), rng.uniform(-0.5, 0.5), rng.uniform(0.5, 2.0)])
+    d = rng.standard_normal(3) * np.array([0.3, 0.3, 0.3])
+    p = fv.params(1, wave_speeds="pvrs")
+    R9 = _frame([1.0, 0, 0])
+    errs = []
+    for eps in (2e-2, 1e-2, 5e-3):
+        WL, WR = tuple(W0 - eps * d), tuple(W0 + eps * d)
+        (SL, _, SR), _, _ = oracle.star_states(p, _u1d(WL), _u1d(WR), R9)
+        eL, eR = _exact_speeds(WL, WR)
+        errs.append(max(abs(SL - eL), abs(SR - eR)))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.0 < r1 < 5.5 and 3.0 < r2 < 5.5, errs
+    assert errs[-1] < 1e-3 * eps
+
+
+def test_heat_conduction_closed_form():
+    """Heat conduction (option): with u = 0 (no stress) and theta = p / rho = 1 + c x^2,
+    the Green-Gauss gradient of theta is 2 c x exactly in cells whose neighbours are
+    interior, its face mean 2 c x_face, and the heat flux adds V kappa 2 c to the energy
+    residual, kappa = mu gamma / ((gamma - 1) Pr): the residual with Pr = 0.71 minus the
+    residual without heat conduction, cell by cell, equals that closed form."""
+    F = uniform(4, 5)
+    xc, _ = F.leaf_info()
+    x = xc[:, 0]
+    c, mu, Pr, g = 0.8, 0.05, 0.71, 1.4
+    rho = 1.0 + 0.3 * x + 0.1 * xc[:, 1]
+    theta = 1.0 + c * x * x
+    z = np.zeros_like(x)
+    F.set_state(prim_to_cons(rho, z, z, z, rho * theta, g))
+    r_on = oracle.full_rhs(F, fv.params(1, ns=1, mu=mu, visc_work=1, prandtl=Pr))
+    r_off = oracle.full_rhs(F, fv.params(1, ns=1, mu=mu, visc_work=1))
+    h = 1.0 / 16
+    ix = np.floor(x / h).astype(int)
+    inner = (ix >= 2) & (ix <= 13)
+    kappa = mu * g / ((g - 1) * Pr)
+    expect = kappa * h ** 3 * 2 * c
+    d = r_on - r_off
+    assert np.allclose(d[4, inner], expect, rtol=1e-9, atol=0)
+    assert np.abs(d[:4]).max() == 0.0  # only the energy row carries heat
+    assert not np.allclose(d[4, ~inner], expect, rtol=1e-6)  # the wall cells differ
+
+
+def test_primitive_muscl_linear_fields_exact():
+    """MUSCL on primitive variables (option iprimtv = 1): with rho, u, v, w, p linear in
+    space the limited slopes are the exact gradients, so every interior face state is
+    the conserved state of the exact primitive values at the face centre
+    (prim2conc.c).  Conserved-variable MUSCL of the same field is not exact there
+    (E is quadratic), and neither is the reference's cons2primtv with squared
+    momenta."""
+    F = uniform(3, 5)
+    xc, _ = F.leaf_info()
+    G = np.array([[0.3, -0.2, 0.1], [0.5, 0.1, -0.4], [-0.2, 0.3, 0.2], [0.1, 0.1, 0.1],
+                  [0.4, -0.3, 0.6]])
+    base = np.array([2.0, 0.3, -0.2, 0.1, 3.0])
+    prim = lambda X: base[:, None] + G @ X.T
+    W = prim(xc)
+    F.set_state(prim_to_cons(*W))
+    h = 1.0 / 8
+    face_off = {0: (1, -1), 1: (0, 1), 2: (1, 1), 3: (0, -1), 4: (2, -1), 5: (2, 1)}
+    inner = np.all((xc > 1.5 * h) & (xc < 1 - 1.5 * h), axis=1)
+    for iprimtv in (1, 0):
+        oracle.faceinterp(F, fv.params(1, iprimtv=iprimtv))
+        SF = F.get_field(fv.FIELD_SF)
+        worst = 0.0
+        for f, (a, s) in face_off.items():
+            X = xc.copy()
+            X[:, a] += s * h / 2
+            expect = prim_to_cons(*prim(X))
+            worst = max(worst, (np.abs(SF[f][:, inner] - expect[:, inner]) /
+                                np.abs(expect[:, inner]).max(axis=1, keepdims=True)).max())
+        if iprimtv:
+            assert worst <= 1e-14, worst
+        else:
+            assert worst > 1e-6, worst
