@@ -124,6 +124,8 @@ _DECLS = [
     ("fvg_set_params", C.c_int, [_vp, C.POINTER(FvgParams)]),
     ("fvg_set_mesh", C.c_int, [_vp, C.POINTER(FvgMesh)]),
     ("fvg_ncell", C.c_int, [_vp]),
+    ("fvg_set_tiling", C.c_int, [_vp, C.c_int]),
+    ("fvg_tile_count", C.c_int, [_vp]),
     ("fvg_set_state", C.c_int, [_vp, _dp]),
     ("fvg_get_state", C.c_int, [_vp, _dp]),
     ("fvg_set_field", C.c_int, [_vp, C.c_int, _dp]),

@@ -22,7 +22,7 @@ ORACLE_DIR = REPO / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "liboracle.so"
 
 HOST_SOURCES = ["host/params.c", "host/forest.c", "host/flatten.c", "host/bind.c", "host/export.c"]
-HIP_SOURCES = ["fvgpu.hip", "fvadj.hip", "fvmesh.hip"]
+HIP_SOURCES = ["fvgpu.hip", "fvadj.hip", "fvmesh.hip", "fvtile.hip"]
 
 ARCH = os.environ.get("FV_OFFLOAD_ARCH", "gfx950")
 

@@ -408,6 +408,14 @@ class Solver:
     def sync(self) -> None:
         self._ck(lib().fvg_sync(self._h), "fvg_sync")
 
+    def set_tiling(self, on: bool) -> None:
+        """LDS-tiled stages for the interior 8^3 tiles (on by default)."""
+        self._ck(lib().fvg_set_tiling(self._h, 1 if on else 0), "fvg_set_tiling")
+
+    @property
+    def tile_count(self) -> int:
+        return int(lib().fvg_tile_count(self._h))
+
     def time_steps(self, n: int, dt: float, method: int = 3):
         tot = C.c_double()
         k = (C.c_double * 8)()

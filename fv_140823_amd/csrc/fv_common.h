@@ -148,7 +148,13 @@ __device__ __forceinline__ T limited(int lim, double eps, bool bnd, T d0, T d1, 
     const T e = (T)eps;
     T a = isinf(im) ? (T)0 : -d1 * im;
     T b = isinf(ip) ? (T)0 : d0 * ip;
-    grad = qdiv((a * (b * b + e) + b * (a * a + e)), (a * a + b * b + (T)2 * e));
+    /* (a (b^2 + e) + b (a^2 + e)) / (a^2 + b^2 + 2 e) with every fused multiply-add
+     * written out: left to the compiler, which of the two products of the numerator
+     * is fused depends on the surrounding code, and the face-list and LDS-tiled
+     * kernels, which must agree bit for bit, inline this in different code */
+    const T num = fma(a, fma(b, b, e), b * fma(a, a, e));
+    const T den = fma(a, a, fma(b, b, (T)2 * e));
+    grad = qdiv(num, den);
   }
   return grad;
 }
@@ -201,6 +207,15 @@ __device__ __forceinline__ int face_block(int n0, int n1, int n2, int r0, int r1
   if (lb < rb0 + rb1) { blk = lb - rb0; return 1; }
   blk = lb - rb0 - rb1;
   return 2;
+}
+
+/* whether face block blk of axis group ax is one the LDS-tiled stage covers (skip: one
+ * flag per block, the groups' blocks in axis order; null: none is) */
+__device__ __forceinline__ bool skipped_face_block(const uint8_t *skip, int n0, int n1, int ax,
+                                                   int blk) {
+  if (!skip) return false;
+  const int b0 = (n0 + FV_BLOCK - 1) / FV_BLOCK, b1 = (n1 + FV_BLOCK - 1) / FV_BLOCK;
+  return skip[(ax == 0 ? 0 : (ax == 1 ? b0 : b0 + b1)) + blk] != 0;
 }
 
 /* ------------------------------------------------------------------ context */
@@ -452,9 +467,24 @@ struct fvg_ctx {
   hipEvent_t ev[2 * 64 + 2];
   bool ev_ok = false;
   /* FP32 copy of S for the mixed ideal-gas face kernel; valid while nothing but
-   * that path's gather has written S since it was filled */
-  float *Sf = nullptr;
+   * that path's gather has written S since it was filled.  A stage reads Sf and writes
+   * the copy of its output into Sfx, then the two swap (the LDS-tiled stage reads its
+   * neighbour tiles' Sf while other workgroups write theirs). */
+  float *Sf = nullptr, *Sfx = nullptr;
   bool sf_valid = false;
+
+  /* LDS-tiled stage (fvtile.hip): the interior tiles (8^3 Morton blocks of one level
+   * whose six face-neighbour blocks are such tiles too) as [ntile][8] (first cell, the
+   * first cells of the neighbour tiles across faces 0..5, level), and per block of the
+   * stage's face kernel and gather a flag: all its faces / cells are the tiles' */
+  int32_t *tiles = nullptr;
+  int ntile = 0;
+  uint8_t *face_skip = nullptr, *gather_skip = nullptr;
+  bool face_all_skip = false, gather_all_skip = false;
+  bool tiling = true; /* fvg_set_tiling */
+  /* the skip flags of the stage being launched (null: the face kernel and the gather
+   * cover the whole mesh) */
+  const uint8_t *stage_fskip = nullptr, *stage_gskip = nullptr;
 
   /* device address of the static flag staging array of this context's GPU (taken once
    * per device at fvg_create; fvg_adapt_flags_static) and the status of that lookup */
@@ -558,4 +588,16 @@ static inline DevMesh dev_mesh(const fvg_ctx *c) {
   m.idist = c->tab + 25 * nl;
   return m;
 }
+
+/* ---- LDS-tiled stage (fvtile.hip) */
+/* Find the interior tiles of a mesh uploaded by fvg_set_mesh and upload their table and
+ * the skip flags of the stage's face kernel and gather; no tiles for other meshes. */
+int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m);
+void fv_tiles_release(fvg_ctx *c);
+/* One RK stage of the interior tiles' cells: MUSCL, flux, residual and the stage update
+ * in LDS; S -> Sout (out of place), SN, optional RHS and FP32 copy of Sout. */
+void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
+                          int irk, double dt, bool write_rhs, bool fast);
+/* whether the tiles can run this stage (mesh, options, precision path) */
+bool fv_tiles_usable(const fvg_ctx *c);
 

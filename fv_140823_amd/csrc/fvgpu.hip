@@ -324,7 +324,7 @@ template <int MODEL, bool NS, typename T, typename R, int OPT = 0>
 __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : (OPT ? 4 : 6)) void k_face_fused(
     DevMesh M, DevParams P, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
-    int *__restrict__ err) {
+    int *__restrict__ err, const uint8_t *__restrict__ skip) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   constexpr bool STAGE = RowStage<NEQ, R>::on;
   __shared__ R gbuf[STAGE ? FV_BLOCK * NEQ : 1];
@@ -333,6 +333,7 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : (OPT ? 4 
    * that range at most once per axis) */
   int blk;
   const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
+  if (skipped_face_block(skip, n0, n1, ax, blk)) return;
   const int q = blk * FV_BLOCK + t;
   const int jq = ax == 0 ? q : (ax == 1 ? n0 + q : n0 + n1 + q);
   R *dst = STAGE ? gbuf + t * NEQ : G + (size_t)jq * NEQ;
@@ -529,15 +530,17 @@ __global__ __launch_bounds__(FV_BLOCK, NS ? 7 : 8) void k_face_fast(DevMesh M, D
                                                                    const float *__restrict__ Sf,
                                                                    const float *__restrict__ SGm,
                                                                    float *__restrict__ G,
-                                                                   int *__restrict__ err) {
+                                                                   int *__restrict__ err,
+                                                                   const uint8_t *__restrict__ skip) {
   __shared__ int list[FV_BLOCK];
   __shared__ int nlist;
   const int t = threadIdx.x;
+  int blk;
+  const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
+  if (skipped_face_block(skip, n0, n1, ax, blk)) return;
   if (t == 0) nlist = 0;
   __syncthreads();
   bool done = true;
-  int blk;
-  const int ax = face_block(n0, n1, n2, r0, r1, r2, blk);
   const int q = blk * FV_BLOCK + t;
   /* FP32 rows (20 B) are stored directly: staging them through LDS as the FP64
    * kernels do measured slower here */
@@ -633,8 +636,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_face(DevMesh M, DevParams P, int b
   }
 }
 
-/* RHS gather (residual_faces_AMR.c:173-175, order f = 0..5, k = 0..3) fused with
- * the RK stage (RK.c:24-47).  irk 1..4; FIRST: also SN = S (RK(0)).
+/* RHS gather (residual_faces_AMR.c:173-175: the sum over a cell's faces and their
+ * sub-faces k = 0..3, here in the face order -x, +x, -y, +y, -z, +z where the reference
+ * runs f = 0..5) fused with the RK stage (RK.c:24-47).  irk 1..4; FIRST: also SN = S
+ * (RK(0)).
  * A block owns 256 consecutive cells, whose cell-major S / SN / RHS rows form one
  * contiguous span: it is moved between HBM and LDS with fully coalesced accesses
  * and each thread works on its own row in LDS.  The stage reads S and writes Sout,
@@ -648,14 +653,16 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
                                                         double *SN,
                                                         double *__restrict__ RHS, int irk,
                                                         double dt, int *__restrict__ err,
-                                                        double *Sout, float *Sf) {
+                                                        double *Sout, float *Sf,
+                                                        const uint8_t *__restrict__ skip) {
   __shared__ double shS[FV_BLOCK * NEQ];
   __shared__ double shN[FV_BLOCK * NEQ];
   const int lb = logical_block();
   const int t = threadIdx.x;
   const int i0 = lb * FV_BLOCK;
-  /* the grid is padded to a multiple of 8 blocks (gather_grid): the padding exits whole */
-  if (i0 >= M.N) return;
+  /* the grid is padded to a multiple of 8 blocks (gather_grid): the padding exits whole;
+   * so does a block whose cells the LDS-tiled stage updates */
+  if (i0 >= M.N || (skip && skip[lb])) return;
   const int i = i0 + t;
   const int nrow = min(FV_BLOCK, M.N - i0);
   const int nval = nrow * NEQ;
@@ -699,8 +706,12 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) r0[f][v] = G[j0 * NEQ + v];
     }
+    /* axis by axis: -x, +x, -y, +y, -z, +z (the LDS-tiled stage sums a cell's faces in
+     * the same order, one axis per pass) */
+    constexpr int ford[6] = {3, 1, 0, 2, 4, 5};
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
+    for (int q = 0; q < 6; ++q) {
+      const int f = ford[q];
       const int cls = (cf >> (2 * f)) & 3;
       double ff[NEQ];
 #pragma unroll
@@ -1025,15 +1036,16 @@ void fv_dcache_clear(fvg_ctx *c) {
 
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
+  fv_tiles_release(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, c->tab,
-                  c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf, c->keys,
+                  c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf, c->Sfx, c->keys,
                   c->hkeys, c->hval};
   for (void *p : ptrs) fv_dfree(c, p);
   c->cflag = nullptr; c->nbr = c->quad = c->fL = c->fR = c->fLL = c->fRR = c->gslot = nullptr;
   c->fmeta = nullptr; c->tab = nullptr;
   c->S = c->SN = c->RHS = c->SG = nullptr;
   c->SF = c->SGm = c->G = nullptr;
-  c->Sf = nullptr;
+  c->Sf = c->Sfx = nullptr;
   c->Sx = nullptr;
   c->stage_in = nullptr;
   c->keys = c->hkeys = nullptr;
@@ -1086,7 +1098,10 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)12 * ns * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->G, (size_t)neq * c->nfp * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->Sx, (size_t)neq * ns * 8, slack));
-  if (rs == 4 && c->prm.model == 1) HIPCHK(c, fv_dmalloc(c, &c->Sf, (size_t)neq * ns * 4, slack));
+  if (rs == 4 && c->prm.model == 1) {
+    HIPCHK(c, fv_dmalloc(c, &c->Sf, (size_t)neq * ns * 4, slack));
+    HIPCHK(c, fv_dmalloc(c, &c->Sfx, (size_t)neq * ns * 4, slack));
+  }
   c->sf_valid = false;
   c->stage_in = nullptr;
   hipStream_t st = c->stream;
@@ -1210,6 +1225,12 @@ int fvg_set_params(fvg_ctx *c, const fvg_params *p) {
 }
 
 int fvg_ncell(const fvg_ctx *c) { return c ? c->N : FVG_ERR_ARG; }
+int fvg_set_tiling(fvg_ctx *c, int on) {
+  if (!c) return FVG_ERR_ARG;
+  c->tiling = on != 0;
+  return FVG_OK;
+}
+int fvg_tile_count(const fvg_ctx *c) { return c ? c->ntile : FVG_ERR_ARG; }
 void *fvg_stream(fvg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 const char *fvg_last_error(const fvg_ctx *c) { return c ? c->last.c_str() : "null context"; }
 
@@ -1304,6 +1325,8 @@ int fvg_set_mesh(fvg_ctx *c, const fvg_mesh *m) {
   rc = fv_upload_tables(c, nl, m->h, m->dist, m->area, m->vol);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(st));
+  rc = fv_tiles_build(c, m);
+  if (rc) return rc;
   c->has_mesh = true;
   return FVG_OK;
 }
@@ -1496,7 +1519,8 @@ static void launch_face_fused_t(fvg_ctx *c, const double *S) {
   const int b0 = nblocks(n0), b1 = nblocks(n1), b2 = nblocks(n2);
   dim3 g(b0 + b1 + b2), b(FV_BLOCK);
   hipLaunchKernelGGL((k_face_fused<MODEL, NS, R, R, OPT>), g, b, 0, c->stream, M, c->dp, n0, n1, n2,
-                     c->nreg[0], c->nreg[1], c->nreg[2], S, (const R *)c->SGm, (R *)c->G, c->err);
+                     c->nreg[0], c->nreg[1], c->nreg[2], S, (const R *)c->SGm, (R *)c->G, c->err,
+                     c->stage_fskip);
 }
 
 template <int MODEL, bool NS, typename R>
@@ -1540,19 +1564,19 @@ static void launch_gather_t(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk
   const R *G = (const R *)c->G;
   if (write_rhs && !do_rk)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
   else if (write_rhs && first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
   else if (write_rhs)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
   else if (first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
   else
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
 }
 
 static void launch_gather(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first, int irk,
@@ -1631,15 +1655,15 @@ static void launch_face_fast(fvg_ctx *c, const double *S) {
   if (c->prm.ns && c->dp.kheat != 0.0)
     hipLaunchKernelGGL((k_face_fast<true, true>), g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
-                       (const float *)c->SGm, (float *)c->G, c->err);
+                       (const float *)c->SGm, (float *)c->G, c->err, c->stage_fskip);
   else if (c->prm.ns)
     hipLaunchKernelGGL(k_face_fast<true>, g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
-                       (const float *)c->SGm, (float *)c->G, c->err);
+                       (const float *)c->SGm, (float *)c->G, c->err, c->stage_fskip);
   else
     hipLaunchKernelGGL(k_face_fast<false>, g, b, 0, c->stream, M, c->dp, n0, n1, n2,
                        c->nreg[0], c->nreg[1], c->nreg[2], S, c->Sf,
-                       (const float *)c->SGm, (float *)c->G, c->err);
+                       (const float *)c->SGm, (float *)c->G, c->err, c->stage_fskip);
 }
 
 /* the FP32 state copy of the mixed ideal-gas path, refreshed if anything else
@@ -1677,16 +1701,30 @@ static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool
     if (ev) { (void)hipEventRecord(ev[*nev], c->stream); kid[*nev] = k; ++*nev; }
   };
   fast = fast && fast_mixed(c) && !first && sp.Sout != sp.S;
+  /* Interior tiles: one LDS-tiled kernel updates their cells and the face kernel and the
+   * gather skip the blocks that hold only their faces / cells.  Out-of-place stages
+   * only: a tile reads its neighbour tiles' S while other workgroups write Sout. */
+  const bool tiles = fv_tiles_usable(c) && !first && sp.Sout != sp.S && sp.Sout != sp.SN;
+  c->stage_fskip = tiles ? c->face_skip : nullptr;
+  c->stage_gskip = tiles ? c->gather_skip : nullptr;
   if (c->prm.ns) { mark(1); dispatch_grad(c, sp.S, false); }
   mark(2);
   /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
-  if (fast) launch_face_fast(c, sp.S);
-  else if (c->prm.ischeme == 2) dispatch_face_fused(c, sp.S);
-  else dispatch_face(c, sp.S);
+  if (!(tiles && c->face_all_skip)) {
+    if (fast) launch_face_fast(c, sp.S);
+    else if (c->prm.ischeme == 2) dispatch_face_fused(c, sp.S);
+    else dispatch_face(c, sp.S);
+  }
   mark(3);
-  sp.Sf = fast ? c->Sf : nullptr;
-  launch_gather(c, sp, write_rhs, true, first, irk, dt);
+  sp.Sf = fast ? c->Sfx : nullptr;
+  if (!(tiles && c->gather_all_skip)) launch_gather(c, sp, write_rhs, true, first, irk, dt);
+  if (tiles) {
+    mark(4);
+    fv_launch_tile_stage(c, sp.S, sp.SN, sp.Sout, sp.Sf, irk, dt, write_rhs, fast);
+  }
   mark(-1);
+  c->stage_fskip = c->stage_gskip = nullptr;
+  if (fast) std::swap(c->Sf, c->Sfx);
 }
 
 /* One time step (RK(0) + the method's stages), out of place: S^n stays in its
@@ -1750,10 +1788,12 @@ void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, i
   dim3 g(gather_grid(c->N)), b(FV_BLOCK);
   if (first)
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr);
+                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr,
+                       (const uint8_t *)nullptr);
   else
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, false>), g, b, 0, c->stream, M, G, S, SN,
-                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr);
+                       (double *)nullptr, irk, dt, c->err, S, (float *)nullptr,
+                       (const uint8_t *)nullptr);
 }
 
 extern "C" {
@@ -1975,19 +2015,30 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     };
     for (int k = 0; k < FV_NKT; ++k) ms_kernel[k] = 0.0;
     float *sf_dummy = fast ? (float *)c->SF : nullptr;
+    const bool tiles = fv_tiles_usable(c);
+    c->stage_fskip = tiles ? c->face_skip : nullptr;
+    c->stage_gskip = tiles ? c->gather_skip : nullptr;
     if (c->prm.ns) ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
-    ms_kernel[2] = batch([&] {
-      if (fast) launch_face_fast(c, c->S);
-      else if (c->prm.ischeme == 2) dispatch_face_fused(c, c->S);
-      else dispatch_face(c, c->S);
-    });
-    const double g1 = batch([&] {
-      launch_gather(c, StagePtrs{c->S, c->S, c->Sx, sf_dummy}, false, true, false, 1, dt);
-    });
-    const double g2 = batch([&] {
-      launch_gather(c, StagePtrs{c->S, c->SN, c->Sx, sf_dummy}, false, true, false, 2, dt);
-    });
-    ms_kernel[3] = (g1 + 2.0 * g2) / 3.0;
+    if (!(tiles && c->face_all_skip))
+      ms_kernel[2] = batch([&] {
+        if (fast) launch_face_fast(c, c->S);
+        else if (c->prm.ischeme == 2) dispatch_face_fused(c, c->S);
+        else dispatch_face(c, c->S);
+      });
+    if (!(tiles && c->gather_all_skip)) {
+      const double g1 = batch([&] {
+        launch_gather(c, StagePtrs{c->S, c->S, c->Sx, sf_dummy}, false, true, false, 1, dt);
+      });
+      const double g2 = batch([&] {
+        launch_gather(c, StagePtrs{c->S, c->SN, c->Sx, sf_dummy}, false, true, false, 2, dt);
+      });
+      ms_kernel[3] = (g1 + 2.0 * g2) / 3.0;
+    }
+    if (tiles)
+      ms_kernel[4] = batch([&] {
+        fv_launch_tile_stage(c, c->S, c->SN, c->Sx, sf_dummy, 2, dt, false, fast);
+      });
+    c->stage_fskip = c->stage_gskip = nullptr;
     if (ev_fail) return set_err(c, FVG_ERR_HIP, "kernel timing events failed");
   }
   int32_t bad;

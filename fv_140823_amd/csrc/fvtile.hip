@@ -1,0 +1,518 @@
+/*
+ * LDS-tiled RK stage for the uniform parts of a mesh.
+ *
+ * A tile is an 8 x 8 x 8 block of cells of one level that are consecutive in the leaf
+ * list (Morton order: the local index interleaves the bits of x, y, z with x lowest, as
+ * the depth-first order over kids by corner node gives, split.c:124-128); an interior
+ * tile is one whose six face-neighbour blocks are tiles too.  One workgroup of 512
+ * threads (one per cell) does a whole stage for an interior tile:
+ *   1. loads the tile and the two cell layers across each of its faces (1,280 cells,
+ *      the MUSCL stencils of all its faces) from HBM into LDS, once;
+ *   2. per axis, evaluates each face of the tile once from LDS: the regular-face MUSCL
+ *      of reconctruct_V3.c, the EOS, HLLC and (Navier-Stokes) the viscous flux with the
+ *      gradients k_grad wrote; the face's flux goes to an LDS buffer;
+ *   3. sums each cell's six face fluxes in the gather's order (-x, +x, -y, +y, -z, +z;
+ *      residual_faces_AMR.c:173-175) and applies the RK stage (RK.c:24-47).
+ * Face fluxes never go to HBM and every stencil row is read once per tile, where the
+ * face-list path writes the face flux, reads it back from both cells and fetches each
+ * stencil row for every face that uses it.  A face is evaluated exactly as the face-list
+ * kernel evaluates it (the same orientation: the lower-indexed cell is the owner), so a
+ * tile cell's new state is bitwise the one the face-list path gives.
+ *
+ * The rest of the mesh (tiles at walls or at level changes, hanging faces, anything
+ * that is not a tile) stays on the face-list kernels, which skip the blocks whose faces
+ * and cells are all the interior tiles'.
+ */
+#include <type_traits>
+
+#include "fv_common.h"
+
+namespace {
+
+constexpr int TT = 512;          /* cells (threads) of a tile */
+constexpr int TREG = 512 + 256;  /* the tile + the two slabs (2 x 8 x 8 cells) of one axis */
+constexpr int TFB = 576;         /* faces of one axis: 9 x 8 x 8 */
+
+/* local index of cell (x, y, z) of a tile: bits x0 y0 z0 x1 y1 z1 x2 y2 z2 */
+__host__ __device__ __forceinline__ int tmorton(int x, int y, int z) {
+  return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) | ((y & 2) << 3) |
+         ((z & 2) << 4) | ((x & 4) << 4) | ((y & 4) << 5) | ((z & 4) << 6);
+}
+
+__host__ __device__ __forceinline__ void demorton(int t, int &x, int &y, int &z) {
+  x = (t & 1) | ((t >> 2) & 2) | ((t >> 4) & 4);
+  y = ((t >> 1) & 1) | ((t >> 3) & 2) | ((t >> 5) & 4);
+  z = ((t >> 2) & 1) | ((t >> 4) & 2) | ((t >> 6) & 4);
+}
+
+/* a tile's first cell and those of its neighbour tiles (a row of the tile table:
+ * base, then across faces 0 -y, 1 +x, 2 +y, 3 -x, 4 -z, 5 +z, then the level) */
+struct TileLinks {
+  int base, my, px, py, mx, mz, pz;
+};
+
+/* cell coordinates of position p along axis a, the other two coordinates u, w (in
+ * axis order) */
+template <int a>
+__device__ __forceinline__ void along(int p, int u, int w, int &x, int &y, int &z) {
+  if constexpr (a == 0) { x = p; y = u; z = w; }
+  else if constexpr (a == 1) { x = u; y = p; z = w; }
+  else { x = u; y = w; z = p; }
+}
+
+/* Region during the pass of axis a: the tile (slot = local index) and the two cell
+ * layers on either side along a (slot 512 + side x 128 + depth x 64 + u x 8 + w; side 0
+ * below the tile, depth 0 next to it).  p in [-2, 10). */
+template <int a>
+__device__ __forceinline__ int reg_slot(int p, int u, int w) {
+  if (p < 0) return 512 + (-1 - p) * 64 + u * 8 + w;
+  if (p > 7) return 640 + (p - 8) * 64 + u * 8 + w;
+  int x, y, z;
+  along<a>(p, u, w, x, y, z);
+  return tmorton(x, y, z);
+}
+
+/* the cell at position p of the row u, w along axis a: the tile's, or the neighbour
+ * tile's below / above it.  The first cell is picked by value (indexing an array of the
+ * seven would put the array in scratch memory). */
+template <int a>
+__device__ __forceinline__ int reg_cell(const TileLinks &tb, int p, int u, int w) {
+  int b = tb.base;
+  if (p < 0) { b = a == 0 ? tb.mx : (a == 1 ? tb.my : tb.mz); p += 8; }
+  else if (p > 7) { b = a == 0 ? tb.px : (a == 1 ? tb.py : tb.pz); p -= 8; }
+  int x, y, z;
+  along<a>(p, u, w, x, y, z);
+  return b + tmorton(x, y, z);
+}
+
+/* One face of axis a between the cells at positions p - 1 and p (p in [0, 8]) of the
+ * row u, w.  The owner is the lower-indexed of the two cells, as in the face list:
+ * the flux is the face-list kernel's for that face (recon_regular, make_state,
+ * hllc_face, -Area), returned as the -1 side cell sees it through its +a face. */
+template <int MODEL, bool NS, int a, typename T, typename R, int OPT, bool FAST, typename ST>
+__device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, const TileLinks &tb,
+                                          int L, const ST *reg, const double *__restrict__ S,
+                                          const R *__restrict__ SGm, int p, int u, int w, R *out,
+                                          int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int cm = reg_cell<a>(tb, p - 1, u, w);
+  const int cp = reg_cell<a>(tb, p, u, w);
+  const bool own_minus = cm < cp;
+  /* stencil in the owner's orientation: far side of the owner, owner, other, far side */
+  const int r0 = reg_slot<a>(p - 2, u, w), r1 = reg_slot<a>(p - 1, u, w),
+            r2 = reg_slot<a>(p, u, w), r3 = reg_slot<a>(p + 1, u, w);
+  const int q0 = own_minus ? r0 : r3, q1 = own_minus ? r1 : r2, q2 = own_minus ? r2 : r1,
+            q3 = own_minus ? r3 : r0;
+  const int iL = own_minus ? cm : cp, iR = own_minus ? cp : cm;
+  const T s = own_minus ? (T)1 : (T)-1;
+  const int k = (L * 3 + a) * 3;
+  const double pcf = M.dist[2 * k], ic = M.idist[k];
+  FState<T> Ls, Rs;
+  bool ok = true;
+  bool done = false;
+  if constexpr (FAST) {
+    /* the FP32 face states of the mixed ideal-gas kernel (face_fast_body, regular) */
+    const float fp = (float)pcf, fic = (float)ic;
+    float U[5], V[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const float A0 = reg[v * TREG + q0], A1 = reg[v * TREG + q1];
+      const float B0 = reg[v * TREG + q2], B1 = reg[v * TREG + q3];
+      U[v] = A1 + fp * limited<float>(P.limiter, P.va_eps, false, B0 - A1, A0 - A1, fic, fic);
+      V[v] = B0 + fp * limited<float>(P.limiter, P.va_eps, false, A1 - B0, B1 - B0, fic, fic);
+    }
+    done = make_state_mixed_ideal(P, U, Ls);
+    done = make_state_mixed_ideal(P, V, Rs) && done;
+  }
+  if (!done) {
+    /* FP64 stencil values: from LDS, or (the FP32 inversion rejected a state) the FP64
+     * rebuild of face_rebuild_block from HBM */
+    double UL[NEQ], UR[NEQ];
+    const double *g0, *g1, *g2, *g3;
+    if constexpr (FAST) {
+      const int c0 = reg_cell<a>(tb, p - 2, u, w), c3 = reg_cell<a>(tb, p + 1, u, w);
+      g0 = S + (size_t)(own_minus ? c0 : c3) * NEQ;
+      g1 = S + (size_t)iL * NEQ;
+      g2 = S + (size_t)iR * NEQ;
+      g3 = S + (size_t)(own_minus ? c3 : c0) * NEQ;
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      double A0, A1, B0, B1;
+      if constexpr (FAST) {
+        A0 = g0[v]; A1 = g1[v]; B0 = g2[v]; B1 = g3[v];
+      } else {
+        A0 = reg[v * TREG + q0]; A1 = reg[v * TREG + q1];
+        B0 = reg[v * TREG + q2]; B1 = reg[v * TREG + q3];
+      }
+      const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0 - A1), (T)(A0 - A1), (T)ic, (T)ic);
+      const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1 - B0), (T)(B1 - B0), (T)ic, (T)ic);
+      UL[v] = A1 + (double)((T)pcf * gL);
+      UR[v] = B0 + (double)((T)pcf * gR);
+    }
+    ok = make_state<MODEL, T>(P, UL, true, Ls);
+    ok = make_state<MODEL, T>(P, UR, true, Rs) && ok;
+  }
+  if constexpr (NS) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      Ls.g[q] = (T)SGm[(size_t)q * M.ns + iL];
+      Rs.g[q] = (T)SGm[(size_t)q * M.ns + iR];
+    }
+    if constexpr ((OPT & OPT_HEAT) != 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        Ls.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iL];
+        Rs.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iR];
+      }
+    }
+  }
+  T o[NEQ];
+  ok = hllc_face<MODEL, NS, a, T, OPT>(P, Ls, Rs, s, o) && ok;
+  const T A = (T)M.area[3 * L + a];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const R g = (R)(-(A) * o[v]);
+    out[v] = own_minus ? g : -g;
+  }
+  if (!ok) atomicMin(err, iL);
+}
+
+/* the two slabs of axis a (256 cells) into the region's slab slots */
+template <int a, bool FAST, int NEQ, typename ST>
+__device__ __forceinline__ void load_slabs(const TileLinks &tb, const double *__restrict__ S,
+                                           const float *__restrict__ Sf, ST *reg, int t) {
+  if (t < 256) {
+    const int side = t >> 7, d = (t >> 6) & 1, u = (t >> 3) & 7, w = t & 7;
+    const int cell = reg_cell<a>(tb, side ? 8 + d : -1 - d, u, w);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      if constexpr (FAST) reg[v * TREG + 512 + t] = Sf[(size_t)cell * NEQ + v];
+      else reg[v * TREG + 512 + t] = S[(size_t)cell * NEQ + v];
+    }
+  }
+}
+
+/* One axis: the slabs of a are in the region.  The -a face of every cell (fb slot = its
+ * local index) and the +a faces on the tile's +a side (slots 512 + u x 8 + w, by the
+ * first wave) into fb; then each cell's -a and +a terms into its sum acc (the gather's
+ * terms: 0 + the signed flux, added to the sum). */
+template <int MODEL, bool NS, int a, typename T, typename R, int OPT, bool FAST, typename ST>
+__device__ __forceinline__ void tile_pass(const DevMesh &M, const DevParams &P, const TileLinks &tb,
+                                          int L, const ST *reg, R *fb, double *acc,
+                                          const double *__restrict__ S, const R *__restrict__ SGm,
+                                          int t, int cx, int cy, int cz, int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  const int c = a == 0 ? cx : (a == 1 ? cy : cz);
+  const int u = a == 0 ? cy : cx, w = a == 2 ? cy : cz;
+  R out[NEQ];
+  tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm, c, u, w, out, err);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) fb[v * TFB + t] = out[v];
+  if (t < 64) {
+    tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm, 8, t >> 3, t & 7, out, err);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) fb[v * TFB + 512 + t] = out[v];
+  }
+  __syncthreads();
+  const int sp = c < 7 ? tmorton(cx + (a == 0), cy + (a == 1), cz + (a == 2)) : 512 + u * 8 + w;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    double m = 0.0, q = 0.0;
+    m += -(double)fb[v * TFB + t];
+    q += (double)fb[v * TFB + sp];
+    double r = acc[v * TT + t];
+    r += m;
+    r += q;
+    acc[v * TT + t] = r;
+  }
+}
+
+/* Four waves per SIMD (128 VGPRs): two workgroups per CU, so that one's loads and
+ * barriers overlap the other's face arithmetic.  The running sums stay in LDS, not in
+ * registers, which is what lets the viscous FP64 faces fit that budget. */
+template <int MODEL, bool NS, typename T, typename R, int OPT, bool FAST>
+__global__ __launch_bounds__(TT, 4) void k_tile_stage(DevMesh M, DevParams P,
+                                                   const int32_t *__restrict__ tiles,
+                                                   const double *__restrict__ S,
+                                                   const float *__restrict__ Sf,
+                                                   const double *__restrict__ SN,
+                                                   const R *__restrict__ SGm, int irk, double dt,
+                                                   double *__restrict__ Sout,
+                                                   float *__restrict__ SfOut,
+                                                   double *__restrict__ RHS, int write_rhs,
+                                                   int *__restrict__ err) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  using ST = typename std::conditional<FAST, float, double>::type;
+  /* one LDS buffer: region [NEQ][TREG] (ST), face fluxes [NEQ][TFB] (R), sums [NEQ][TT]
+   * (double); region and fluxes are also the staging span of the stage update */
+  constexpr int B_REG = (int)sizeof(ST) * NEQ * TREG, B_FB = (int)sizeof(R) * NEQ * TFB;
+  static_assert(B_REG % 16 == 0 && B_FB % 16 == 0, "LDS carve alignment");
+  static_assert(B_REG + B_FB >= 8 * NEQ * TT, "staging span");
+  __shared__ __align__(16) unsigned char smem[B_REG + B_FB + 8 * NEQ * TT];
+  ST *reg = (ST *)smem;
+  R *fb = (R *)(smem + B_REG);
+  double *acc = (double *)(smem + B_REG + B_FB);
+  const int t = threadIdx.x;
+  const int32_t *row = tiles + (size_t)blockIdx.x * 8;
+  const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
+  const int L = row[7];
+  /* 1. the tile (its cells are consecutive: coalesced rows) and the x slabs into LDS */
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    if constexpr (FAST) reg[v * TREG + t] = Sf[(size_t)(tb.base + t) * NEQ + v];
+    else reg[v * TREG + t] = S[(size_t)(tb.base + t) * NEQ + v];
+    acc[v * TT + t] = 0.0;
+  }
+  load_slabs<0, FAST, NEQ, ST>(tb, S, Sf, reg, t);
+  __syncthreads();
+  int cx, cy, cz;
+  demorton(t, cx, cy, cz);
+  /* 2. faces axis by axis, each cell's sum in the gather's order (-x, +x, -y, +y, -z,
+   * +z); the next axis's slabs replace this one's once its faces are done */
+  tile_pass<MODEL, NS, 0, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
+  load_slabs<1, FAST, NEQ, ST>(tb, S, Sf, reg, t);
+  __syncthreads();
+  tile_pass<MODEL, NS, 1, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
+  load_slabs<2, FAST, NEQ, ST>(tb, S, Sf, reg, t);
+  __syncthreads();
+  tile_pass<MODEL, NS, 2, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
+  double rhs[NEQ];
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    rhs[v] = acc[v * TT + t];
+    bad |= (rhs[v] != rhs[v]);
+  }
+  if (bad) atomicMin(err, tb.base + t);
+  /* 3. the stage update of the tile's 512 consecutive cells, S / SN / Sout moved as one
+   * contiguous span through LDS (region and flux buffers are free now) */
+  double own[NEQ];
+  double *sh = (double *)(void *)smem;
+  if constexpr (FAST) {
+    __syncthreads();
+    for (int k = t; k < TT * NEQ; k += TT) sh[k] = S[(size_t)tb.base * NEQ + k];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) own[v] = sh[t * NEQ + v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) own[v] = reg[v * TREG + t];
+  }
+  __syncthreads();
+  for (int k = t; k < TT * NEQ; k += TT) sh[k] = SN[(size_t)tb.base * NEQ + k];
+  __syncthreads();
+  const double Vol = M.vol[L];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const double s = own[v], sn = sh[t * NEQ + v], r = rhs[v];
+    double o;
+    switch (irk) {
+      case 1: o = sn + dt * (1.0 / Vol) * r; break;
+      case 2: o = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
+      case 3: o = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
+      case 4: o = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
+      default: o = s; break;
+    }
+    own[v] = o;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) sh[t * NEQ + v] = own[v];
+  __syncthreads();
+  for (int k = t; k < TT * NEQ; k += TT) {
+    Sout[(size_t)tb.base * NEQ + k] = sh[k];
+    if (SfOut) SfOut[(size_t)tb.base * NEQ + k] = (float)sh[k];
+  }
+  if (write_rhs) {
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) sh[t * NEQ + v] = rhs[v];
+    __syncthreads();
+    for (int k = t; k < TT * NEQ; k += TT) RHS[(size_t)tb.base * NEQ + k] = sh[k];
+  }
+}
+
+template <int MODEL, bool NS, typename T, typename R, int OPT, bool FAST>
+void launch_tiles_t(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
+                    int irk, double dt, bool write_rhs) {
+  DevMesh M = dev_mesh(c);
+  hipLaunchKernelGGL((k_tile_stage<MODEL, NS, T, R, OPT, FAST>), dim3(c->ntile), dim3(TT), 0,
+                     c->stream, M, c->dp, (const int32_t *)c->tiles, S, (const float *)c->Sf, SN,
+                     (const R *)c->SGm, irk, dt, Sout, SfOut, c->RHS, write_rhs ? 1 : 0, c->err);
+}
+
+template <int MODEL, bool NS>
+void launch_tiles_m(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
+                    int irk, double dt, bool write_rhs, bool fast) {
+  const bool heat = c->dp.kheat != 0.0;
+  if (c->rsize == 8) {
+    if constexpr (MODEL == 1 && NS) {
+      if (heat) return launch_tiles_t<MODEL, NS, double, double, OPT_HEAT, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+    }
+    return launch_tiles_t<MODEL, NS, double, double, 0, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+  }
+  if constexpr (MODEL == 1) {
+    if (fast) {
+      if constexpr (NS) {
+        if (heat) return launch_tiles_t<1, NS, float, float, OPT_HEAT, true>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+      }
+      return launch_tiles_t<1, NS, float, float, 0, true>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+    }
+    if constexpr (NS) {
+      if (heat) return launch_tiles_t<1, NS, float, float, OPT_HEAT, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+    }
+  }
+  launch_tiles_t<MODEL, NS, float, float, 0, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs);
+}
+
+}  // namespace
+
+bool fv_tiles_usable(const fvg_ctx *c) {
+  /* MUSCL (the tile evaluates regular faces only), no option but heat conduction */
+  return c->tiling && c->ntile > 0 && c->prm.ischeme == 2 && c->prm.iprimtv == 0 &&
+         c->prm.wave_speeds == FVG_WAVES_DAVIS;
+}
+
+void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
+                          int irk, double dt, bool write_rhs, bool fast) {
+  switch (c->prm.model) {
+    case 1:
+      if (c->prm.ns) launch_tiles_m<1, true>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, fast);
+      else launch_tiles_m<1, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, fast);
+      break;
+    case 2: launch_tiles_m<2, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, false); break;
+    default:
+      if (c->prm.ns) launch_tiles_m<3, true>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, false);
+      else launch_tiles_m<3, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, false);
+      break;
+  }
+}
+
+void fv_tiles_release(fvg_ctx *c) {
+  if (c->tiles) (void)hipFree(c->tiles);
+  if (c->face_skip) (void)hipFree(c->face_skip);
+  if (c->gather_skip) (void)hipFree(c->gather_skip);
+  c->tiles = nullptr;
+  c->face_skip = c->gather_skip = nullptr;
+  c->ntile = 0;
+  c->face_all_skip = c->gather_all_skip = false;
+}
+
+/* ------------------------------------------------------------------ host: tiles */
+
+int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
+  fv_tiles_release(c);
+  const int N = m->ncell;
+  if (N < TT) return FVG_OK;
+  auto level = [&](int i) { return (int)((m->cflag[i] >> 16) & 0xff); };
+  auto cls = [&](int i, int f) { return (int)((m->cflag[i] >> (2 * f)) & 3); };
+  auto nbr = [&](int f, int i) { return m->nbr[(size_t)f * N + i]; };
+  /* candidate tiles: 512 consecutive cells from a multiple of 512 whose +x / +y / +z
+   * neighbours inside the block are the Morton ones */
+  std::vector<int32_t> tile_of((size_t)N, -1);
+  std::vector<int32_t> base;
+  static const int fplus[3] = {1, 2, 5};
+  for (int b = 0; b + TT <= N; b += TT) {
+    const int L = level(b);
+    bool ok = true;
+    for (int t = 0; t < TT && ok; ++t) {
+      const int i = b + t;
+      if (level(i) != L) { ok = false; break; }
+      int xyz[3];
+      demorton(t, xyz[0], xyz[1], xyz[2]);
+      for (int a = 0; a < 3 && ok; ++a) {
+        if (xyz[a] == 7) continue;
+        int q[3] = {xyz[0], xyz[1], xyz[2]};
+        ++q[a];
+        const int f = fplus[a];
+        ok = cls(i, f) == CLS_SAME && nbr(f, i) == b + tmorton(q[0], q[1], q[2]);
+      }
+    }
+    if (!ok) continue;
+    for (int t = 0; t < TT; ++t) tile_of[(size_t)b + t] = (int32_t)base.size();
+    base.push_back(b);
+  }
+  if (base.empty()) return FVG_OK;
+  /* interior tiles: across each face the six cell rows continue into another tile, at
+   * the Morton positions of its opposite side */
+  std::vector<int32_t> rows;
+  std::vector<uint8_t> interior((size_t)N, 0);
+  for (size_t k = 0; k < base.size(); ++k) {
+    const int b = base[k];
+    int32_t nb[6];
+    bool ok = true;
+    for (int f = 0; f < 6 && ok; ++f) {
+      const int a = c_face_axis_host(f);
+      const bool plus = f == 1 || f == 2 || f == 5;
+      int tn = -1;
+      for (int u = 0; u < 8 && ok; ++u)
+        for (int w = 0; w < 8 && ok; ++w) {
+          int in[3], out[3];
+          const int e = plus ? 7 : 0, o = plus ? 0 : 7;
+          if (a == 0) { in[0] = e; in[1] = u; in[2] = w; out[0] = o; out[1] = u; out[2] = w; }
+          else if (a == 1) { in[0] = u; in[1] = e; in[2] = w; out[0] = u; out[1] = o; out[2] = w; }
+          else { in[0] = u; in[1] = w; in[2] = e; out[0] = u; out[1] = w; out[2] = o; }
+          const int i = b + tmorton(in[0], in[1], in[2]);
+          if (cls(i, f) != CLS_SAME) { ok = false; break; }
+          const int n = nbr(f, i);
+          if (n < 0 || n >= N || tile_of[(size_t)n] < 0) { ok = false; break; }
+          const int t2 = tile_of[(size_t)n];
+          if (tn < 0) tn = t2;
+          if (t2 != tn || n != base[(size_t)t2] + tmorton(out[0], out[1], out[2])) ok = false;
+        }
+      nb[f] = ok ? base[(size_t)tn] : -1;
+    }
+    if (!ok) continue;
+    rows.push_back(b);
+    for (int f = 0; f < 6; ++f) rows.push_back(nb[f]);
+    rows.push_back(level(b));
+    for (int t = 0; t < TT; ++t) interior[(size_t)b + t] = 1;
+  }
+  const int nt = (int)(rows.size() / 8);
+  if (nt == 0) return FVG_OK;
+  /* skip flags: stage face-kernel blocks (per axis group, FV_BLOCK faces each) and
+   * gather blocks (FV_BLOCK cells) that hold only interior-tile faces / cells */
+  std::vector<uint8_t> fskip, gskip;
+  bool fall = true, gall = true;
+  int64_t start = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int n = m->nface_axis[a];
+    for (int q0 = 0; q0 < n; q0 += FV_BLOCK) {
+      bool all = true;
+      for (int q = q0; q < n && q < q0 + FV_BLOCK && all; ++q) {
+        const int64_t j = start + q;
+        all = interior[(size_t)m->face_L[j]] && interior[(size_t)m->face_R[j]];
+      }
+      fskip.push_back(all ? 1 : 0);
+      fall = fall && all;
+    }
+    start += n;
+  }
+  for (int i0 = 0; i0 < N; i0 += FV_BLOCK) {
+    bool all = true;
+    for (int i = i0; i < N && i < i0 + FV_BLOCK && all; ++i) all = interior[(size_t)i] != 0;
+    gskip.push_back(all ? 1 : 0);
+    gall = gall && all;
+  }
+  /* the gather's grid is padded to a multiple of 8 blocks; the padding has no cells */
+  while (gskip.size() % 8) gskip.push_back(1);
+  hipError_t e = hipMalloc((void **)&c->tiles, rows.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->face_skip, fskip.size());
+  if (e == hipSuccess) e = hipMalloc((void **)&c->gather_skip, gskip.size());
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->tiles, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->face_skip, fskip.data(), fskip.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->gather_skip, gskip.data(), gskip.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    fv_tiles_release(c);
+    return set_err(c, FVG_ERR_HIP, hipGetErrorString(e));
+  }
+  c->ntile = nt;
+  c->face_all_skip = fall;
+  c->gather_all_skip = gall;
+  return FVG_OK;
+}

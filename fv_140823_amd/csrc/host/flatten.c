@@ -17,8 +17,8 @@
  * its MUSCL stencil is four cells of one level (face_LL, face_L, face_R,
  * face_RR), and the kernels evaluate it without the class and neighbour lookups.
  * gslot[f][i] names the face (and side) whose flux enters leaf i through its
- * face f, in the reference's accumulation order f = 0..5, k = 0..3
- * (residual_faces_AMR.c:37-175).
+ * face f (for a finer neighbour the first of its 4 consecutive sub-faces, k = 0..3;
+ * residual_faces_AMR.c:37-175).
  * Geometry becomes per-level tables: cell size, the distance() factors per
  * neighbour class (distance.c:36-144), face area (meshproperties.c:29-65) and
  * volume (meshproperties.c:66-111).  Leaves that disagree with their level's

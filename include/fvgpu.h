@@ -197,6 +197,15 @@ int fvg_destroy(fvg_ctx *ctx);
 int fvg_set_params(fvg_ctx *ctx, const fvg_params *p);
 int fvg_set_mesh(fvg_ctx *ctx, const fvg_mesh *m);
 int fvg_ncell(const fvg_ctx *ctx);
+/* LDS-tiled stages (extension, no counterpart in the reference): fvg_set_mesh finds the
+ * interior tiles of the mesh (8 x 8 x 8 same-level blocks of consecutive cells whose six
+ * neighbour blocks are such tiles too); the out-of-place stages of fvg_steps /
+ * fvg_time_steps update their cells with one kernel that keeps the face fluxes in LDS,
+ * with results bitwise equal to the face-list path's.  The face flux buffer
+ * (FVG_FIELD_FACEFLUX) then holds only the faces outside them.  on = 0 turns it off
+ * (default on).  fvg_tile_count = the interior tiles of the current mesh. */
+int fvg_set_tiling(fvg_ctx *ctx, int on);
+int fvg_tile_count(const fvg_ctx *ctx);
 int fvg_set_state(fvg_ctx *ctx, const double *S);     /* host [neq][ncell] */
 int fvg_get_state(fvg_ctx *ctx, double *S);           /* host [neq][ncell] */
 int fvg_set_field(fvg_ctx *ctx, int which, const double *src);  /* S, SN, RHS, SF, SG from host f64 */

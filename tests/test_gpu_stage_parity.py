@@ -24,7 +24,8 @@ the stage formula with fused multiply-adds) give residuals that differ by a few
 
 The cases cover every face-kernel instantiation the workloads time: models 1/2/3,
 minmod and van Albada, Euler and Navier-Stokes (viscous work on), uniform, 2:1,
-periodic and periodic 2:1 meshes, an inlet side (bc 1), first order, and the
+periodic and periodic 2:1 meshes, 32^3 meshes whose interior 8^3 tiles the LDS-tiled
+stage kernel updates, an inlet side (bc 1), first order, and the
 three stage sequences (forward Euler, SSP-RK2 = irk 1,4, SSP-RK3 = irk 1,2,3); and
 the kernels' option variants: MUSCL on primitive variables, PVRS wave speeds and heat
 conduction, alone and together.
@@ -77,6 +78,11 @@ def _mesh(kind, neq, bc=(2, 2, 2, 2, 2, 2)):
         return uniform(3, neq, periodic=(1, 1, 1), bc=(0,) * 6)
     if kind == "periodic_amr":
         return amr(3, neq, periodic=(1, 1, 1))
+    # 32^3 meshes with interior 8^3 tiles: the LDS-tiled stage updates their cells
+    if kind == "periodic32":
+        return uniform(5, neq, periodic=(1, 1, 1), bc=(0,) * 6)
+    if kind == "walls32":
+        return uniform(5, neq, bc=bc)
     raise ValueError(kind)
 
 
@@ -138,6 +144,10 @@ CASES = {
                                   {"prandtl": 0.71, "pvrs": True, "iprimtv": True}),
     "m2_c3_iprimtv_amr_rk3": (2, 2, 0, "minmod", "amr", 3, {"c3": True, "iprimtv": True}),
     "m3_iprimtv_amr_rk3": (3, 2, 0, "minmod", "amr", 3, {"iprimtv": True}),
+    # meshes with interior tiles (k_tile_stage)
+    "m1_ns_heat_tiles_periodic32_rk3": (1, 2, 1, "minmod", "periodic32", 3, {"prandtl": 0.71}),
+    "m1_vanalbada_tiles_walls32_rk3": (1, 2, 0, "vanalbada", "walls32", 3, {}),
+    "m3_ns_tiles_walls32_rk2": (3, 2, 1, "minmod", "walls32", 2, {}),
 }
 
 
