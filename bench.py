@@ -35,7 +35,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["recon", "grad", "face", "gather_rk"]
+KERNELS = ["recon", "grad", "face", "gather_rk", "tile_stage"]  # fvg_time_steps kernel ids
 METRIC = "cell-updates/s, FP64 plain vs FP32-flux mixed @≤1e-6 rel L1; adjoint/fwd ratio"
 
 
@@ -87,14 +87,26 @@ def ref_bytes_per_cell(part, neq, f, w, ns_visc):
             "stage": k0 + k1 + k2 + k3 + k4}[part]
 
 
-def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage):
+def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
     """Bytes a launch of this build's fused kernels must move at least: each input
     read once, each output written once (MUSCL inside the face kernel).  rs is the
-    byte width of the face flux (8 plain, 4 mixed)."""
+    byte width of the face flux and the gradients (8 plain, 4 mixed).  For the
+    LDS-tiled stage (m["tiles"] interior 8^3 tiles) each tile reads its cells and two
+    cell layers beyond each face (1,280 cells) once, the gradients of its cells and of
+    the first of those layers (896 cells), SN, and writes the new state (and, mixed,
+    its FP32 copy, the region then being read from that copy)."""
     N, F, Q, Fr = m["N"], m["F"], m["Q"], m["Freg"]
     meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
-    if kernel == "grad":                       # momentum rows of S -> 9 gradients
-        return 8 * 3 * N + meta_c + 9 * rs * N
+    grows = 12 if heat else 9                  # gradient rows: momenta (+ p / rho)
+    if kernel == "grad":                       # S rows -> momentum (and p / rho) gradients
+        return 8 * (neq if heat else 3) * N + meta_c + grows * rs * N
+    if kernel == "tile_stage":
+        nt = m.get("tiles", 0)
+        region = 1280 * neq * (4 + 8 * 512 / 1280 if rs == 4 else 8)
+        per = region + 896 * grows * rs * (1 if ns_visc else 0) + 8 * neq * 512 * 2 + 32
+        if rs == 4:
+            per += 4 * neq * 512
+        return nt * per
     if kernel == "face":
         # S, mesh, face records (L, R, meta), far stencil cells of regular faces, flux
         b = 8 * neq * N + meta_c + 12 * F + 8 * Fr + neq * rs * F
@@ -299,7 +311,10 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
     S0 = F.get_state()
     g = _solver(flat, S0, p)
     dt = g.cfl_dt(meta["cfl"])
+    mf["tiles"] = g.tile_count
+    mf["grid_tile"] = 512 * g.tile_count
     g.close()
+    log(f"{mf['tiles']} interior 8^3 tiles ({512 * mf['tiles']} leaves) on the LDS-tiled stage", quiet)
     pm = type(p).from_buffer_copy(p)
     pm.precision = fv.FVG_MIXED
     log(f"timing plain FP64: {a.steps} steps after {a.warmup} warm-up", quiet)
@@ -313,16 +328,19 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
     log(f"mixed vs plain after {a.acc_steps} steps: {acc}", quiet)
     fpc = mf["F"] / mf["N"]
     fname = "k_face_fused" if p.ischeme == 2 else "k_face"
-    stage_kernels = ["grad", "face", "gather_rk"] if p.ns else ["face", "gather_rk"]
+    # the stage's launches: those that ran (with interior tiles the face kernel and the
+    # gather may cover nothing and are not launched)
+    stage_kernels = [k for k in ("grad", "face", "gather_rk", "tile_stage") if k_p[KERNELS.index(k)] > 0]
     names = {"grad": ("k_grad", mf["grid_grad"]), "face": (fname, mf["grid_face"]),
-             "gather_rk": ("k_gather_rk", mf["grid_cell"])}
+             "gather_rk": ("k_gather_rk", mf["grid_cell"]), "tile_stage": ("k_tile_stage", mf["grid_tile"])}
+    heat = bool(p.ns and p.prandtl > 0)
     # the roofline of one RK stage of the plain form: its launches' mean durations
     t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)
     B = ref_bytes_per_cell("stage", neq, fpc, 8, p.ns)
     achieved = B * mf["N"] / (t_stage * 1e-3) / 1e9
     tr = [measured_traffic(meta["name"], *names[k]) for k in stage_kernels]
     traffic = sum(tr) if all(t is not None for t in tr) else None
-    comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False) for k in stage_kernels)
+    comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False, heat) for k in stage_kernels)
     roof = dict(bound="hbm", kernel="RK stage: " + " + ".join(names[k][0] for k in stage_kernels),
                 achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
@@ -339,7 +357,7 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
                                frac=round(Bw * val / 1e9 / HBM_PEAK_GBS, 4))
     kern = {}
     for i, k in enumerate(KERNELS):
-        if k_p[i] <= 0 or k not in names:
+        if (k_p[i] <= 0 and k_m[i] <= 0) or k not in names:
             continue
         e = dict(name=names[k][0], plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
         tb = measured_traffic(meta["name"], *names[k])
@@ -348,12 +366,12 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
             e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
         for tag, ms, rs in (("plain", k_p[i], 8), ("mixed", k_m[i], 4)):
             if ms:
-                cb = compulsory_bytes(k, mf, neq, rs, p.ns, False)
+                cb = compulsory_bytes(k, mf, neq, rs, p.ns, False, heat)
                 e[f"{tag}_compulsory_bytes"] = int(cb)
                 e[f"{tag}_compulsory_gbs"] = round(cb / (ms * 1e-3) / 1e9, 1)
         kern[k] = e
     cfg = {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"], "hanging_quads": mf["Q"],
-           "regular_faces": mf["Freg"], "levels": mf["levels"],
+           "regular_faces": mf["Freg"], "levels": mf["levels"], "interior_tiles": mf["tiles"],
            "model": {1: f"ideal gas gamma {p.gamma:g}", 2: "barotropic (model 2)",
                      3: "two-phase barotropic (model 3)"}[p.model] + (" Navier-Stokes" if p.ns else ""),
            "scheme": ("MUSCL-" + ("minmod" if p.limiter == 0 else "van Albada") if p.ischeme == 2

@@ -142,6 +142,9 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
       const int nb = nbs[f];
       const int nk = cls == CLS_FINER ? 4 : 1;
       const double A = M.area[3 * (cls == CLS_FINER ? L + 1 : L) + d];
+      /* the time loop's rows: n A / V once per face, not one division per term (the
+       * reference's term order, 0.5 (Un + U) n A / V, is kept for gradcalc()) */
+      const double wgt = nd * A / V;
       for (int k = 0; k < nk; ++k) {
         double Un[NEQ];
         if (cls == CLS_FINER) load_S<NEQ>(S, M.ns, M.quad[4 * nb + k], Un);
@@ -158,7 +161,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
           for (int v = 0; v < NEQ; ++v) Un[v] = Ur[f][v];
         }
 #pragma unroll
-        for (int v = V0; v < V1; ++v) acc[v] += 0.5 * (Un[v] + U[v]) * nd * A / V;
+        for (int v = V0; v < V1; ++v) {
+          if constexpr (FULL) acc[v] += 0.5 * (Un[v] + U[v]) * nd * A / V;
+          else acc[v] += 0.5 * (Un[v] + U[v]) * wgt;
+        }
         if constexpr (FULL) {
           double r, u[3], e, p, c, y;
           eos64<MODEL>(P, Un, false, r, u, e, p, c, y);
@@ -168,7 +174,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
           const double thn = cls == CLS_FINER ? theta_ideal(P, Un)
                              : (cls == CLS_BOUNDARY ? (-1 - nb == 1 ? theta_ideal(P, Un) : town)
                                                     : thr[f]);
-          acct += 0.5 * (thn + town) * nd * A / V;
+          acct += 0.5 * (thn + town) * wgt;
         }
       }
     }
