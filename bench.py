@@ -35,7 +35,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["recon", "grad", "face", "gather_rk", "tile_stage"]  # fvg_time_steps kernel ids
+KERNELS = ["recon", "grad", "face", "gather_rk", "tile_stage", "tile_grad"]  # fvg_time_steps kernel ids
 METRIC = "cell-updates/s, FP64 plain vs FP32-flux mixed @≤1e-6 rel L1; adjoint/fwd ratio"
 
 
@@ -100,6 +100,8 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
     grows = 12 if heat else 9                  # gradient rows: momenta (+ p / rho)
     if kernel == "grad":                       # S rows -> momentum (and p / rho) gradients
         return 8 * (neq if heat else 3) * N + meta_c + grows * rs * N
+    if kernel == "tile_grad":                  # tile + one layer beyond each face -> gradients
+        return m.get("tiles", 0) * (896 * 8 * (neq if heat else 3) + 512 * grows * rs + 32)
     if kernel == "tile_stage":
         nt = m.get("tiles", 0)
         region = 1280 * neq * (4 + 8 * 512 / 1280 if rs == 4 else 8)
@@ -330,9 +332,11 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
     fname = "k_face_fused" if p.ischeme == 2 else "k_face"
     # the stage's launches: those that ran (with interior tiles the face kernel and the
     # gather may cover nothing and are not launched)
-    stage_kernels = [k for k in ("grad", "face", "gather_rk", "tile_stage") if k_p[KERNELS.index(k)] > 0]
+    stage_kernels = [k for k in ("grad", "tile_grad", "face", "gather_rk", "tile_stage")
+                     if k_p[KERNELS.index(k)] > 0]
     names = {"grad": ("k_grad", mf["grid_grad"]), "face": (fname, mf["grid_face"]),
-             "gather_rk": ("k_gather_rk", mf["grid_cell"]), "tile_stage": ("k_tile_stage", mf["grid_tile"])}
+             "gather_rk": ("k_gather_rk", mf["grid_cell"]), "tile_stage": ("k_tile_stage", mf["grid_tile"]),
+             "tile_grad": ("k_tile_grad", mf["grid_tile"])}
     heat = bool(p.ns and p.prandtl > 0)
     # the roofline of one RK stage of the plain form: its launches' mean durations
     t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)

@@ -209,6 +209,18 @@ __device__ __forceinline__ int face_block(int n0, int n1, int n2, int r0, int r1
   return 2;
 }
 
+/* theta = p / rho of an ideal / stiffened-gas state, (gamma - 1) e_int - gamma p_inf / rho
+ * (the p of facevalues.c:40 divided by rho), without the sound speed.  The fused
+ * multiply-adds are written out so that the gradient kernel and the LDS-tiled one,
+ * which inline this in different code, agree bit for bit. */
+__device__ __forceinline__ double theta_ideal(const DevParams &P, const double *U) {
+  const double ir = 1.0 / U[0];
+  const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  const double q = fma(u, u, fma(v, v, w * w));
+  const double ei = fma(-0.5, q, U[4] * ir);
+  return fma(P.gamma - 1.0, ei, -(P.gamma * P.pinf * ir));
+}
+
 /* whether face block blk of axis group ax is one the LDS-tiled stage covers (skip: one
  * flag per block, the groups' blocks in axis order; null: none is) */
 __device__ __forceinline__ bool skipped_face_block(const uint8_t *skip, int n0, int n1, int ax,
@@ -598,6 +610,9 @@ void fv_tiles_release(fvg_ctx *c);
  * in LDS; S -> Sout (out of place), SN, optional RHS and FP32 copy of Sout. */
 void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
                           int irk, double dt, bool write_rhs, bool fast);
+/* The momentum (and p / rho) gradients of the interior tiles' cells into SGm, bitwise
+ * those of k_grad, which skips those cells' blocks while c->stage_gskip is set. */
+void fv_launch_tile_grad(fvg_ctx *c, const double *S);
 /* whether the tiles can run this stage (mesh, options, precision path) */
 bool fv_tiles_usable(const fvg_ctx *c);
 

@@ -72,26 +72,21 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
   }
 }
 
-/* theta = p / rho of an ideal / stiffened-gas state, (gamma - 1) e_int - gamma p_inf / rho
- * (the p of facevalues.c:40 divided by rho), without the sound speed */
-__device__ __forceinline__ double theta_ideal(const DevParams &P, const double *U) {
-  const double ir = 1.0 / U[0];
-  const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
-  const double ei = (U[4] * ir) - 0.5 * (u * u + v * v + w * w);
-  return (P.gamma - 1.0) * ei - P.gamma * P.pinf * ir;
-}
-
 /* Green-Gauss gradients (gradcalc.c:23-42).  FULL: all neq+1 rows (incl. p) to
  * SG f64 [neq+1][3][ns]; otherwise the momentum rows 1..3 to SGm [9][ns]. */
 template <int MODEL, bool FULL, typename R, bool HEAT = false>
 __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
                                                    const double *__restrict__ S,
-                                                   double *__restrict__ SG, R *__restrict__ SGm) {
+                                                   double *__restrict__ SG, R *__restrict__ SGm,
+                                                   const uint8_t *__restrict__ skip) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   /* HEAT (heat conduction option, model 1): also the Green-Gauss gradient of p / rho,
    * from the same face means, into SGm rows 9..11 */
   static_assert(!HEAT || (MODEL == 1 && !FULL), "heat conduction: model 1, momentum rows");
-  const int i = logical_block() * FV_BLOCK + threadIdx.x;
+  const int lb = logical_block();
+  /* a block of cells that the LDS-tiled gradient kernel covers exits whole */
+  if (skip && skip[lb]) return;
+  const int i = lb * FV_BLOCK + threadIdx.x;
   if (i >= M.N) return;
   const uint32_t cf = M.cflag[i];
   const int L = (cf >> 16) & 0xff;
@@ -1463,24 +1458,25 @@ template <int MODEL>
 static void launch_grad(fvg_ctx *c, const double *S, bool full) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
+  const uint8_t *skip = full ? nullptr : c->stage_gskip;
   if (full)
     hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (double *)nullptr);
+                       (double *)nullptr, skip);
   else if (MODEL == 1 && c->dp.kheat != 0.0) {
     if constexpr (MODEL == 1) {
       if (c->rsize == 8)
         hipLaunchKernelGGL((k_grad<MODEL, false, double, true>), g, b, 0, c->stream, M, c->dp, S,
-                           c->SG, (double *)c->SGm);
+                           c->SG, (double *)c->SGm, skip);
       else
         hipLaunchKernelGGL((k_grad<MODEL, false, float, true>), g, b, 0, c->stream, M, c->dp, S,
-                           c->SG, (float *)c->SGm);
+                           c->SG, (float *)c->SGm, skip);
     }
   } else if (c->rsize == 8)
     hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (double *)c->SGm);
+                       (double *)c->SGm, skip);
   else
     hipLaunchKernelGGL((k_grad<MODEL, false, float>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (float *)c->SGm);
+                       (float *)c->SGm, skip);
 }
 
 /* The option variant a parameter set runs: none, the heat-only one (the viscous
@@ -1713,7 +1709,11 @@ static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool
   const bool tiles = fv_tiles_usable(c) && !first && sp.Sout != sp.S && sp.Sout != sp.SN;
   c->stage_fskip = tiles ? c->face_skip : nullptr;
   c->stage_gskip = tiles ? c->gather_skip : nullptr;
-  if (c->prm.ns) { mark(1); dispatch_grad(c, sp.S, false); }
+  if (c->prm.ns) {
+    mark(1);
+    if (!(tiles && c->gather_all_skip)) dispatch_grad(c, sp.S, false);
+    if (tiles) { mark(5); fv_launch_tile_grad(c, sp.S); }
+  }
   mark(2);
   /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
   if (!(tiles && c->face_all_skip)) {
@@ -1770,7 +1770,7 @@ void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd) {
   DevParams Pz = c->dp;
   for (int v = 0; v < 5; ++v) Pz.inlet[v] = 0.0;
   hipLaunchKernelGGL((k_grad<1, false, double>), dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
-                     M, Pz, Sd, (double *)nullptr, SGmd);
+                     M, Pz, Sd, (double *)nullptr, SGmd, (const uint8_t *)nullptr);
 }
 
 /* the adjoint's forward stages: always the FP64-read path, whose face states
@@ -2024,7 +2024,9 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     const bool tiles = fv_tiles_usable(c);
     c->stage_fskip = tiles ? c->face_skip : nullptr;
     c->stage_gskip = tiles ? c->gather_skip : nullptr;
-    if (c->prm.ns) ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
+    if (c->prm.ns && !(tiles && c->gather_all_skip))
+      ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
+    if (c->prm.ns && tiles) ms_kernel[5] = batch([&] { fv_launch_tile_grad(c, c->S); });
     if (!(tiles && c->face_all_skip))
       ms_kernel[2] = batch([&] {
         if (fast) launch_face_fast(c, c->S);

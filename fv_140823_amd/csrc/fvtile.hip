@@ -335,6 +335,112 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   }
 }
 
+/* Green-Gauss gradients of the interior tiles' cells: k_grad's momentum rows (and, with
+ * heat conduction, the gradient of p / rho) from the tile and the cell layer beyond each
+ * of its faces, staged once in LDS (no neighbour-index loads, coalesced rows).  The
+ * arithmetic is k_grad's, term for term, so the values are bitwise the same.  Halo slot
+ * of face f: 512 + f x 64 + u x 8 + w, (u, w) the other two coordinates in axis order. */
+template <int MODEL, typename R, bool HEAT>
+__global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
+                                                  const int32_t *__restrict__ tiles,
+                                                  const double *__restrict__ S,
+                                                  R *__restrict__ SGm) {
+  constexpr int NEQ = ModelTraits<MODEL>::NEQ;
+  constexpr int NV = HEAT ? NEQ : 3; /* staged rows: all (theta needs them) or the momenta */
+  constexpr int V0 = HEAT ? 0 : 1;   /* first staged variable */
+  constexpr int HR = TT + 384;
+  __shared__ double reg[NV * HR];
+  __shared__ double th[HEAT ? HR : 1];
+  const int t = threadIdx.x;
+  const int32_t *row = tiles + (size_t)blockIdx.x * 8;
+  const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
+  const int L = row[7];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) reg[v * HR + t] = S[(size_t)(tb.base + t) * NEQ + V0 + v];
+  if (t < 384) {
+    const int f = t >> 6, u = (t >> 3) & 7, w = t & 7;
+    int cell;
+    switch (f) {
+      case 0: cell = tb.my + tmorton(u, 7, w); break;
+      case 1: cell = tb.px + tmorton(0, u, w); break;
+      case 2: cell = tb.py + tmorton(u, 0, w); break;
+      case 3: cell = tb.mx + tmorton(7, u, w); break;
+      case 4: cell = tb.mz + tmorton(u, w, 7); break;
+      default: cell = tb.pz + tmorton(u, w, 0); break;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) reg[v * HR + TT + t] = S[(size_t)cell * NEQ + V0 + v];
+  }
+  __syncthreads();
+  if constexpr (HEAT) {
+    double U[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) U[v] = reg[v * HR + t];
+    th[t] = theta_ideal(P, U);
+    if (t < 384) {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) U[v] = reg[v * HR + TT + t];
+      th[TT + t] = theta_ideal(P, U);
+    }
+    __syncthreads();
+  }
+  int x, y, z;
+  demorton(t, x, y, z);
+  /* slot of the neighbour across face f */
+  auto nslot = [&](int f) -> int {
+    switch (f) {
+      case 0: return y > 0 ? tmorton(x, y - 1, z) : TT + 0 * 64 + x * 8 + z;
+      case 1: return x < 7 ? tmorton(x + 1, y, z) : TT + 1 * 64 + y * 8 + z;
+      case 2: return y < 7 ? tmorton(x, y + 1, z) : TT + 2 * 64 + x * 8 + z;
+      case 3: return x > 0 ? tmorton(x - 1, y, z) : TT + 3 * 64 + y * 8 + z;
+      case 4: return z > 0 ? tmorton(x, y, z - 1) : TT + 4 * 64 + x * 8 + y;
+      default: return z < 7 ? tmorton(x, y, z + 1) : TT + 5 * 64 + x * 8 + y;
+    }
+  };
+  const double V = M.vol[L];
+  double U[4];
+#pragma unroll
+  for (int v = 1; v < 4; ++v) U[v] = reg[(v - V0) * HR + t];
+  const double town = HEAT ? th[t] : 0.0;
+  const int i = tb.base + t;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int fa = d == 0 ? 1 : (d == 1 ? 0 : 4);
+    const int fb = d == 0 ? 3 : (d == 1 ? 2 : 5);
+    double acc[4], acct = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int f = k == 0 ? fa : fb;
+      const double nd = (f == 1 || f == 2 || f == 5) ? 1.0 : -1.0; /* c_face_sign */
+      const double A = M.area[3 * L + d];
+      const double wgt = nd * A / V;
+      const int r = nslot(f);
+#pragma unroll
+      for (int v = 1; v < 4; ++v) acc[v] += 0.5 * (reg[(v - V0) * HR + r] + U[v]) * wgt;
+      if constexpr (HEAT) acct += 0.5 * (th[r] + town) * wgt;
+    }
+    if constexpr (HEAT) SGm[((size_t)9 + d) * M.ns + i] = (R)acct;
+#pragma unroll
+    for (int v = 1; v < 4; ++v) SGm[((size_t)(v - 1) * 3 + d) * M.ns + i] = (R)acc[v];
+  }
+}
+
+template <int MODEL, typename R>
+void launch_tile_grad_r(fvg_ctx *c, const double *S) {
+  DevMesh M = dev_mesh(c);
+  if constexpr (MODEL == 1) {
+    if (c->dp.kheat != 0.0) {
+      hipLaunchKernelGGL((k_tile_grad<1, R, true>), dim3(c->ntile), dim3(TT), 0, c->stream, M, c->dp,
+                         (const int32_t *)c->tiles, S, (R *)c->SGm);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tile_grad<MODEL, R, false>), dim3(c->ntile), dim3(TT), 0, c->stream, M, c->dp,
+                     (const int32_t *)c->tiles, S, (R *)c->SGm);
+}
+
 template <int MODEL, bool NS, typename T, typename R, int OPT, bool FAST>
 void launch_tiles_t(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,
                     int irk, double dt, bool write_rhs) {
@@ -388,6 +494,17 @@ void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double 
       if (c->prm.ns) launch_tiles_m<3, true>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, false);
       else launch_tiles_m<3, false>(c, S, SN, Sout, SfOut, irk, dt, write_rhs, false);
       break;
+  }
+}
+
+void fv_launch_tile_grad(fvg_ctx *c, const double *S) {
+  if (c->prm.model == 2) return; /* no viscous form */
+  if (c->prm.model == 1) {
+    if (c->rsize == 8) launch_tile_grad_r<1, double>(c, S);
+    else launch_tile_grad_r<1, float>(c, S);
+  } else {
+    if (c->rsize == 8) launch_tile_grad_r<3, double>(c, S);
+    else launch_tile_grad_r<3, float>(c, S);
   }
 }
 

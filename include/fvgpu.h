@@ -289,8 +289,9 @@ int fvg_sync(fvg_ctx *ctx);
 const char *fvg_last_error(const fvg_ctx *ctx);
 /* nsteps time steps; *ms_total = their device time (two events around them).  Then
  * ms_kernel[k] = the mean duration (ms) of each kernel of a stage, by kernel id (1
- * gradient, 2 face flux, 3 gather+RK; others 0), from batches of back-to-back
- * launches of the same work after the steps (the state is left as the steps made it). */
+ * gradient, 2 face flux, 3 gather+RK, 4 LDS-tiled stage, 5 LDS-tiled gradient; 0 for a
+ * kernel the stage does not launch), from batches of back-to-back launches of the same
+ * work after the steps (the state is left as the steps made it). */
 int fvg_time_steps(fvg_ctx *ctx, int nsteps, int method, double dt, double *ms_total,
                    double *ms_kernel /* [8] */);
 
