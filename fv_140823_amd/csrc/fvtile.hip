@@ -45,6 +45,10 @@ __host__ __device__ __forceinline__ void demorton(int t, int &x, int &y, int &z)
   z = ((t >> 2) & 1) | ((t >> 4) & 2) | ((t >> 6) & 4);
 }
 
+/* Tiles are listed in leaf (Morton) order and taken in the XCD-contiguous order of
+ * logical_block(): each XCD works through one run of neighbouring tiles, whose slabs
+ * and gradient rows then come from its own L2. */
+
 /* a tile's first cell and those of its neighbour tiles (a row of the tile table:
  * base, then across faces 0 -y, 1 +x, 2 +y, 3 -x, 4 -z, 5 +z, then the level) */
 struct TileLinks {
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   R *fb = (R *)(smem + B_REG);
   double *acc = (double *)(smem + B_REG + B_FB);
   const int t = threadIdx.x;
-  const int32_t *row = tiles + (size_t)blockIdx.x * 8;
+  const int32_t *row = tiles + (size_t)logical_block() * 8;
   const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
   const int L = row[7];
   /* 1. the tile (its cells are consecutive: coalesced rows) and the x slabs into LDS */
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
   __shared__ double reg[NV * HR];
   __shared__ double th[HEAT ? HR : 1];
   const int t = threadIdx.x;
-  const int32_t *row = tiles + (size_t)blockIdx.x * 8;
+  const int32_t *row = tiles + (size_t)logical_block() * 8;
   const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
   const int L = row[7];
 #pragma unroll
