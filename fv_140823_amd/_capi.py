@@ -25,6 +25,7 @@ FVG_PLAIN, FVG_MIXED = 0, 1
 FVG_MINMOD, FVG_VANALBADA = 0, 1
 FVG_WAVES_DAVIS, FVG_WAVES_PVRS = 0, 1
 FIELD_S, FIELD_SN, FIELD_RHS, FIELD_SF, FIELD_SG, FIELD_FACEFLUX, FIELD_STAGE_IN = range(7)
+FVG_TILING_OFF, FVG_TILING_ON, FVG_TILING_AUTO = range(3)
 
 ERR_NAMES = {
     FVG_ERR_ARG: "bad argument",

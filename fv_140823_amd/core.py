@@ -408,9 +408,11 @@ class Solver:
     def sync(self) -> None:
         self._ck(lib().fvg_sync(self._h), "fvg_sync")
 
-    def set_tiling(self, on: bool) -> None:
-        """LDS-tiled stages for the interior 8^3 tiles (on by default)."""
-        self._ck(lib().fvg_set_tiling(self._h, 1 if on else 0), "fvg_set_tiling")
+    def set_tiling(self, mode) -> None:
+        """LDS-tiled stages for the interior 8^3 tiles: True / False force them on / off,
+        "auto" (the default) lets the library choose (fvg_set_tiling)."""
+        m = {"auto": 2, True: 1, False: 0}[mode] if isinstance(mode, (bool, str)) else int(mode)
+        self._ck(lib().fvg_set_tiling(self._h, m), "fvg_set_tiling")
 
     @property
     def tile_count(self) -> int:

@@ -493,7 +493,7 @@ struct fvg_ctx {
   int ntile = 0;
   uint8_t *face_skip = nullptr, *gather_skip = nullptr;
   bool face_all_skip = false, gather_all_skip = false;
-  bool tiling = true; /* fvg_set_tiling */
+  int tiling = FVG_TILING_AUTO; /* fvg_set_tiling */
   /* the skip flags of the stage being launched (null: the face kernel and the gather
    * cover the whole mesh) */
   const uint8_t *stage_fskip = nullptr, *stage_gskip = nullptr;

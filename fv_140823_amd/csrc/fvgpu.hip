@@ -1226,9 +1226,11 @@ int fvg_set_params(fvg_ctx *c, const fvg_params *p) {
 }
 
 int fvg_ncell(const fvg_ctx *c) { return c ? c->N : FVG_ERR_ARG; }
-int fvg_set_tiling(fvg_ctx *c, int on) {
+int fvg_set_tiling(fvg_ctx *c, int mode) {
   if (!c) return FVG_ERR_ARG;
-  c->tiling = on != 0;
+  if (mode != FVG_TILING_OFF && mode != FVG_TILING_ON && mode != FVG_TILING_AUTO)
+    return set_err(c, FVG_ERR_ARG, "tiling mode must be FVG_TILING_OFF, _ON or _AUTO");
+  c->tiling = mode;
   return FVG_OK;
 }
 int fvg_tile_count(const fvg_ctx *c) { return c ? c->ntile : FVG_ERR_ARG; }

@@ -202,9 +202,13 @@ int fvg_ncell(const fvg_ctx *ctx);
  * neighbour blocks are such tiles too); the out-of-place stages of fvg_steps /
  * fvg_time_steps update their cells with one kernel that keeps the face fluxes in LDS,
  * with results bitwise equal to the face-list path's.  The face flux buffer
- * (FVG_FIELD_FACEFLUX) then holds only the faces outside them.  on = 0 turns it off
- * (default on).  fvg_tile_count = the interior tiles of the current mesh. */
-int fvg_set_tiling(fvg_ctx *ctx, int on);
+ * (FVG_FIELD_FACEFLUX) then holds only the faces outside them.  mode: FVG_TILING_OFF,
+ * FVG_TILING_ON (whenever the mesh has interior tiles), FVG_TILING_AUTO (the default:
+ * the ideal gas with at least 256 interior tiles holding half of the leaves or more,
+ * where the tiled stage measured faster; the stiff models and meshes with few tiles keep
+ * the face-list kernels).  fvg_tile_count = the interior tiles of the current mesh. */
+enum { FVG_TILING_OFF = 0, FVG_TILING_ON = 1, FVG_TILING_AUTO = 2 };
+int fvg_set_tiling(fvg_ctx *ctx, int mode);
 int fvg_tile_count(const fvg_ctx *ctx);
 int fvg_set_state(fvg_ctx *ctx, const double *S);     /* host [neq][ncell] */
 int fvg_get_state(fvg_ctx *ctx, double *S);           /* host [neq][ncell] */

@@ -185,6 +185,9 @@ def test_timed_stage_matches_oracle(name):
     Fr, *_ = _case(name)  # a second forest of the same mesh for residuals of GPU inputs
     print(f"case stage-parity {name}: {F.nleaves} leaves, method {method}", flush=True)
     g = fv.Solver.from_forest(F, p)
+    if "tiles" in name:
+        g.set_tiling(True)  # the tiled stage also below the automatic choice's size
+        assert g.tile_count > 0
     dt = oracle.cfl_dt(F, p, 0.4)
     assert abs(g.cfl_dt(0.4) - dt) <= 1e-14 * dt
     st = STAGES[method]

@@ -140,3 +140,27 @@ def test_tiles_follow_the_mesh():
     _, (S_off, R_off, _) = _run(F, p, S, 3, dt, False)
     assert np.array_equal(S_on, S_off)
     assert np.array_equal(R_on, R_off)
+
+
+def test_tiling_modes():
+    """fvg_set_tiling: the automatic choice keeps the face-list kernels for a mesh with few
+    tiles (64 here, below 256) and the forced mode runs the tiled kernel on it; kernel id
+    4 of fvg_time_steps is the tiled stage's duration, 0 when it is not launched.  A mode
+    outside the three is refused."""
+    F = uniform(5, 5, periodic=(1, 1, 1), bc=(0,) * 6)
+    xc, _ = F.leaf_info()
+    g = fv.Solver.from_forest(F, fv.params(1))
+    g.set_state(state(1, xc, seed=4))
+    dt = g.cfl_dt(0.4)
+    _, k_auto = g.time_steps(2, dt, 3)
+    g.set_tiling(True)
+    _, k_on = g.time_steps(2, dt, 3)
+    g.set_tiling(False)
+    _, k_off = g.time_steps(2, dt, 3)
+    print(f"case tiling modes: tile kernel ms auto {k_auto[4]}, on {k_on[4]}, off {k_off[4]}", flush=True)
+    assert g.tile_count == 64
+    assert k_auto[4] == 0 and k_off[4] == 0 and k_on[4] > 0
+    assert k_on[2] == 0 and k_auto[2] > 0  # every face in a tile: no face-list launch when forced
+    with pytest.raises(fv.FvError):
+        g.set_tiling(5)
+    g.close()
