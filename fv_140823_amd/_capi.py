@@ -127,6 +127,7 @@ _DECLS = [
     ("fvg_ncell", C.c_int, [_vp]),
     ("fvg_set_tiling", C.c_int, [_vp, C.c_int]),
     ("fvg_tile_count", C.c_int, [_vp]),
+    ("fvg_set_graphs", C.c_int, [_vp, C.c_int]),
     ("fvg_set_state", C.c_int, [_vp, _dp]),
     ("fvg_get_state", C.c_int, [_vp, _dp]),
     ("fvg_set_field", C.c_int, [_vp, C.c_int, _dp]),

@@ -414,6 +414,10 @@ class Solver:
         m = {"auto": 2, True: 1, False: 0}[mode] if isinstance(mode, (bool, str)) else int(mode)
         self._ck(lib().fvg_set_tiling(self._h, m), "fvg_set_tiling")
 
+    def set_graphs(self, on: bool) -> None:
+        """Steps six at a time as one captured HIP graph (off by default)."""
+        self._ck(lib().fvg_set_graphs(self._h, 1 if on else 0), "fvg_set_graphs")
+
     @property
     def tile_count(self) -> int:
         return int(lib().fvg_tile_count(self._h))

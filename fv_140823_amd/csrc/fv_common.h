@@ -494,6 +494,12 @@ struct fvg_ctx {
   uint8_t *face_skip = nullptr, *gather_skip = nullptr;
   bool face_all_skip = false, gather_all_skip = false;
   int tiling = FVG_TILING_AUTO; /* fvg_set_tiling */
+  /* six time steps of fvg_steps captured as one HIP graph (fvg_set_graphs), replayed
+   * while everything its launches bake in (pointers, sizes, parameters, dt, method) is
+   * unchanged: step_graph_key holds those */
+  bool graphs = false;
+  hipGraphExec_t step_graph = nullptr;
+  std::vector<unsigned char> step_graph_key;
   /* the skip flags of the stage being launched (null: the face kernel and the gather
    * cover the whole mesh) */
   const uint8_t *stage_fskip = nullptr, *stage_gskip = nullptr;

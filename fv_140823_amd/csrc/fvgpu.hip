@@ -1035,9 +1035,16 @@ void fv_dcache_clear(fvg_ctx *c) {
   c->dcache.clear();
 }
 
+static void release_step_graph(fvg_ctx *c) {
+  if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+  c->step_graph = nullptr;
+  c->step_graph_key.clear();
+}
+
 static void free_mesh(fvg_ctx *c) {
   fv_adj_release(c);
   fv_tiles_release(c);
+  release_step_graph(c);
   void *ptrs[] = {c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->gslot, c->fmeta, c->tab,
                   c->S, c->SN, c->Sx, c->RHS, c->SG, c->SF, c->SGm, c->G, c->Sf, c->Sfx, c->keys,
                   c->hkeys, c->hval};
@@ -1234,6 +1241,12 @@ int fvg_set_tiling(fvg_ctx *c, int mode) {
   return FVG_OK;
 }
 int fvg_tile_count(const fvg_ctx *c) { return c ? c->ntile : FVG_ERR_ARG; }
+int fvg_set_graphs(fvg_ctx *c, int on) {
+  if (!c) return FVG_ERR_ARG;
+  c->graphs = on != 0;
+  if (!c->graphs) release_step_graph(c);
+  return FVG_OK;
+}
 void *fvg_stream(fvg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 const char *fvg_last_error(const fvg_ctx *c) { return c ? c->last.c_str() : "null context"; }
 
@@ -1763,6 +1776,97 @@ static void time_step(fvg_ctx *c, int method, double dt, bool write_rhs_last, hi
   c->sf_valid = fast; /* the last gather wrote the FP32 copy of the new S */
 }
 
+/* ---- six steps as one HIP graph */
+
+constexpr int kGraphSteps = 6; /* the state pointers cycle with period 3 (one or three
+                                  stages per step) or 2 (two), the FP32 copies with 2 */
+
+/* Everything the captured launches bake in: method, dt, the device pointers and sizes
+ * they are given, the parameters and the tiling decisions. */
+static void step_graph_key(const fvg_ctx *c, int method, double dt, std::vector<unsigned char> &k) {
+  struct Fixed {
+    int method, N, ns, F, nfp, Q, nlevel, ntile, nfa[3], nreg[3], usable, fall, gall, neq;
+    size_t rsize;
+    double dt;
+    const void *ptr[22];
+  } f{};
+  f.method = method;
+  f.N = c->N; f.ns = c->ns; f.F = c->F; f.nfp = c->nfp; f.Q = c->Q; f.nlevel = c->nlevel;
+  f.ntile = c->ntile;
+  for (int a = 0; a < 3; ++a) { f.nfa[a] = c->nfa[a]; f.nreg[a] = c->nreg[a]; }
+  f.usable = fv_tiles_usable(c) ? 1 : 0;
+  f.fall = c->face_all_skip ? 1 : 0;
+  f.gall = c->gather_all_skip ? 1 : 0;
+  f.neq = c->neq;
+  f.rsize = c->rsize;
+  f.dt = dt;
+  const void *p[22] = {c->S, c->SN, c->Sx, c->Sf, c->Sfx, c->G, c->SGm, c->RHS, c->SF, c->err,
+                       c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->fmeta, c->gslot,
+                       c->tab, c->tiles, c->face_skip};
+  for (int i = 0; i < 22; ++i) f.ptr[i] = p[i];
+  k.resize(sizeof f + sizeof c->dp + sizeof c->prm + sizeof(void *));
+  unsigned char *o = k.data();
+  memcpy(o, &f, sizeof f);
+  memcpy(o + sizeof f, &c->dp, sizeof c->dp);
+  memcpy(o + sizeof f + sizeof c->dp, &c->prm, sizeof c->prm);
+  const void *gs = c->gather_skip;
+  memcpy(o + sizeof f + sizeof c->dp + sizeof c->prm, &gs, sizeof gs);
+}
+
+/* The graph of kGraphSteps steps for (method, dt), captured now if the cached one does
+ * not match; false (eager launches) if capture fails. */
+static bool step_graph_ready(fvg_ctx *c, int method, double dt) {
+  if (fast_mixed(c)) ensure_sf(c); /* an eager conversion, not a node of the graph */
+  std::vector<unsigned char> key;
+  step_graph_key(c, method, dt, key);
+  if (c->step_graph && key == c->step_graph_key) return true;
+  release_step_graph(c);
+  double *S0 = c->S, *SN0 = c->SN, *Sx0 = c->Sx, *stage_in0 = c->stage_in;
+  float *Sf0 = c->Sf, *Sfx0 = c->Sfx;
+  const bool sfv0 = c->sf_valid;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  for (int s = 0; s < kGraphSteps; ++s) time_step(c, method, dt, false, nullptr, nullptr, nullptr);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  const bool cycled = c->S == S0 && c->SN == SN0 && c->Sx == Sx0 && c->Sf == Sf0 && c->Sfx == Sfx0;
+  /* nothing ran: the host's view of the buffers is the one before the capture */
+  c->S = S0; c->SN = SN0; c->Sx = Sx0; c->Sf = Sf0; c->Sfx = Sfx0;
+  c->stage_in = stage_in0;
+  c->sf_valid = sfv0;
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess && cycled) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess || !cycled || !x) {
+    if (x) (void)hipGraphExecDestroy(x);
+    (void)hipGetLastError();
+    return false;
+  }
+  c->step_graph = x;
+  c->step_graph_key = std::move(key);
+  return true;
+}
+
+/* nsteps time steps: whole groups of kGraphSteps as graph launches, the rest (always the
+ * last step, which writes RHS if asked) eagerly */
+static void run_steps(fvg_ctx *c, int nsteps, int method, double dt, bool write_rhs_last) {
+  int body = c->graphs ? (nsteps - 1) / kGraphSteps * kGraphSteps : 0;
+  if (body > 0 && !step_graph_ready(c, method, dt)) body = 0;
+  for (int r = 0; r < body / kGraphSteps; ++r) {
+    if (hipGraphLaunch(c->step_graph, c->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      release_step_graph(c);
+      body = r * kGraphSteps;
+      break;
+    }
+  }
+  if (body > 0) c->sf_valid = fast_mixed(c); /* the graph's last gather wrote the FP32 copy */
+  for (int s = body; s < nsteps; ++s)
+    time_step(c, method, dt, write_rhs_last && s == nsteps - 1, nullptr, nullptr, nullptr);
+}
+
 void fv_launch_recon(fvg_ctx *c, const double *S) { dispatch_recon(c, S); }
 
 void fv_launch_grad(fvg_ctx *c, const double *S) { dispatch_grad(c, S, false); }
@@ -1964,7 +2068,7 @@ int fvg_steps(fvg_ctx *c, int nsteps, int method, double dt) {
   (void)hipSetDevice(c->device);
   reset_err(c);
   /* RHS is written on the last stage only: afterwards it holds that stage's residual */
-  for (int s = 0; s < nsteps; ++s) time_step(c, method, dt, s == nsteps - 1, nullptr, nullptr, nullptr);
+  run_steps(c, nsteps, method, dt, true);
   int rc = finish(c, "steps");
   if (rc) return rc;
   int32_t bad;
@@ -1993,7 +2097,7 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipEvent_t &t0 = c->ev[2 * 64], &t1 = c->ev[2 * 64 + 1];
   HIPCHK(c, hipEventRecord(t0, c->stream));
-  for (int s = 0; s < nsteps; ++s) time_step(c, method, dt, false, nullptr, nullptr, nullptr);
+  run_steps(c, nsteps, method, dt, false);
   HIPCHK(c, hipEventRecord(t1, c->stream));
   HIPCHK(c, hipEventSynchronize(t1));
   float tot = 0.f;
