@@ -128,6 +128,7 @@ _DECLS = [
     ("fvg_set_tiling", C.c_int, [_vp, C.c_int]),
     ("fvg_tile_count", C.c_int, [_vp]),
     ("fvg_set_graphs", C.c_int, [_vp, C.c_int]),
+    ("fvh_tile_table", C.c_int, [C.POINTER(FvgMesh), C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
     ("fvg_set_state", C.c_int, [_vp, _dp]),
     ("fvg_get_state", C.c_int, [_vp, _dp]),
     ("fvg_set_field", C.c_int, [_vp, C.c_int, _dp]),

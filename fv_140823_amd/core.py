@@ -217,6 +217,17 @@ class FlatMesh:
     def nface(self) -> int:
         return self.mesh.nface
 
+    def tile_table(self) -> np.ndarray:
+        """The interior 8^3 tiles fvg_set_mesh finds on this mesh (host only): rows of
+        (first leaf, first leaves of the neighbour tiles across faces 0..5, level)."""
+        n = C.c_int()
+        check(lib().fvh_tile_table(C.byref(self.mesh), None, 0, C.byref(n)), "fvh_tile_table")
+        out = np.zeros((n.value, 8), np.int32)
+        if n.value:
+            check(lib().fvh_tile_table(C.byref(self.mesh), out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        n.value, C.byref(n)), "fvh_tile_table")
+        return out
+
     def arrays(self) -> dict:
         m = self.mesh
         N, F, Q, nl = m.ncell, m.nface, m.nquad, m.nlevel

@@ -532,10 +532,16 @@ void fv_tiles_release(fvg_ctx *c) {
 
 /* ------------------------------------------------------------------ host: tiles */
 
-int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
-  fv_tiles_release(c);
+/* The interior tiles of a mesh and the skip flags of the stage's face kernel and gather
+ * (host only). */
+static void find_tiles(const fvg_mesh *m, std::vector<int32_t> &rows, std::vector<uint8_t> &fskip,
+                       std::vector<uint8_t> &gskip, bool &fall, bool &gall) {
+  rows.clear();
+  fskip.clear();
+  gskip.clear();
+  fall = gall = false;
   const int N = m->ncell;
-  if (N < TT) return FVG_OK;
+  if (N < TT) return;
   auto level = [&](int i) { return (int)((m->cflag[i] >> 16) & 0xff); };
   auto cls = [&](int i, int f) { return (int)((m->cflag[i] >> (2 * f)) & 3); };
   auto nbr = [&](int f, int i) { return m->nbr[(size_t)f * N + i]; };
@@ -564,10 +570,9 @@ int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
     for (int t = 0; t < TT; ++t) tile_of[(size_t)b + t] = (int32_t)base.size();
     base.push_back(b);
   }
-  if (base.empty()) return FVG_OK;
+  if (base.empty()) return;
   /* interior tiles: across each face the six cell rows continue into another tile, at
    * the Morton positions of its opposite side */
-  std::vector<int32_t> rows;
   std::vector<uint8_t> interior((size_t)N, 0);
   for (size_t k = 0; k < base.size(); ++k) {
     const int b = base[k];
@@ -600,12 +605,10 @@ int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
     rows.push_back(level(b));
     for (int t = 0; t < TT; ++t) interior[(size_t)b + t] = 1;
   }
-  const int nt = (int)(rows.size() / 8);
-  if (nt == 0) return FVG_OK;
+  if (rows.empty()) return;
   /* skip flags: stage face-kernel blocks (per axis group, FV_BLOCK faces each) and
    * gather blocks (FV_BLOCK cells) that hold only interior-tile faces / cells */
-  std::vector<uint8_t> fskip, gskip;
-  bool fall = true, gall = true;
+  fall = gall = true;
   int64_t start = 0;
   for (int a = 0; a < 3; ++a) {
     const int n = m->nface_axis[a];
@@ -628,6 +631,16 @@ int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
   }
   /* the gather's grid is padded to a multiple of 8 blocks; the padding has no cells */
   while (gskip.size() % 8) gskip.push_back(1);
+}
+
+int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
+  fv_tiles_release(c);
+  std::vector<int32_t> rows;
+  std::vector<uint8_t> fskip, gskip;
+  bool fall, gall;
+  find_tiles(m, rows, fskip, gskip, fall, gall);
+  const int nt = (int)(rows.size() / 8);
+  if (nt == 0) return FVG_OK;
   hipError_t e = hipMalloc((void **)&c->tiles, rows.size() * sizeof(int32_t));
   if (e == hipSuccess) e = hipMalloc((void **)&c->face_skip, fskip.size());
   if (e == hipSuccess) e = hipMalloc((void **)&c->gather_skip, gskip.size());
@@ -645,5 +658,17 @@ int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
   c->ntile = nt;
   c->face_all_skip = fall;
   c->gather_all_skip = gall;
+  return FVG_OK;
+}
+
+extern "C" int fvh_tile_table(const fvg_mesh *m, int32_t *rows, int cap, int *ntile) {
+  if (!m || !ntile || (cap > 0 && !rows)) return FVG_ERR_ARG;
+  std::vector<int32_t> r;
+  std::vector<uint8_t> fskip, gskip;
+  bool fall, gall;
+  find_tiles(m, r, fskip, gskip, fall, gall);
+  *ntile = (int)(r.size() / 8);
+  const int n = *ntile < cap ? *ntile : cap;
+  if (n > 0) memcpy(rows, r.data(), (size_t)n * 8 * sizeof(int32_t));
   return FVG_OK;
 }

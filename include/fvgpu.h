@@ -209,6 +209,10 @@ int fvg_ncell(const fvg_ctx *ctx);
  * the face-list kernels).  fvg_tile_count = the interior tiles of the current mesh. */
 enum { FVG_TILING_OFF = 0, FVG_TILING_ON = 1, FVG_TILING_AUTO = 2 };
 int fvg_set_tiling(fvg_ctx *ctx, int mode);
+/* The interior tiles fvg_set_mesh would find in m, host only: *ntile = their count, and
+ * the first min(ntile, cap) rows [8] (first leaf; first leaves of the neighbour tiles
+ * across faces 0..5; level) into rows. */
+int fvh_tile_table(const fvg_mesh *m, int32_t *rows, int cap, int *ntile);
 int fvg_tile_count(const fvg_ctx *ctx);
 /* HIP graphs (extension): fvg_steps and fvg_time_steps run their steps six at a time as
  * one captured graph (six steps bring the rotating state buffers back to where they
