@@ -487,11 +487,14 @@ struct fvg_ctx {
 
   /* LDS-tiled stage (fvtile.hip): the interior tiles (8^3 Morton blocks of one level
    * whose six face-neighbour blocks are such tiles too) as [ntile][8] (first cell, the
-   * first cells of the neighbour tiles across faces 0..5, level), and per block of the
-   * stage's face kernel and gather a flag: all its faces / cells are the tiles' */
+   * first cells of the neighbour tiles across faces 0..5, level); per block of the
+   * stage's face kernel a flag (1: all its faces are the tiles'), per block of cells one
+   * (0 none, 1 all, 2 some of its cells are the tiles') */
   int32_t *tiles = nullptr;
   int ntile = 0;
   uint8_t *face_skip = nullptr, *gather_skip = nullptr;
+  uint8_t *tile_cell = nullptr; /* per cell: 1 in an interior tile (only where a cell block
+                                   is partly covered: gather_skip 2) */
   bool face_all_skip = false, gather_all_skip = false;
   int tiling = FVG_TILING_AUTO; /* fvg_set_tiling */
   /* six time steps of fvg_steps captured as one HIP graph (fvg_set_graphs), replayed
@@ -502,7 +505,7 @@ struct fvg_ctx {
   std::vector<unsigned char> step_graph_key;
   /* the skip flags of the stage being launched (null: the face kernel and the gather
    * cover the whole mesh) */
-  const uint8_t *stage_fskip = nullptr, *stage_gskip = nullptr;
+  const uint8_t *stage_fskip = nullptr, *stage_gskip = nullptr, *stage_cmask = nullptr;
 
   /* device address of the static flag staging array of this context's GPU (taken once
    * per device at fvg_create; fvg_adapt_flags_static) and the status of that lookup */

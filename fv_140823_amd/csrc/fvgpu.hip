@@ -78,16 +78,19 @@ template <int MODEL, bool FULL, typename R, bool HEAT = false>
 __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
                                                    const double *__restrict__ S,
                                                    double *__restrict__ SG, R *__restrict__ SGm,
-                                                   const uint8_t *__restrict__ skip) {
+                                                   const uint8_t *__restrict__ skip,
+                                                   const uint8_t *__restrict__ cmask) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   /* HEAT (heat conduction option, model 1): also the Green-Gauss gradient of p / rho,
    * from the same face means, into SGm rows 9..11 */
   static_assert(!HEAT || (MODEL == 1 && !FULL), "heat conduction: model 1, momentum rows");
   const int lb = logical_block();
-  /* a block of cells that the LDS-tiled gradient kernel covers exits whole */
-  if (skip && skip[lb]) return;
+  /* a block of cells that the LDS-tiled gradient kernel covers exits whole; in a block it
+   * covers in part, its cells do */
+  const int sk = skip ? skip[lb] : 0;
+  if (sk == 1) return;
   const int i = lb * FV_BLOCK + threadIdx.x;
-  if (i >= M.N) return;
+  if (i >= M.N || (sk == 2 && cmask[i])) return;
   const uint32_t cf = M.cflag[i];
   const int L = (cf >> 16) & 0xff;
   const double V = M.vol[L];
@@ -655,15 +658,19 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
                                                         double *__restrict__ RHS, int irk,
                                                         double dt, int *__restrict__ err,
                                                         double *Sout, float *Sf,
-                                                        const uint8_t *__restrict__ skip) {
+                                                        const uint8_t *__restrict__ skip,
+                                                        const uint8_t *__restrict__ cmask) {
   __shared__ double shS[FV_BLOCK * NEQ];
   __shared__ double shN[FV_BLOCK * NEQ];
   const int lb = logical_block();
   const int t = threadIdx.x;
   const int i0 = lb * FV_BLOCK;
   /* the grid is padded to a multiple of 8 blocks (gather_grid): the padding exits whole;
-   * so does a block whose cells the LDS-tiled stage updates */
-  if (i0 >= M.N || (skip && skip[lb])) return;
+   * so does a block whose cells the LDS-tiled stage updates.  In a block whose cells it
+   * updates in part (skip 2), cm marks them: their rows are neither checked nor written. */
+  const int sk = (skip && i0 < M.N) ? skip[lb] : 0;
+  if (i0 >= M.N || sk == 1) return;
+  const uint8_t *cm = sk == 2 ? cmask : nullptr;
   const int i = i0 + t;
   const int nrow = min(FV_BLOCK, M.N - i0);
   const int nval = nrow * NEQ;
@@ -740,7 +747,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
     bool bad = false;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) bad |= (rhs[v] != rhs[v]);
-    if (bad) atomicMin(err, i);
+    if (bad && !(cm && cm[i])) atomicMin(err, i);
   }
   if constexpr (DO_RK) {
 #pragma unroll
@@ -781,6 +788,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
   __syncthreads();
   if constexpr (DO_RK) {
     for (int k = t; k < nval; k += FV_BLOCK) {
+      if (cm && cm[i0 + k / NEQ]) continue;
       Sout[base + k] = shS[k];
       if (Sf) Sf[base + k] = (float)shS[k];
       if constexpr (FIRST) SN[base + k] = shN[k];
@@ -792,10 +800,12 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
         for (int v = 0; v < NEQ; ++v) shS[t * NEQ + v] = rhs[v];
       }
       __syncthreads();
-      for (int k = t; k < nval; k += FV_BLOCK) RHS[base + k] = shS[k];
+      for (int k = t; k < nval; k += FV_BLOCK)
+        if (!(cm && cm[i0 + k / NEQ])) RHS[base + k] = shS[k];
     }
   } else if constexpr (WRITE_RHS) {
-    for (int k = t; k < nval; k += FV_BLOCK) RHS[base + k] = shS[k];
+    for (int k = t; k < nval; k += FV_BLOCK)
+      if (!(cm && cm[i0 + k / NEQ])) RHS[base + k] = shS[k];
   }
 }
 
@@ -1473,25 +1483,25 @@ template <int MODEL>
 static void launch_grad(fvg_ctx *c, const double *S, bool full) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
-  const uint8_t *skip = full ? nullptr : c->stage_gskip;
+  const uint8_t *skip = full ? nullptr : c->stage_gskip, *cmask = full ? nullptr : c->stage_cmask;
   if (full)
     hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (double *)nullptr, skip);
+                       (double *)nullptr, skip, cmask);
   else if (MODEL == 1 && c->dp.kheat != 0.0) {
     if constexpr (MODEL == 1) {
       if (c->rsize == 8)
         hipLaunchKernelGGL((k_grad<MODEL, false, double, true>), g, b, 0, c->stream, M, c->dp, S,
-                           c->SG, (double *)c->SGm, skip);
+                           c->SG, (double *)c->SGm, skip, cmask);
       else
         hipLaunchKernelGGL((k_grad<MODEL, false, float, true>), g, b, 0, c->stream, M, c->dp, S,
-                           c->SG, (float *)c->SGm, skip);
+                           c->SG, (float *)c->SGm, skip, cmask);
     }
   } else if (c->rsize == 8)
     hipLaunchKernelGGL((k_grad<MODEL, false, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (double *)c->SGm, skip);
+                       (double *)c->SGm, skip, cmask);
   else
     hipLaunchKernelGGL((k_grad<MODEL, false, float>), g, b, 0, c->stream, M, c->dp, S, c->SG,
-                       (float *)c->SGm, skip);
+                       (float *)c->SGm, skip, cmask);
 }
 
 /* The option variant a parameter set runs: none, the heat-only one (the viscous
@@ -1581,19 +1591,24 @@ static void launch_gather_t(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk
   const R *G = (const R *)c->G;
   if (write_rhs && !do_rk)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, false, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip,
+                       c->stage_cmask);
   else if (write_rhs && first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip,
+                       c->stage_cmask);
   else if (write_rhs)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, true, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip,
+                       c->stage_cmask);
   else if (first)
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, true>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip,
+                       c->stage_cmask);
   else
     hipLaunchKernelGGL((k_gather_rk<NEQ, R, false, true, false>), g, b, 0, c->stream, M, G, sp.S,
-                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip);
+                       sp.SN, c->RHS, irk, dt, c->err, sp.Sout, sp.Sf, c->stage_gskip,
+                       c->stage_cmask);
 }
 
 static void launch_gather(fvg_ctx *c, StagePtrs sp, bool write_rhs, bool do_rk, bool first, int irk,
@@ -1724,6 +1739,7 @@ static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool
   const bool tiles = fv_tiles_usable(c) && !first && sp.Sout != sp.S && sp.Sout != sp.SN;
   c->stage_fskip = tiles ? c->face_skip : nullptr;
   c->stage_gskip = tiles ? c->gather_skip : nullptr;
+  c->stage_cmask = tiles ? c->tile_cell : nullptr;
   if (c->prm.ns) {
     mark(1);
     if (!(tiles && c->gather_all_skip)) dispatch_grad(c, sp.S, false);
@@ -1744,7 +1760,7 @@ static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool
     fv_launch_tile_stage(c, sp.S, sp.SN, sp.Sout, sp.Sf, irk, dt, write_rhs, fast);
   }
   mark(-1);
-  c->stage_fskip = c->stage_gskip = nullptr;
+  c->stage_fskip = c->stage_gskip = c->stage_cmask = nullptr;
   if (fast) std::swap(c->Sf, c->Sfx);
 }
 
@@ -1788,7 +1804,7 @@ static void step_graph_key(const fvg_ctx *c, int method, double dt, std::vector<
     int method, N, ns, F, nfp, Q, nlevel, ntile, nfa[3], nreg[3], usable, fall, gall, neq;
     size_t rsize;
     double dt;
-    const void *ptr[22];
+    const void *ptr[23];
   } f{};
   f.method = method;
   f.N = c->N; f.ns = c->ns; f.F = c->F; f.nfp = c->nfp; f.Q = c->Q; f.nlevel = c->nlevel;
@@ -1800,10 +1816,10 @@ static void step_graph_key(const fvg_ctx *c, int method, double dt, std::vector<
   f.neq = c->neq;
   f.rsize = c->rsize;
   f.dt = dt;
-  const void *p[22] = {c->S, c->SN, c->Sx, c->Sf, c->Sfx, c->G, c->SGm, c->RHS, c->SF, c->err,
+  const void *p[23] = {c->S, c->SN, c->Sx, c->Sf, c->Sfx, c->G, c->SGm, c->RHS, c->SF, c->err,
                        c->cflag, c->nbr, c->quad, c->fL, c->fR, c->fLL, c->fRR, c->fmeta, c->gslot,
-                       c->tab, c->tiles, c->face_skip};
-  for (int i = 0; i < 22; ++i) f.ptr[i] = p[i];
+                       c->tab, c->tiles, c->face_skip, c->tile_cell};
+  for (int i = 0; i < 23; ++i) f.ptr[i] = p[i];
   k.resize(sizeof f + sizeof c->dp + sizeof c->prm + sizeof(void *));
   unsigned char *o = k.data();
   memcpy(o, &f, sizeof f);
@@ -1876,7 +1892,8 @@ void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd) {
   DevParams Pz = c->dp;
   for (int v = 0; v < 5; ++v) Pz.inlet[v] = 0.0;
   hipLaunchKernelGGL((k_grad<1, false, double>), dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
-                     M, Pz, Sd, (double *)nullptr, SGmd, (const uint8_t *)nullptr);
+                     M, Pz, Sd, (double *)nullptr, SGmd, (const uint8_t *)nullptr,
+                     (const uint8_t *)nullptr);
 }
 
 /* the adjoint's forward stages: always the FP64-read path, whose face states
@@ -1901,11 +1918,11 @@ void fv_launch_gather_ptrs(fvg_ctx *c, const double *G, double *S, double *SN, i
   if (first)
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, true>), g, b, 0, c->stream, M, G, S, SN,
                        (double *)nullptr, irk, dt, c->err, S, (float *)nullptr,
-                       (const uint8_t *)nullptr);
+                       (const uint8_t *)nullptr, (const uint8_t *)nullptr);
   else
     hipLaunchKernelGGL((k_gather_rk<5, double, false, true, false>), g, b, 0, c->stream, M, G, S, SN,
                        (double *)nullptr, irk, dt, c->err, S, (float *)nullptr,
-                       (const uint8_t *)nullptr);
+                       (const uint8_t *)nullptr, (const uint8_t *)nullptr);
 }
 
 extern "C" {
@@ -2130,6 +2147,7 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     const bool tiles = fv_tiles_usable(c);
     c->stage_fskip = tiles ? c->face_skip : nullptr;
     c->stage_gskip = tiles ? c->gather_skip : nullptr;
+    c->stage_cmask = tiles ? c->tile_cell : nullptr;
     if (c->prm.ns && !(tiles && c->gather_all_skip))
       ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
     if (c->prm.ns && tiles) ms_kernel[5] = batch([&] { fv_launch_tile_grad(c, c->S); });
@@ -2152,7 +2170,7 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
       ms_kernel[4] = batch([&] {
         fv_launch_tile_stage(c, c->S, c->SN, c->Sx, sf_dummy, 2, dt, false, fast);
       });
-    c->stage_fskip = c->stage_gskip = nullptr;
+    c->stage_fskip = c->stage_gskip = c->stage_cmask = nullptr;
     if (ev_fail) return set_err(c, FVG_ERR_HIP, "kernel timing events failed");
   }
   int32_t bad;

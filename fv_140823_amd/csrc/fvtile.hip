@@ -21,7 +21,8 @@
  *
  * The rest of the mesh (tiles at walls or at level changes, hanging faces, anything
  * that is not a tile) stays on the face-list kernels, which skip the blocks whose faces
- * and cells are all the interior tiles'.
+ * and cells are all the interior tiles' (and, in a block of cells covered in part, those
+ * cells).
  */
 #include <type_traits>
 
@@ -524,35 +525,41 @@ void fv_tiles_release(fvg_ctx *c) {
   if (c->tiles) (void)hipFree(c->tiles);
   if (c->face_skip) (void)hipFree(c->face_skip);
   if (c->gather_skip) (void)hipFree(c->gather_skip);
+  if (c->tile_cell) (void)hipFree(c->tile_cell);
   c->tiles = nullptr;
-  c->face_skip = c->gather_skip = nullptr;
+  c->face_skip = c->gather_skip = c->tile_cell = nullptr;
   c->ntile = 0;
   c->face_all_skip = c->gather_all_skip = false;
 }
 
 /* ------------------------------------------------------------------ host: tiles */
 
-/* The interior tiles of a mesh and the skip flags of the stage's face kernel and gather
- * (host only). */
+/* The interior tiles of a mesh and the skip flags of the stage's face kernel and cell
+ * blocks (host only); cellmask (one byte per cell, 1 in an interior tile) is filled when a
+ * cell block is covered in part. */
 static void find_tiles(const fvg_mesh *m, std::vector<int32_t> &rows, std::vector<uint8_t> &fskip,
-                       std::vector<uint8_t> &gskip, bool &fall, bool &gall) {
+                       std::vector<uint8_t> &gskip, std::vector<uint8_t> &cellmask, bool &fall,
+                       bool &gall) {
   rows.clear();
   fskip.clear();
   gskip.clear();
+  cellmask.clear();
   fall = gall = false;
   const int N = m->ncell;
   if (N < TT) return;
   auto level = [&](int i) { return (int)((m->cflag[i] >> 16) & 0xff); };
   auto cls = [&](int i, int f) { return (int)((m->cflag[i] >> (2 * f)) & 3); };
   auto nbr = [&](int f, int i) { return m->nbr[(size_t)f * N + i]; };
-  /* candidate tiles: 512 consecutive cells from a multiple of 512 whose +x / +y / +z
-   * neighbours inside the block are the Morton ones */
+  /* candidate tiles: 512 consecutive cells of one level whose +x / +y / +z neighbours
+   * inside the block are the Morton ones.  Refinement elsewhere shifts the blocks of a
+   * region by any count of leaves, so every start is tried; a start fails within its
+   * first few cells unless it begins a block. */
   std::vector<int32_t> tile_of((size_t)N, -1);
   std::vector<int32_t> base;
   static const int fplus[3] = {1, 2, 5};
-  for (int b = 0; b + TT <= N; b += TT) {
+  for (int b = 0; b + TT <= N;) {
     const int L = level(b);
-    bool ok = true;
+    bool ok = level(b + TT - 1) == L;
     for (int t = 0; t < TT && ok; ++t) {
       const int i = b + t;
       if (level(i) != L) { ok = false; break; }
@@ -566,9 +573,10 @@ static void find_tiles(const fvg_mesh *m, std::vector<int32_t> &rows, std::vecto
         ok = cls(i, f) == CLS_SAME && nbr(f, i) == b + tmorton(q[0], q[1], q[2]);
       }
     }
-    if (!ok) continue;
+    if (!ok) { ++b; continue; }
     for (int t = 0; t < TT; ++t) tile_of[(size_t)b + t] = (int32_t)base.size();
     base.push_back(b);
+    b += TT;
   }
   if (base.empty()) return;
   /* interior tiles: across each face the six cell rows continue into another tile, at
@@ -623,27 +631,36 @@ static void find_tiles(const fvg_mesh *m, std::vector<int32_t> &rows, std::vecto
     }
     start += n;
   }
+  bool partial = false;
   for (int i0 = 0; i0 < N; i0 += FV_BLOCK) {
-    bool all = true;
-    for (int i = i0; i < N && i < i0 + FV_BLOCK && all; ++i) all = interior[(size_t)i] != 0;
-    gskip.push_back(all ? 1 : 0);
-    gall = gall && all;
+    int in = 0, n = 0;
+    for (int i = i0; i < N && i < i0 + FV_BLOCK; ++i, ++n) in += interior[(size_t)i];
+    const int flag = in == n ? 1 : (in ? 2 : 0);
+    gskip.push_back((uint8_t)flag);
+    gall = gall && flag == 1;
+    partial = partial || flag == 2;
   }
   /* the gather's grid is padded to a multiple of 8 blocks; the padding has no cells */
   while (gskip.size() % 8) gskip.push_back(1);
+  if (partial) cellmask.swap(interior);
 }
 
 int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
   fv_tiles_release(c);
   std::vector<int32_t> rows;
-  std::vector<uint8_t> fskip, gskip;
+  std::vector<uint8_t> fskip, gskip, cellmask;
   bool fall, gall;
-  find_tiles(m, rows, fskip, gskip, fall, gall);
+  find_tiles(m, rows, fskip, gskip, cellmask, fall, gall);
   const int nt = (int)(rows.size() / 8);
   if (nt == 0) return FVG_OK;
   hipError_t e = hipMalloc((void **)&c->tiles, rows.size() * sizeof(int32_t));
   if (e == hipSuccess) e = hipMalloc((void **)&c->face_skip, fskip.size());
   if (e == hipSuccess) e = hipMalloc((void **)&c->gather_skip, gskip.size());
+  if (e == hipSuccess && !cellmask.empty()) {
+    e = hipMalloc((void **)&c->tile_cell, cellmask.size());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->tile_cell, cellmask.data(), cellmask.size(), hipMemcpyHostToDevice, c->stream);
+  }
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->tiles, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
@@ -664,9 +681,9 @@ int fv_tiles_build(fvg_ctx *c, const fvg_mesh *m) {
 extern "C" int fvh_tile_table(const fvg_mesh *m, int32_t *rows, int cap, int *ntile) {
   if (!m || !ntile || (cap > 0 && !rows)) return FVG_ERR_ARG;
   std::vector<int32_t> r;
-  std::vector<uint8_t> fskip, gskip;
+  std::vector<uint8_t> fskip, gskip, cellmask;
   bool fall, gall;
-  find_tiles(m, r, fskip, gskip, fall, gall);
+  find_tiles(m, r, fskip, gskip, cellmask, fall, gall);
   *ntile = (int)(r.size() / 8);
   const int n = *ntile < cap ? *ntile : cap;
   if (n > 0) memcpy(rows, r.data(), (size_t)n * 8 * sizeof(int32_t));

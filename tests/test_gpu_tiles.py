@@ -11,8 +11,10 @@ the states and the last stage's residual are bitwise those of the face-list path
 test_gpu_stage_parity, which also runs the tile meshes here against the oracle.
 
 Meshes: 32^3 periodic (every tile interior: the face kernel and the gather are not
-launched at all), 32^3 with outflow walls (the 8 inner tiles; blocks of both kinds),
-and a 32^3 mesh refined in a sphere (tiles beside level changes and hanging faces).
+launched at all), 32^3 with outflow walls (the 8 inner tiles; blocks of both kinds), a
+32^3 mesh refined in a sphere (tiles beside level changes and hanging faces), and the
+periodic box with its first 2^3 leaves refined (tiles off the 512 alignment: blocks of
+cells covered in part, which the gather and the gradient kernel mask cell by cell).
 """
 import numpy as np
 import pytest
@@ -31,6 +33,13 @@ def _mesh(kind, neq):
         return uniform(5, neq)
     if kind == "amr32":
         return amr(5, neq, passes=1, centre=(0.3, 0.3, 0.3), radius=0.12)
+    if kind == "shifted32":
+        # the first 2^3 leaves refined: every later tile starts 56 leaves off a multiple
+        # of 512, so cell blocks are covered in part (per-cell masks)
+        F = uniform(5, neq, periodic=(1, 1, 1), bc=(0,) * 6)
+        xc, _ = F.leaf_info()
+        F.adapt(((xc < 1.0 / 16).all(axis=1)).astype(np.int8), 7)
+        return F
     raise ValueError(kind)
 
 
@@ -52,7 +61,7 @@ CONFIGS = {
     "m3_euler_mixed_rk3": (3, 0, "minmod", "mixed", 3, {}),
 }
 
-MESHES = ["periodic32", "walls32", "amr32"]
+MESHES = ["periodic32", "walls32", "amr32", "shifted32"]
 
 
 def _setup(mesh, name):
@@ -104,6 +113,8 @@ def test_tiled_stage_bitwise_equals_face_list(mesh, name):
         assert nt_on == 64
     elif mesh == "walls32":
         assert nt_on == 8
+    elif mesh == "shifted32":
+        assert nt_on == 57
     assert bad_on == -1 and bad_off == -1
     assert np.all(np.isfinite(S_on))
     assert np.abs(S_on - S).max() > 0

@@ -1,15 +1,16 @@
 """The host side of the LDS-tiled stage: which blocks of a mesh are interior tiles.
 
 fvh_tile_table runs the search fvg_set_mesh runs (no GPU): 8 x 8 x 8 blocks of 512
-consecutive leaves of one level, from a multiple of 512, laid out in Morton order (local
-index = bits x0 y0 z0 x1 y1 z1 x2 y2 z2), whose six face-neighbour blocks are such tiles
-too.  Each row is checked here against the flattened mesh's own neighbour table,
+consecutive leaves of one level, from any position in the leaf list, laid out in Morton
+order (local index = bits x0 y0 z0 x1 y1 z1 x2 y2 z2), whose six face-neighbour blocks are
+such tiles too.  Each row is checked here against the flattened mesh's own neighbour table,
 independently of the search: the +x / +y / +z neighbour of every leaf inside the block is
 the Morton one, and across each face the boundary layer continues into the named
 neighbour tile at the Morton positions of its opposite layer.  The counts are the
 geometric ones: a periodic 32^3 box is 4^3 tiles, all interior; with outflow walls only
 the 2^3 inner ones are interior; refining the last 2^3 leaves removes one tile and the
-interior status of its six neighbours.
+interior status of its six neighbours, and refining the first 2^3 leaves does the same
+while shifting every later block by 56 leaves off the multiples of 512.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,7 @@ def check_rows(flat, rows):
     assert np.array_equal(morton(XYZ), T)
     for row in rows:
         b, nb, lev = int(row[0]), row[1:7], int(row[7])
-        assert b % 512 == 0 and b + 512 <= N
+        assert 0 <= b and b + 512 <= N
         cells = b + T
         assert np.all(level[cells] == lev)
         for f in range(6):
@@ -80,15 +81,20 @@ def test_tile_table(kind, expected):
     flat.close()
 
 
-def test_tile_table_after_refinement():
+@pytest.mark.parametrize("corner", ["last", "first"])
+def test_tile_table_after_refinement(corner):
     F = uniform(5, 5, periodic=(1, 1, 1), bc=(0,) * 6)
     xc, _ = F.leaf_info()
-    F.adapt(((xc > 1.0 - 1.0 / 16).all(axis=1)).astype(np.int8), 7)
+    sel = (xc > 1.0 - 1.0 / 16) if corner == "last" else (xc < 1.0 / 16)
+    F.adapt(sel.all(axis=1).astype(np.int8), 7)
     flat = fv.FlatMesh(F.run)
     rows = flat.tile_table()
     assert F.nleaves == 32768 + 56 and len(rows) == 57
     check_rows(flat, rows)
-    # the refined block was the last one: no tile starts there, nor at its six neighbours
-    last = 32768 - 512
-    assert last not in set(rows[:, 0].tolist())
+    starts = set(rows[:, 0].tolist())
+    if corner == "last":
+        assert 32768 - 512 not in starts  # the refined block
+        assert all(b % 512 == 0 for b in starts)
+    else:
+        assert 0 not in starts and all(b % 512 == 56 for b in starts)
     flat.close()
