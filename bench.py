@@ -55,6 +55,8 @@ def _args():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--tiling", default="auto", choices=["auto", "on", "off"],
+                    help="LDS-tiled stage for interior 8^3 tiles (fvg_set_tiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 6 streaming copies of MIB MiB (calibrates rocprof byte counters)")
     return ap.parse_args()
@@ -178,10 +180,14 @@ def mesh_facts(flat, F):
                 grid_grad=rnd(flat.ncell))
 
 
+TILING = "auto"  # --tiling
+
+
 def _solver(flat, S0, p):
     import fv_140823_amd as fv
     g = fv.Solver(p)
     g.set_mesh(flat)
+    g.set_tiling({"auto": "auto", "on": True, "off": False}[TILING])
     g.set_state(S0)
     return g
 
@@ -313,7 +319,7 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
     S0 = F.get_state()
     g = _solver(flat, S0, p)
     dt = g.cfl_dt(meta["cfl"])
-    mf["tiles"] = g.tile_count
+    mf["tiles"] = g.tile_count if TILING != "off" else 0
     mf["grid_tile"] = 512 * g.tile_count
     g.close()
     log(f"{mf['tiles']} interior 8^3 tiles ({512 * mf['tiles']} leaves) on the LDS-tiled stage", quiet)
@@ -342,7 +348,8 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
     t_stage = sum(k_p[KERNELS.index(k)] for k in stage_kernels)
     B = ref_bytes_per_cell("stage", neq, fpc, 8, p.ns)
     achieved = B * mf["N"] / (t_stage * 1e-3) / 1e9
-    tr = [measured_traffic(meta["name"], *names[k]) for k in stage_kernels]
+    wkey = meta["name"] + ("" if TILING == "auto" else f" tiling={TILING}")
+    tr = [measured_traffic(wkey, *names[k]) for k in stage_kernels]
     traffic = sum(tr) if all(t is not None for t in tr) else None
     comp = sum(compulsory_bytes(k, mf, neq, 8, p.ns, False, heat) for k in stage_kernels)
     roof = dict(bound="hbm", kernel="RK stage: " + " + ".join(names[k][0] for k in stage_kernels),
@@ -364,7 +371,7 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
         if (k_p[i] <= 0 and k_m[i] <= 0) or k not in names:
             continue
         e = dict(name=names[k][0], plain_ms=round(k_p[i], 5), mixed_ms=round(k_m[i], 5))
-        tb = measured_traffic(meta["name"], *names[k])
+        tb = measured_traffic(wkey, *names[k])
         if tb is not None:
             e["plain_traffic_bytes"] = tb
             e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
@@ -376,6 +383,7 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
         kern[k] = e
     cfg = {"workload": meta["name"], "leaves": mf["N"], "faces": mf["F"], "hanging_quads": mf["Q"],
            "regular_faces": mf["Freg"], "levels": mf["levels"], "interior_tiles": mf["tiles"],
+           "tiling": TILING, "tiled_stage_ran": bool(k_p[KERNELS.index("tile_stage")] > 0),
            "model": {1: f"ideal gas gamma {p.gamma:g}", 2: "barotropic (model 2)",
                      3: "two-phase barotropic (model 3)"}[p.model] + (" Navier-Stokes" if p.ns else ""),
            "scheme": ("MUSCL-" + ("minmod" if p.limiter == 0 else "van Albada") if p.ischeme == 2
@@ -471,6 +479,8 @@ def adjoint_block(quiet):
 
 def main():
     a = _args()
+    global TILING
+    TILING = a.tiling
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the path has no "
               "multi-GPU halo exchange; the sharding is described in DESIGN.md", file=sys.stderr)

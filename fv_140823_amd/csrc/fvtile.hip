@@ -487,12 +487,14 @@ bool fv_tiles_usable(const fvg_ctx *c) {
       c->prm.wave_speeds != FVG_WAVES_DAVIS)
     return false;
   if (c->tiling == FVG_TILING_ON) return true;
-  /* Auto, from measurements on 128^3 meshes (stage times, tiled / face-list): Taylor-Green
+  /* Auto, from measurements on 128^3 meshes (step times, tiled / face-list): Taylor-Green
    * (Navier-Stokes, every leaf in a tile) 0.65 plain / 0.72 mixed, Sod (2/3 of the leaves
-   * in tiles) 0.91 / 0.99, the barotropic cavitation 1.07 / 1.15; the 64^3 Sod mesh (216
+   * in tiles) 0.91 / 1.01, the barotropic cavitation 1.07 / 1.15; the 64^3 Sod mesh (216
    * tiles, 42% of the leaves) 1.35 / 1.56: a tile is one workgroup's serial work, and the
-   * face-list kernels still sweep the blocks around the tiles. */
-  return c->prm.model == 1 && c->ntile >= 256 && (int64_t)c->ntile * TT * 2 >= c->N;
+   * face-list kernels still sweep the blocks around the tiles.  So: the ideal gas, plain
+   * or viscous, with at least 256 interior tiles holding half of the leaves. */
+  return c->prm.model == 1 && (c->prm.ns || c->rsize == 8) && c->ntile >= 256 &&
+         (int64_t)c->ntile * TT * 2 >= c->N;
 }
 
 void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double *Sout, float *SfOut,

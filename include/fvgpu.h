@@ -204,9 +204,10 @@ int fvg_ncell(const fvg_ctx *ctx);
  * with results bitwise equal to the face-list path's.  The face flux buffer
  * (FVG_FIELD_FACEFLUX) then holds only the faces outside them.  mode: FVG_TILING_OFF,
  * FVG_TILING_ON (whenever the mesh has interior tiles), FVG_TILING_AUTO (the default:
- * the ideal gas with at least 256 interior tiles holding half of the leaves or more,
- * where the tiled stage measured faster; the stiff models and meshes with few tiles keep
- * the face-list kernels).  fvg_tile_count = the interior tiles of the current mesh. */
+ * the ideal gas, plain FP64 or viscous, with at least 256 interior tiles holding half of
+ * the leaves or more, where the tiled stage measured faster; the stiff models, the mixed
+ * Euler form and meshes with few tiles keep the face-list kernels).  fvg_tile_count = the
+ * interior tiles of the current mesh. */
 enum { FVG_TILING_OFF = 0, FVG_TILING_ON = 1, FVG_TILING_AUTO = 2 };
 int fvg_set_tiling(fvg_ctx *ctx, int mode);
 /* The interior tiles fvg_set_mesh would find in m, host only: *ntile = their count, and
