@@ -5,14 +5,16 @@
  * list (Morton order: the local index interleaves the bits of x, y, z with x lowest, as
  * the depth-first order over kids by corner node gives, split.c:124-128); an interior
  * tile is one whose six face-neighbour blocks are tiles too.  One workgroup of 512
- * threads (one per cell) does a whole stage for an interior tile:
- *   1. loads the tile and the two cell layers across each of its faces (1,280 cells,
- *      the MUSCL stencils of all its faces) from HBM into LDS, once;
+ * threads (one per cell) does a whole stage for an interior tile (k_tile_stage):
+ *   1. loads the tile from HBM into LDS once and, per axis, the two cell layers on either
+ *      side of it along that axis (the MUSCL stencils of the axis's faces);
  *   2. per axis, evaluates each face of the tile once from LDS: the regular-face MUSCL
  *      of reconctruct_V3.c, the EOS, HLLC and (Navier-Stokes) the viscous flux with the
- *      gradients k_grad wrote; the face's flux goes to an LDS buffer;
+ *      gradients k_tile_grad / k_grad wrote; the face's flux goes to an LDS buffer;
  *   3. sums each cell's six face fluxes in the gather's order (-x, +x, -y, +y, -z, +z;
  *      residual_faces_AMR.c:173-175) and applies the RK stage (RK.c:24-47).
+ * k_tile_grad forms the Green-Gauss gradients of the tiles' cells from the tile and the
+ * cell layer beyond each face, staged in LDS, with k_grad's arithmetic.
  * Face fluxes never go to HBM and every stencil row is read once per tile, where the
  * face-list path writes the face flux, reads it back from both cells and fetches each
  * stencil row for every face that uses it.  A face is evaluated exactly as the face-list
