@@ -148,6 +148,7 @@ CASES = {
     "m1_ns_heat_tiles_periodic32_rk3": (1, 2, 1, "minmod", "periodic32", 3, {"prandtl": 0.71}),
     "m1_vanalbada_tiles_walls32_rk3": (1, 2, 0, "vanalbada", "walls32", 3, {}),
     "m3_ns_tiles_walls32_rk2": (3, 2, 1, "minmod", "walls32", 2, {}),
+    "m2_c3_tiles_walls32_rk3": (2, 2, 0, "minmod", "walls32", 3, {"c3": True}),
 }
 
 
