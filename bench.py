@@ -94,12 +94,16 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
     read once, each output written once (MUSCL inside the face kernel).  rs is the
     byte width of the face flux and the gradients (8 plain, 4 mixed).  For the
     LDS-tiled stage (m["tiles"] interior 8^3 tiles) each tile reads its cells and two
-    cell layers beyond each face (1,280 cells) once, the gradients of its cells and of
-    the first of those layers (896 cells), SN, and writes the new state (and, mixed,
-    its FP32 copy, the region then being read from that copy)."""
+    cell layers beyond each face (1,280 cells) once, the viscous rows of its cells and,
+    for the first of those layers (384 cells), the rows of that layer's axis, SN, and
+    writes the new state (and, mixed, its FP32 copy, the region then being read from
+    that copy).  Viscous rows per cell: the six symmetric combinations of the momentum
+    gradients (three per axis), plus the three components of grad(p / rho) with heat
+    conduction (one per axis)."""
     N, F, Q, Fr = m["N"], m["F"], m["Q"], m["Freg"]
     meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
-    grows = 12 if heat else 9                  # gradient rows: momenta (+ p / rho)
+    grows = 9 if heat else 6                   # viscous rows per cell
+    arows = 4 if heat else 3                   # rows an axis's faces read
     if kernel == "grad":                       # S rows -> momentum (and p / rho) gradients
         return 8 * (neq if heat else 3) * N + meta_c + grows * rs * N
     if kernel == "tile_grad":                  # tile + one layer beyond each face -> gradients
@@ -107,7 +111,7 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
     if kernel == "tile_stage":
         nt = m.get("tiles", 0)
         region = 1280 * neq * (4 + 8 * 512 / 1280 if rs == 4 else 8)
-        per = region + 896 * grows * rs * (1 if ns_visc else 0) + 8 * neq * 512 * 2 + 32
+        per = region + (512 * grows + 384 * arows) * rs * (1 if ns_visc else 0) + 8 * neq * 512 * 2 + 32
         if rs == 4:
             per += 4 * neq * 512
         return nt * per
@@ -115,7 +119,7 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
         # S, mesh, face records (L, R, meta), far stencil cells of regular faces, flux
         b = 8 * neq * N + meta_c + 12 * F + 8 * Fr + neq * rs * F
         if ns_visc:
-            b += 9 * rs * N
+            b += grows * rs * N
         return b
     if kernel == "gather_rk":
         b = 4 * N + 24 * N + neq * rs * F + 8 * neq * N * (1 if first_stage else 2) + 8 * neq * N

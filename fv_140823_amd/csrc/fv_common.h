@@ -559,8 +559,10 @@ int fv_upload_tables(fvg_ctx *c, int nl, const double *h, const double *dist, co
 /* forward pieces shared with fvadj.hip (defined in fvgpu.hip) */
 void fv_launch_recon(fvg_ctx *c, const double *S);
 void fv_stage(fvg_ctx *c, int irk, double dt, bool first, bool write_rhs);
-/* momentum gradients (gradcalc, rows 1..3) of S into c->SGm (precision of the context) */
-void fv_launch_grad(fvg_ctx *c, const double *S);
+/* the nine momentum gradients (gradcalc, rows 1..3; SGm[3 i + j] = d(rho u_i)/dx_j) of S
+ * into c->SGm (precision of the context), for the adjoint and tangent-linear kernels (the
+ * time loop stores the viscous rows of vrows there instead) */
+void fv_launch_grad_raw(fvg_ctx *c, const double *S);
 /* the same gradients in FP64 of a tangent state Sd (the inlet ghost, a constant, has
  * tangent zero) into SGmd [9][ns] */
 void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd);

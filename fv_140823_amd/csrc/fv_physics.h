@@ -68,9 +68,46 @@ template <> struct ModelTraits<3> { static constexpr int NEQ = 5; };
 template <typename T> struct FState {
   T r, u[3], e, p, c, y;
   T q[5]; /* conserved vector (SOL::vec) */
-  T g[9]; /* gradients of rho u, rho v, rho w (SG rows 1..3) */
+  T tv[3]; /* the cell's viscous rows for the face's axis a: tv[i] = vrow(a, i) (vrows) */
   T gt[3]; /* gradient of p / rho (heat conduction option) */
 };
+
+/* The viscous flux across an axis-a face uses row a of the stress tensor (flux_viscous.c:
+ * 58-73): tau[a][i] = mu (gr[3a+i] + gr[3i+a]) for i != a and mu (2 gr[4a] - 2/3 div gr) for
+ * i = a, gr the face mean of the two cells' momentum gradients (gr[3i+j] = d(rho u_i)/dx_j).
+ * Both are linear in the gradients, so each cell keeps the six symmetric rows
+ *   D_d = 2 g[4d] - (2/3)((g[0] + g[4]) + g[8])   (rows 0, 1, 2)
+ *   X_01 = g[1] + g[3], X_02 = g[2] + g[6], X_12 = g[5] + g[7]   (rows 3, 4, 5)
+ * and a face forms tau[a][i] = mu ((row of L + row of R) / 2): three values per side
+ * instead of seven.  Rows 6..8 hold the gradient of p / rho (heat conduction option).
+ * Written without contraction, so that every kernel that forms them rounds the same. */
+enum { VROW_HEAT = 6, VROWS = 9 };
+
+FV_HD constexpr int vrow(int a, int i) {
+  return a == i ? a : (a + i == 1 ? 3 : (a + i == 2 ? 4 : 5));
+}
+
+template <typename T>
+FV_HD __forceinline__ void vrows(const T *g, T *out) {
+#pragma clang fp contract(off)
+  const T div = (g[0] + g[4]) + g[8];
+  const T c23 = (T)2.0 / (T)3.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[d] = (T)2.0 * g[4 * d] - c23 * div;
+  out[3] = g[1] + g[3];
+  out[4] = g[2] + g[6];
+  out[5] = g[5] + g[7];
+}
+
+/* the viscous rows of axis a of a cell from its nine momentum gradients (callers that
+ * hold those, as the tangent-linear model and the host checks) */
+template <int a, typename T>
+FV_HD __forceinline__ void vrows_axis(const T *g, T *tv) {
+  T r[6];
+  vrows<T>(g, r);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) tv[i] = r[vrow(a, i)];
+}
 
 /* EOS of a conserved vector, always in FP64 (returns false on a failed root) */
 template <int MODEL>
@@ -225,30 +262,21 @@ FV_HD __forceinline__ bool hllc_face(const DevParams &P, const FState<T> &L,
   if constexpr (MODEL == 1) F[4] = Ku[a] * (Ke + Kp);
   if constexpr (MODEL == 3) F[4] = Ku[a] * Kr * Ky;
   if constexpr (NS) {
-    /* flux_viscous.c:58-73,199-232: face means of velocity and of the gradients
-     * of the conserved momenta (SG rows 1..3) */
-    T uf[3], gr[9];
+    /* flux_viscous.c:58-73,199-232: face means of velocity and of row a of the stress
+     * tensor, from each side's viscous rows of axis a (vrows) */
+    T uf[3], ta[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) uf[i] = (T)0.5 * (L.u[i] + R.u[i]);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) gr[k] = (T)0.5 * (L.g[k] + R.g[k]);
     const T mu = (T)P.mu + zero;
-    /* gr[3*i + j] = d(rho u_i)/dx_j */
-    T div = gr[0] + gr[4] + gr[8];
-    T tau[3][3];
-    tau[0][0] = mu * ((T)2.0 * gr[0] - ((T)2.0 / (T)3.0) * div);
-    tau[1][1] = mu * ((T)2.0 * gr[4] - ((T)2.0 / (T)3.0) * div);
-    tau[2][2] = mu * ((T)2.0 * gr[8] - ((T)2.0 / (T)3.0) * div);
-    tau[0][1] = tau[1][0] = mu * (gr[1] + gr[3]);
-    tau[0][2] = tau[2][0] = mu * (gr[2] + gr[6]);
-    tau[1][2] = tau[2][1] = mu * (gr[5] + gr[7]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) F[1 + i] = F[1 + i] - tau[a][i];
+    for (int i = 0; i < 3; ++i) ta[i] = mu * ((T)0.5 * (L.tv[i] + R.tv[i]));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) F[1 + i] = F[1 + i] - ta[i];
     /* flux_viscous.c:227-232 writes row 4 whenever con->eos == 1, whatever the
      * model: for model 3 that row is the rho Y flux, and the reference subtracts
      * the viscous work from it too */
     if constexpr (MODEL == 1 || MODEL == 3) {
-      if (P.visc_work) F[4] = F[4] - (uf[0] * tau[a][0] + uf[1] * tau[a][1] + uf[2] * tau[a][2]);
+      if (P.visc_work) F[4] = F[4] - (uf[0] * ta[0] + uf[1] * ta[1] + uf[2] * ta[2]);
     }
     /* heat conduction (option): the energy flux loses q_a = -kheat d(p/rho)/dx_a, with the
      * face mean of the two cells' gradients of p/rho */

@@ -610,11 +610,14 @@ __device__ __forceinline__ void tlm_face_body(const DevMesh &M, const DevParams 
   make_state_ideal<D>(P, UL, L);
   make_state_ideal<D>(P, UR, R);
   if constexpr (NS) {
+    D gL[9], gR[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      L.g[k] = D(SGm[(size_t)k * M.ns + iL], SGmd[(size_t)k * M.ns + iL]);
-      R.g[k] = D(SGm[(size_t)k * M.ns + iR], SGmd[(size_t)k * M.ns + iR]);
+      gL[k] = D(SGm[(size_t)k * M.ns + iL], SGmd[(size_t)k * M.ns + iL]);
+      gR[k] = D(SGm[(size_t)k * M.ns + iR], SGmd[(size_t)k * M.ns + iR]);
     }
+    vrows_axis<a, D>(gL, L.tv);
+    vrows_axis<a, D>(gR, R.tv);
   }
   D out[5];
   hllc_face<1, NS, a, D>(P, L, R, D((double)c_face_sign[fl]), out);
@@ -710,7 +713,7 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
    * contiguous range (the kernels' bounds checks drop the padding) */
   const int nc = (int)round_up(nblk(c->N), 8);
   const bool ns = c->prm.ns != 0;
-  if (ns) fv_launch_grad(c, S); /* the momentum gradients of the stage's input, into c->SGm */
+  if (ns) fv_launch_grad_raw(c, S); /* the momentum gradients of the stage's input, into c->SGm */
   dim3 gf((int)round_up(b0 + b1 + b2, 8)), b(FV_BLOCK);
   if (c->rsize == 8) {
     const double *SGm = (const double *)c->SGm;
@@ -960,7 +963,7 @@ int fvg_tlm(fvg_ctx *c, const double *dir, int nsteps, double dt, double *J, dou
                            a->SFd);
       }
       if (c->prm.ns) {
-        fv_launch_grad(c, c->S);
+        fv_launch_grad_raw(c, c->S);
         fv_launch_grad_tangent(c, a->Sd, a->SGmd);
         hipLaunchKernelGGL(k_tlm_face<true>, dim3(b0 + b1 + b2), dim3(FV_BLOCK), 0, c->stream, M,
                            c->dp, b0, b1, n0, n1, n2, c->S, a->Sd, (const double *)c->SF, a->SFd,
@@ -1069,11 +1072,20 @@ int fvg_face_flux_ns_host(const fvg_params *p, int axis, int sign, const double 
   FState<double> L, R;
   make_state_ideal<double>(P, UL, L);
   make_state_ideal<double>(P, UR, R);
-  for (int k = 0; k < 9; ++k) { L.g[k] = gL[k]; R.g[k] = gR[k]; }
   const double s = sign < 0 ? -1.0 : 1.0;
-  if (axis == 0) hllc_face<1, true, 0, double>(P, L, R, s, out);
-  else if (axis == 1) hllc_face<1, true, 1, double>(P, L, R, s, out);
-  else hllc_face<1, true, 2, double>(P, L, R, s, out);
+  if (axis == 0) {
+    vrows_axis<0, double>(gL, L.tv);
+    vrows_axis<0, double>(gR, R.tv);
+    hllc_face<1, true, 0, double>(P, L, R, s, out);
+  } else if (axis == 1) {
+    vrows_axis<1, double>(gL, L.tv);
+    vrows_axis<1, double>(gR, R.tv);
+    hllc_face<1, true, 1, double>(P, L, R, s, out);
+  } else {
+    vrows_axis<2, double>(gL, L.tv);
+    vrows_axis<2, double>(gR, R.tv);
+    hllc_face<1, true, 2, double>(P, L, R, s, out);
+  }
   return FVG_OK;
 }
 

@@ -5,7 +5,7 @@
  *   S, SN, RHS          f64 [neq][ns]          conserved state, RK base, residual
  *   SF                  R   [6][neq][ns]       MUSCL face states (R = f64 plain, f32 mixed)
  *   SG                  f64 [neq+1][3][ns]     full Green-Gauss gradients (gradcalc)
- *   SGm                 R   [9][ns]            momentum gradients for the viscous flux
+ *   SGm                 R   [9][ns]            viscous rows (vrows) and grad(p / rho)
  *   G                   R   [neq][nfp]         owner-side face flux, -Area*(F.n + HLLC)
  *   cflag/nbr/gslot     per cell               classes, neighbours, gather slots
  *   fL/fR/fmeta         per face, grouped by axis
@@ -73,8 +73,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
 }
 
 /* Green-Gauss gradients (gradcalc.c:23-42).  FULL: all neq+1 rows (incl. p) to
- * SG f64 [neq+1][3][ns]; otherwise the momentum rows 1..3 to SGm [9][ns]. */
-template <int MODEL, bool FULL, typename R, bool HEAT = false>
+ * SG f64 [neq+1][3][ns]; otherwise the gradients of the momenta (rows 1..3) as the
+ * six viscous rows of vrows to SGm [9][ns] (RAW: the nine gradients themselves,
+ * SGm[3 i + j] = d(rho u_i)/dx_j, for the adjoint and tangent-linear kernels). */
+template <int MODEL, bool FULL, typename R, bool HEAT = false, bool RAW = false>
 __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
                                                    const double *__restrict__ S,
                                                    double *__restrict__ SG, R *__restrict__ SGm,
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   /* HEAT (heat conduction option, model 1): also the Green-Gauss gradient of p / rho,
    * from the same face means, into SGm rows 9..11 */
-  static_assert(!HEAT || (MODEL == 1 && !FULL), "heat conduction: model 1, momentum rows");
+  static_assert(!HEAT || (MODEL == 1 && !FULL && !RAW), "heat conduction: model 1, viscous rows");
   const int lb = logical_block();
   /* a block of cells that the LDS-tiled gradient kernel covers exits whole; in a block it
    * covers in part, its cells do */
@@ -124,6 +126,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
 #pragma unroll
     for (int f = 0; f < 6; ++f) thr[f] = theta_ideal(P, Ur[f]);
   }
+  double gm[9]; /* gm[3 i + d] = d(rho u_i)/dx_d */
   /* per axis d, faces of that axis in increasing face id */
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -176,13 +179,24 @@ __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
         }
       }
     }
-    if constexpr (HEAT) SGm[((size_t)9 + d) * M.ns + i] = (R)acct;
+    if constexpr (HEAT) SGm[((size_t)VROW_HEAT + d) * M.ns + i] = (R)acct;
     if constexpr (FULL) {
 #pragma unroll
       for (int v = 0; v <= NEQ; ++v) SG[((size_t)v * 3 + d) * M.ns + i] = acc[v];
     } else {
 #pragma unroll
-      for (int v = 1; v < 4; ++v) SGm[((size_t)(v - 1) * 3 + d) * M.ns + i] = (R)acc[v];
+      for (int v = 1; v < 4; ++v) gm[(v - 1) * 3 + d] = acc[v];
+    }
+  }
+  if constexpr (!FULL) {
+    if constexpr (RAW) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) SGm[(size_t)k * M.ns + i] = (R)gm[k];
+    } else {
+      double vr[6];
+      vrows<double>(gm, vr);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) SGm[(size_t)k * M.ns + i] = (R)vr[k];
     }
   }
 }
@@ -203,15 +217,26 @@ __device__ __forceinline__ void store_rows(R *__restrict__ G, const R *gbuf, int
   for (int k = threadIdx.x; k < nrow * NEQ; k += FV_BLOCK) g[k] = gbuf[k];
 }
 
-/* the gradients of p / rho of both cells (SGm rows 9..11; the kernels with the
+/* the gradients of p / rho of both cells (SGm rows 6..8; the kernels with the
  * heat-conduction bit read them, and use them where P.kheat != 0) */
 template <typename T, typename R>
 __device__ __forceinline__ void load_heat_grad(const DevMesh &M, const R *__restrict__ SGm,
                                                int iL, int iR, FState<T> &L, FState<T> &Rs) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    L.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iL];
-    Rs.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iR];
+    L.gt[d] = (T)SGm[(size_t)(VROW_HEAT + d) * M.ns + iL];
+    Rs.gt[d] = (T)SGm[(size_t)(VROW_HEAT + d) * M.ns + iR];
+  }
+}
+
+/* both cells' viscous rows of axis a (SGm rows vrow(a, 0..2)) */
+template <typename T, typename R>
+__device__ __forceinline__ void load_visc_rows(const DevMesh &M, const R *__restrict__ SGm, int a,
+                                               int iL, int iR, FState<T> &L, FState<T> &Rs) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    L.tv[k] = (T)SGm[(size_t)vrow(a, k) * M.ns + iL];
+    Rs.tv[k] = (T)SGm[(size_t)vrow(a, k) * M.ns + iR];
   }
 }
 
@@ -249,11 +274,7 @@ __device__ __forceinline__ void face_body(const DevMesh &M, const DevParams &P, 
   bool ok = make_state<MODEL, T>(P, UL, clip, L);
   ok = make_state<MODEL, T>(P, UR, clip && bc != 1, Rs) && ok;
   if constexpr (NS) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
-      Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
-    }
+    load_visc_rows(M, SGm, a, iL, iR, L, Rs);
     if constexpr ((OPT & OPT_HEAT) != 0) load_heat_grad<T, R>(M, SGm, iL, iR, L, Rs);
   }
   T out[NEQ];
@@ -306,11 +327,7 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
   bool ok = make_state<MODEL, T>(P, UL, true, L);
   ok = make_state<MODEL, T>(P, UR, bc != 1, Rs) && ok;
   if constexpr (NS) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      L.g[k] = (T)SGm[(size_t)k * M.ns + iL];
-      Rs.g[k] = (T)SGm[(size_t)k * M.ns + iR];
-    }
+    load_visc_rows(M, SGm, a, iL, iR, L, Rs);
     if constexpr ((OPT & OPT_HEAT) != 0) load_heat_grad<T, R>(M, SGm, iL, iR, L, Rs);
   }
   T out[NEQ];
@@ -417,11 +434,7 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
   }
   if (!fp32) return false;
   if constexpr (NS) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      L.g[k] = SGm[(size_t)k * M.ns + iL];
-      Rs.g[k] = SGm[(size_t)k * M.ns + iR];
-    }
+    load_visc_rows(M, SGm, a, iL, iR, L, Rs);
     if constexpr (HEAT) load_heat_grad<float, float>(M, SGm, iL, iR, L, Rs);
   }
   float out[5];
@@ -504,10 +517,7 @@ __device__ void face_rebuild_block(const DevMesh &M, const DevParams &P,
       bool ok = make_state<1, float>(P, sX[t][0], true, L);
       ok = make_state<1, float>(P, sX[t][1], bc != 1, Rs) && ok;
       if constexpr (NS) {
-        for (int k = 0; k < 9; ++k) {
-          L.g[k] = SGm[(size_t)k * M.ns + iL];
-          Rs.g[k] = SGm[(size_t)k * M.ns + iR];
-        }
+        load_visc_rows(M, SGm, a, iL, iR, L, Rs);
         if constexpr (HEAT) load_heat_grad<float, float>(M, SGm, iL, iR, L, Rs);
       }
       const float s = (float)c_face_sign[fl];
@@ -1113,7 +1123,7 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, fv_dmalloc(c, &c->SG, (size_t)(neq + 1) * 3 * ns * 8, slack));
   HIPCHK(c, fv_dmalloc(c, &c->SF, (size_t)6 * neq * ns * rs, slack));
   /* 9 momentum gradients, then 3 of p / rho (heat conduction option) */
-  HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)12 * ns * rs, slack));
+  HIPCHK(c, fv_dmalloc(c, &c->SGm, (size_t)VROWS * ns * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->G, (size_t)neq * c->nfp * rs, slack));
   HIPCHK(c, fv_dmalloc(c, &c->Sx, (size_t)neq * ns * 8, slack));
   if (rs == 4 && c->prm.model == 1) {
@@ -1129,7 +1139,7 @@ int fv_alloc_mesh(fvg_ctx *c, int N, int F, int Q, int nl, const int32_t nfa[3],
   HIPCHK(c, hipMemsetAsync(c->RHS, 0, (size_t)neq * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SG, 0, (size_t)(neq + 1) * 3 * ns * 8, st));
   HIPCHK(c, hipMemsetAsync(c->SF, 0, (size_t)6 * neq * ns * rs, st));
-  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)12 * ns * rs, st));
+  HIPCHK(c, hipMemsetAsync(c->SGm, 0, (size_t)VROWS * ns * rs, st));
   HIPCHK(c, hipMemsetAsync(c->G, 0, (size_t)neq * c->nfp * rs, st));
   fv_dcache_trim(c);
   return FVG_OK;
@@ -1480,14 +1490,21 @@ static void launch_recon(fvg_ctx *c, const double *S) {
 }
 
 template <int MODEL>
-static void launch_grad(fvg_ctx *c, const double *S, bool full) {
+static void launch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   const uint8_t *skip = full ? nullptr : c->stage_gskip, *cmask = full ? nullptr : c->stage_cmask;
   if (full)
     hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)nullptr, skip, cmask);
-  else if (MODEL == 1 && c->dp.kheat != 0.0) {
+  else if (raw) {
+    if (c->rsize == 8)
+      hipLaunchKernelGGL((k_grad<MODEL, false, double, false, true>), g, b, 0, c->stream, M, c->dp,
+                         S, c->SG, (double *)c->SGm, skip, cmask);
+    else
+      hipLaunchKernelGGL((k_grad<MODEL, false, float, false, true>), g, b, 0, c->stream, M, c->dp,
+                         S, c->SG, (float *)c->SGm, skip, cmask);
+  } else if (MODEL == 1 && c->dp.kheat != 0.0) {
     if constexpr (MODEL == 1) {
       if (c->rsize == 8)
         hipLaunchKernelGGL((k_grad<MODEL, false, double, true>), g, b, 0, c->stream, M, c->dp, S,
@@ -1633,11 +1650,11 @@ static void dispatch_recon(fvg_ctx *c, const double *S) {
     default: launch_recon<3>(c, S); break;
   }
 }
-static void dispatch_grad(fvg_ctx *c, const double *S, bool full) {
+static void dispatch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false) {
   switch (c->prm.model) {
-    case 1: launch_grad<1>(c, S, full); break;
-    case 2: launch_grad<2>(c, S, full); break;
-    default: launch_grad<3>(c, S, full); break;
+    case 1: launch_grad<1>(c, S, full, raw); break;
+    case 2: launch_grad<2>(c, S, full, raw); break;
+    default: launch_grad<3>(c, S, full, raw); break;
   }
 }
 static void dispatch_face_fused(fvg_ctx *c, const double *S) {
@@ -1885,14 +1902,14 @@ static void run_steps(fvg_ctx *c, int nsteps, int method, double dt, bool write_
 
 void fv_launch_recon(fvg_ctx *c, const double *S) { dispatch_recon(c, S); }
 
-void fv_launch_grad(fvg_ctx *c, const double *S) { dispatch_grad(c, S, false); }
+void fv_launch_grad_raw(fvg_ctx *c, const double *S) { dispatch_grad(c, S, false, true); }
 
 void fv_launch_grad_tangent(fvg_ctx *c, const double *Sd, double *SGmd) {
   DevMesh M = dev_mesh(c);
   DevParams Pz = c->dp;
   for (int v = 0; v < 5; ++v) Pz.inlet[v] = 0.0;
-  hipLaunchKernelGGL((k_grad<1, false, double>), dim3(nblocks(c->N)), dim3(FV_BLOCK), 0, c->stream,
-                     M, Pz, Sd, (double *)nullptr, SGmd, (const uint8_t *)nullptr,
+  hipLaunchKernelGGL((k_grad<1, false, double, false, true>), dim3(nblocks(c->N)), dim3(FV_BLOCK), 0,
+                     c->stream, M, Pz, Sd, (double *)nullptr, SGmd, (const uint8_t *)nullptr,
                      (const uint8_t *)nullptr);
 }
 

@@ -10,11 +10,13 @@
  *      side of it along that axis (the MUSCL stencils of the axis's faces);
  *   2. per axis, evaluates each face of the tile once from LDS: the regular-face MUSCL
  *      of reconctruct_V3.c, the EOS, HLLC and (Navier-Stokes) the viscous flux with the
- *      gradients k_tile_grad / k_grad wrote; the face's flux goes to an LDS buffer;
+ *      viscous rows k_tile_grad / k_grad wrote (three per cell for an axis's faces, four
+ *      with heat conduction); the face's flux goes to an LDS buffer;
  *   3. sums each cell's six face fluxes in the gather's order (-x, +x, -y, +y, -z, +z;
  *      residual_faces_AMR.c:173-175) and applies the RK stage (RK.c:24-47).
  * k_tile_grad forms the Green-Gauss gradients of the tiles' cells from the tile and the
- * cell layer beyond each face, staged in LDS, with k_grad's arithmetic.
+ * cell layer beyond each face, staged in LDS, with k_grad's arithmetic, and stores their
+ * viscous rows (vrows) as k_grad does.
  * Face fluxes never go to HBM and every stencil row is read once per tile, where the
  * face-list path writes the face flux, reads it back from both cells and fetches each
  * stencil row for every face that uses it.  A face is evaluated exactly as the face-list
@@ -162,16 +164,13 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
   }
   if constexpr (NS) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      Ls.g[q] = (T)SGm[(size_t)q * M.ns + iL];
-      Rs.g[q] = (T)SGm[(size_t)q * M.ns + iR];
+    for (int k = 0; k < 3; ++k) {
+      Ls.tv[k] = (T)SGm[(size_t)vrow(a, k) * M.ns + iL];
+      Rs.tv[k] = (T)SGm[(size_t)vrow(a, k) * M.ns + iR];
     }
     if constexpr ((OPT & OPT_HEAT) != 0) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        Ls.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iL];
-        Rs.gt[d] = (T)SGm[(size_t)(9 + d) * M.ns + iR];
-      }
+      Ls.gt[a] = (T)SGm[(size_t)(VROW_HEAT + a) * M.ns + iL];
+      Rs.gt[a] = (T)SGm[(size_t)(VROW_HEAT + a) * M.ns + iR];
     }
   }
   T o[NEQ];
@@ -342,10 +341,11 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   }
 }
 
-/* Green-Gauss gradients of the interior tiles' cells: k_grad's momentum rows (and, with
- * heat conduction, the gradient of p / rho) from the tile and the cell layer beyond each
- * of its faces, staged once in LDS (no neighbour-index loads, coalesced rows).  The
- * arithmetic is k_grad's, term for term, so the values are bitwise the same.  Halo slot
+/* Green-Gauss gradients of the interior tiles' cells: k_grad's viscous rows of the
+ * momentum gradients (and, with heat conduction, the gradient of p / rho) from the tile
+ * and the cell layer beyond each of its faces, staged once in LDS (no neighbour-index
+ * loads, coalesced rows).  The arithmetic is k_grad's, term for term, so the values are
+ * bitwise the same.  Halo slot
  * of face f: 512 + f x 64 + u x 8 + w, (u, w) the other two coordinates in axis order. */
 template <int MODEL, typename R, bool HEAT>
 __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
@@ -410,6 +410,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
   for (int v = 1; v < 4; ++v) U[v] = reg[(v - V0) * HR + t];
   const double town = HEAT ? th[t] : 0.0;
   const int i = tb.base + t;
+  double gm[9]; /* gm[3 i + d] = d(rho u_i)/dx_d */
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int fa = d == 0 ? 1 : (d == 1 ? 0 : 4);
@@ -428,10 +429,14 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
       for (int v = 1; v < 4; ++v) acc[v] += 0.5 * (reg[(v - V0) * HR + r] + U[v]) * wgt;
       if constexpr (HEAT) acct += 0.5 * (th[r] + town) * wgt;
     }
-    if constexpr (HEAT) SGm[((size_t)9 + d) * M.ns + i] = (R)acct;
+    if constexpr (HEAT) SGm[((size_t)VROW_HEAT + d) * M.ns + i] = (R)acct;
 #pragma unroll
-    for (int v = 1; v < 4; ++v) SGm[((size_t)(v - 1) * 3 + d) * M.ns + i] = (R)acc[v];
+    for (int v = 1; v < 4; ++v) gm[(v - 1) * 3 + d] = acc[v];
   }
+  double vr[6];
+  vrows<double>(gm, vr);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) SGm[(size_t)k * M.ns + i] = (R)vr[k];
 }
 
 template <int MODEL, typename R>
