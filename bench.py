@@ -21,6 +21,7 @@ Beside the headline, on the same run:
   adjoint       the isentropic-vortex rollout (496,896 leaves, 50 steps): adjoint /
                 forward time with checkpoints every 10 steps and with the rollout
                 stored, and the worst relative difference from central differences;
+                the mixed form's forward and adjoint timed beside the plain ones;
   extra         the 3D blast with AMR (base 32^3 + 2 levels, evolved 200 steps with
                 adaptation every 10), timed the same way on its final mesh.
 """
@@ -89,7 +90,7 @@ def ref_bytes_per_cell(part, neq, f, w, ns_visc):
             "stage": k0 + k1 + k2 + k3 + k4}[part]
 
 
-def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
+def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False, sfp32=False):
     """Bytes a launch of this build's fused kernels must move at least: each input
     read once, each output written once (MUSCL inside the face kernel).  rs is the
     byte width of the face flux and the gradients (8 plain, 4 mixed).  For the
@@ -99,15 +100,17 @@ def compulsory_bytes(kernel, m, neq, rs, ns_visc, first_stage, heat=False):
     writes the new state (and, mixed, its FP32 copy, the region then being read from
     that copy).  Viscous rows per cell: the six symmetric combinations of the momentum
     gradients (three per axis), plus the three components of grad(p / rho) with heat
-    conduction (one per axis)."""
+    conduction (one per axis).  sfp32: the gradient kernels read the FP32 copy of the
+    state (the mixed ideal-gas form)."""
     N, F, Q, Fr = m["N"], m["F"], m["Q"], m["Freg"]
     meta_c = 4 * N + 24 * N + 16 * Q          # cflag + nbr + quad neighbours
     grows = 9 if heat else 6                   # viscous rows per cell
     arows = 4 if heat else 3                   # rows an axis's faces read
+    sb = 4 if sfp32 else 8                     # bytes of a state value the gradients read
     if kernel == "grad":                       # S rows -> momentum (and p / rho) gradients
-        return 8 * (neq if heat else 3) * N + meta_c + grows * rs * N
+        return sb * (neq if heat else 3) * N + meta_c + grows * rs * N
     if kernel == "tile_grad":                  # tile + one layer beyond each face -> gradients
-        return m.get("tiles", 0) * (896 * 8 * (neq if heat else 3) + 512 * grows * rs + 32)
+        return m.get("tiles", 0) * (896 * sb * (neq if heat else 3) + 512 * grows * rs + 32)
     if kernel == "tile_stage":
         nt = m.get("tiles", 0)
         region = 1280 * neq * (4 + 8 * 512 / 1280 if rs == 4 else 8)
@@ -381,7 +384,8 @@ def time_workload(F, p, meta, a, quiet, with_kernels=True):
             e["plain_traffic_gbs"] = round(tb / (k_p[i] * 1e-3) / 1e9, 1)
         for tag, ms, rs in (("plain", k_p[i], 8), ("mixed", k_m[i], 4)):
             if ms:
-                cb = compulsory_bytes(k, mf, neq, rs, p.ns, False, heat)
+                cb = compulsory_bytes(k, mf, neq, rs, p.ns, False, heat,
+                                      sfp32=rs == 4 and p.model == 1 and p.ischeme == 2)
                 e[f"{tag}_compulsory_bytes"] = int(cb)
                 e[f"{tag}_compulsory_gbs"] = round(cb / (ms * 1e-3) / 1e9, 1)
         kern[k] = e
@@ -410,7 +414,9 @@ def adjoint_block(quiet):
     van Albada MUSCL, 50 SSP-RK3 steps, J = time-integrated probe pressure.  Forward =
     the plain rollout (fvg_steps); adjoint = the reverse sweep (fvg_adj_backward into
     HBM, recomputing checkpointed segments), after a forward that keeps checkpoints
-    every 10 steps, or every stage state (spacing chosen from free memory)."""
+    every 10 steps, or every stage state (spacing chosen from free memory).  The mixed
+    form's rollout and reverse sweep (checkpoints every 10 steps) are timed beside them,
+    with the relative L2 difference of its gradient from the plain one."""
     import torch
 
     import fv_140823_amd as fv
@@ -472,11 +478,39 @@ def adjoint_block(quiet):
         fd = ((ph - mh) + (pl - ml)) / (2 * eps)
         worst = max(worst, abs(float((lam * v).sum()) - fd) / abs(fd))
     g.close()
+    # the mixed form (FP32 face states, fluxes and their reverse; FP64 accumulation) on the
+    # same rollout, checkpoints every 10 steps, timed against the plain form
+    pm = type(p).from_buffer_copy(p)
+    pm.precision = fv.FVG_MIXED
+    gm = fv.Solver.from_forest(F, pm)
+    gm.adj_set_probe(probe)
+    gm.set_state(S0)
+    gm.adj_forward(2, 1, dt)  # warm-up
+    fwd_m, adj_m = [], []
+    for _ in range(2):
+        gm.set_state(S0)
+        gm.sync()
+        t0 = time.perf_counter()
+        gm.steps(n, dt, 3)
+        gm.sync()
+        fwd_m.append(time.perf_counter() - t0)
+        gm.set_state(S0)
+        gm.adj_forward(n, meta["ckpt"], dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gm.adj_backward_device(lam_dev.data_ptr())
+        adj_m.append(time.perf_counter() - t0)
+    lam_m = lam_dev.cpu().numpy()
+    gm.close()
+    mixed = dict(forward_ms=round(min(fwd_m) * 1e3, 3), adjoint_ms_ckpt10=round(min(adj_m) * 1e3, 3),
+                 forward_speedup_vs_plain=round(t_fwd / min(fwd_m), 3),
+                 adjoint_speedup_vs_plain=round(res["ckpt10"]["adjoint_ms"] / (min(adj_m) * 1e3), 3),
+                 grad_rel_l2_vs_plain=float(f"{np.linalg.norm(lam_m[:4] - lam[:4]) / np.linalg.norm(lam[:4]):.3e}"))
     out = dict(workload=meta["name"], leaves=F.nleaves, steps=n, dt=dt, forward_ms=round(t_fwd * 1e3, 3),
                ratio_ckpt10=res["ckpt10"]["ratio"], ratio_store=res["store"]["ratio"],
                ckpt10=res["ckpt10"], store=res["store"],
                store_grad_equal_ckpt10=bool(np.array_equal(grads["store"], grads["ckpt10"])),
-               fd_worst=float(f"{worst:.3e}"), fd_directions=3, fd_eps=eps)
+               fd_worst=float(f"{worst:.3e}"), fd_directions=3, fd_eps=eps, mixed=mixed)
     log(f"adjoint: {out}", quiet)
     return out
 

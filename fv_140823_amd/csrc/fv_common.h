@@ -61,6 +61,15 @@ __device__ __forceinline__ void load_S(const double *__restrict__ S, int ns, int
   for (int v = 0; v < NEQ; ++v) U[v] = p[v];
 }
 
+/* the same from the FP32 copy of the state (mixed form) */
+template <int NEQ>
+__device__ __forceinline__ void load_S(const float *__restrict__ S, int ns, int i, double *U) {
+  (void)ns;
+  const float *p = S + (size_t)i * NEQ;
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) U[v] = (double)p[v];
+}
+
 /* neighbour state across face f for MUSCL deltas (facevalues_MUSCL.c:18-49 and
  * the bc ghost of deltavalues.c:12-13) */
 template <int NEQ>
@@ -623,7 +632,7 @@ void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double 
                           int irk, double dt, bool write_rhs, bool fast);
 /* The momentum (and p / rho) gradients of the interior tiles' cells into SGm, bitwise
  * those of k_grad, which skips those cells' blocks while c->stage_gskip is set. */
-void fv_launch_tile_grad(fvg_ctx *c, const double *S);
+void fv_launch_tile_grad(fvg_ctx *c, const double *S, const float *Sf = nullptr);
 /* whether the tiles can run this stage (mesh, options, precision path) */
 bool fv_tiles_usable(const fvg_ctx *c);
 

@@ -75,10 +75,12 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(DevMesh M, DevParams P,
 /* Green-Gauss gradients (gradcalc.c:23-42).  FULL: all neq+1 rows (incl. p) to
  * SG f64 [neq+1][3][ns]; otherwise the gradients of the momenta (rows 1..3) as the
  * six viscous rows of vrows to SGm [9][ns] (RAW: the nine gradients themselves,
- * SGm[3 i + j] = d(rho u_i)/dx_j, for the adjoint and tangent-linear kernels). */
-template <int MODEL, bool FULL, typename R, bool HEAT = false, bool RAW = false>
+ * SGm[3 i + j] = d(rho u_i)/dx_j, for the adjoint and tangent-linear kernels).
+ * SS: the state's storage, the FP64 state or (the mixed form's viscous rows, which
+ * feed its FP32 viscous flux) its FP32 copy; the sums are FP64 either way. */
+template <int MODEL, bool FULL, typename R, bool HEAT = false, bool RAW = false, typename SS = double>
 __global__ __launch_bounds__(FV_BLOCK) void k_grad(DevMesh M, DevParams P,
-                                                   const double *__restrict__ S,
+                                                   const SS *__restrict__ S,
                                                    double *__restrict__ SG, R *__restrict__ SGm,
                                                    const uint8_t *__restrict__ skip,
                                                    const uint8_t *__restrict__ cmask) {
@@ -1490,10 +1492,23 @@ static void launch_recon(fvg_ctx *c, const double *S) {
 }
 
 template <int MODEL>
-static void launch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false) {
+static void launch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false,
+                        const float *Sf = nullptr) {
   DevMesh M = dev_mesh(c);
   dim3 g(nblocks(c->N)), b(FV_BLOCK);
   const uint8_t *skip = full ? nullptr : c->stage_gskip, *cmask = full ? nullptr : c->stage_cmask;
+  if (Sf && !full && !raw && c->rsize == 4) {
+    /* the mixed ideal-gas stage: the viscous rows from the FP32 copy of S */
+    if constexpr (MODEL == 1) {
+      if (c->dp.kheat != 0.0)
+        hipLaunchKernelGGL((k_grad<MODEL, false, float, true, false, float>), g, b, 0, c->stream, M,
+                           c->dp, Sf, c->SG, (float *)c->SGm, skip, cmask);
+      else
+        hipLaunchKernelGGL((k_grad<MODEL, false, float, false, false, float>), g, b, 0, c->stream, M,
+                           c->dp, Sf, c->SG, (float *)c->SGm, skip, cmask);
+      return;
+    }
+  }
   if (full)
     hipLaunchKernelGGL((k_grad<MODEL, true, double>), g, b, 0, c->stream, M, c->dp, S, c->SG,
                        (double *)nullptr, skip, cmask);
@@ -1650,9 +1665,10 @@ static void dispatch_recon(fvg_ctx *c, const double *S) {
     default: launch_recon<3>(c, S); break;
   }
 }
-static void dispatch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false) {
+static void dispatch_grad(fvg_ctx *c, const double *S, bool full, bool raw = false,
+                          const float *Sf = nullptr) {
   switch (c->prm.model) {
-    case 1: launch_grad<1>(c, S, full, raw); break;
+    case 1: launch_grad<1>(c, S, full, raw, Sf); break;
     case 2: launch_grad<2>(c, S, full, raw); break;
     default: launch_grad<3>(c, S, full, raw); break;
   }
@@ -1759,8 +1775,10 @@ static void stage(fvg_ctx *c, StagePtrs sp, int irk, double dt, bool first, bool
   c->stage_cmask = tiles ? c->tile_cell : nullptr;
   if (c->prm.ns) {
     mark(1);
-    if (!(tiles && c->gather_all_skip)) dispatch_grad(c, sp.S, false);
-    if (tiles) { mark(5); fv_launch_tile_grad(c, sp.S); }
+    /* the mixed ideal-gas form reads its FP32 copy of S for the viscous rows */
+    const float *sfin = fast ? c->Sf : nullptr;
+    if (!(tiles && c->gather_all_skip)) dispatch_grad(c, sp.S, false, false, sfin);
+    if (tiles) { mark(5); fv_launch_tile_grad(c, sp.S, sfin); }
   }
   mark(2);
   /* MUSCL: the reconstruction runs inside the face kernel (no SF round trip) */
@@ -2165,9 +2183,10 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
     c->stage_fskip = tiles ? c->face_skip : nullptr;
     c->stage_gskip = tiles ? c->gather_skip : nullptr;
     c->stage_cmask = tiles ? c->tile_cell : nullptr;
+    const float *sfin = fast ? c->Sf : nullptr;
     if (c->prm.ns && !(tiles && c->gather_all_skip))
-      ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false); });
-    if (c->prm.ns && tiles) ms_kernel[5] = batch([&] { fv_launch_tile_grad(c, c->S); });
+      ms_kernel[1] = batch([&] { dispatch_grad(c, c->S, false, false, sfin); });
+    if (c->prm.ns && tiles) ms_kernel[5] = batch([&] { fv_launch_tile_grad(c, c->S, sfin); });
     if (!(tiles && c->face_all_skip))
       ms_kernel[2] = batch([&] {
         if (fast) launch_face_fast(c, c->S);

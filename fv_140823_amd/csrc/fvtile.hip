@@ -16,7 +16,8 @@
  *      residual_faces_AMR.c:173-175) and applies the RK stage (RK.c:24-47).
  * k_tile_grad forms the Green-Gauss gradients of the tiles' cells from the tile and the
  * cell layer beyond each face, staged in LDS, with k_grad's arithmetic, and stores their
- * viscous rows (vrows) as k_grad does.
+ * viscous rows (vrows) as k_grad does (both read the FP32 copy of the state in the mixed
+ * ideal-gas form, whose viscous flux is FP32).
  * Face fluxes never go to HBM and every stencil row is read once per tile, where the
  * face-list path writes the face flux, reads it back from both cells and fetches each
  * stencil row for every face that uses it.  A face is evaluated exactly as the face-list
@@ -347,10 +348,10 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
  * loads, coalesced rows).  The arithmetic is k_grad's, term for term, so the values are
  * bitwise the same.  Halo slot
  * of face f: 512 + f x 64 + u x 8 + w, (u, w) the other two coordinates in axis order. */
-template <int MODEL, typename R, bool HEAT>
+template <int MODEL, typename R, bool HEAT, typename SS = double>
 __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
                                                   const int32_t *__restrict__ tiles,
-                                                  const double *__restrict__ S,
+                                                  const SS *__restrict__ S,
                                                   R *__restrict__ SGm) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   constexpr int NV = HEAT ? NEQ : 3; /* staged rows: all (theta needs them) or the momenta */
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
   const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
   const int L = row[7];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) reg[v * HR + t] = S[(size_t)(tb.base + t) * NEQ + V0 + v];
+  for (int v = 0; v < NV; ++v) reg[v * HR + t] = (double)S[(size_t)(tb.base + t) * NEQ + V0 + v];
   if (t < 384) {
     const int f = t >> 6, u = (t >> 3) & 7, w = t & 7;
     int cell;
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
       default: cell = tb.pz + tmorton(u, w, 0); break;
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v) reg[v * HR + TT + t] = S[(size_t)cell * NEQ + V0 + v];
+    for (int v = 0; v < NV; ++v) reg[v * HR + TT + t] = (double)S[(size_t)cell * NEQ + V0 + v];
   }
   __syncthreads();
   if constexpr (HEAT) {
@@ -440,8 +441,20 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
 }
 
 template <int MODEL, typename R>
-void launch_tile_grad_r(fvg_ctx *c, const double *S) {
+void launch_tile_grad_r(fvg_ctx *c, const double *S, const float *Sf) {
   DevMesh M = dev_mesh(c);
+  if constexpr (MODEL == 1 && sizeof(R) == 4) {
+    if (Sf) {
+      /* the mixed ideal-gas stage: from the FP32 copy of S */
+      if (c->dp.kheat != 0.0)
+        hipLaunchKernelGGL((k_tile_grad<1, R, true, float>), dim3(c->ntile), dim3(TT), 0, c->stream, M,
+                           c->dp, (const int32_t *)c->tiles, Sf, (R *)c->SGm);
+      else
+        hipLaunchKernelGGL((k_tile_grad<1, R, false, float>), dim3(c->ntile), dim3(TT), 0, c->stream, M,
+                           c->dp, (const int32_t *)c->tiles, Sf, (R *)c->SGm);
+      return;
+    }
+  }
   if constexpr (MODEL == 1) {
     if (c->dp.kheat != 0.0) {
       hipLaunchKernelGGL((k_tile_grad<1, R, true>), dim3(c->ntile), dim3(TT), 0, c->stream, M, c->dp,
@@ -519,14 +532,14 @@ void fv_launch_tile_stage(fvg_ctx *c, const double *S, const double *SN, double 
   }
 }
 
-void fv_launch_tile_grad(fvg_ctx *c, const double *S) {
+void fv_launch_tile_grad(fvg_ctx *c, const double *S, const float *Sf) {
   if (c->prm.model == 2) return; /* no viscous form */
   if (c->prm.model == 1) {
-    if (c->rsize == 8) launch_tile_grad_r<1, double>(c, S);
-    else launch_tile_grad_r<1, float>(c, S);
+    if (c->rsize == 8) launch_tile_grad_r<1, double>(c, S, nullptr);
+    else launch_tile_grad_r<1, float>(c, S, Sf);
   } else {
-    if (c->rsize == 8) launch_tile_grad_r<3, double>(c, S);
-    else launch_tile_grad_r<3, float>(c, S);
+    if (c->rsize == 8) launch_tile_grad_r<3, double>(c, S, nullptr);
+    else launch_tile_grad_r<3, float>(c, S, nullptr);
   }
 }
 
