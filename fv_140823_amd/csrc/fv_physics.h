@@ -505,13 +505,15 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
 #pragma unroll
   for (int v = 0; v < 5; ++v) Fb[v] = ob[v] * s;
   if (region == 2 || region == 3) {
-    /* forward values as hllc_face forms them; the bars multiply by reciprocals */
+    /* the forward's star-state values (hllc_face), formed with three reciprocals that
+     * the bars multiply by as well: they enter only the derivatives, so their last-bit
+     * differences from the forward's quotients do not matter */
     const T W = Kr * (SK - uK);
     const T B = SK - SS;
-    const T cs = qdiv(W, B);
-    const T Y = SS + qdiv(Kp, W);
-    const T X = qdiv(Ke, Kr) + (SS - uK) * Y;
     const T iKr = qdiv((T)1, Kr), iW = qdiv((T)1, W), iB = qdiv((T)1, B);
+    const T cs = W * iB;
+    const T Y = SS + Kp * iW;
+    const T X = Ke * iKr + (SS - uK) * Y;
     T Us[5];
     Us[0] = cs;
 #pragma unroll

@@ -93,15 +93,16 @@ __device__ __forceinline__ void adj_face_body(const DevMesh &M, const DevParams 
     recon_face<5, T>(M, P, S, iL, fl, XL);
     if (bc == 0) recon_face<5, T>(M, P, S, iR, fr, XR);
   }
-  const double VL = cell_vol(M, iL), VR = cell_vol(M, iR);
+  /* mu = cdt lam / Vol with one quotient per side, not one per variable */
+  const double cL = cdt / cell_vol(M, iL), cR = cdt / cell_vol(M, iR);
   T UL[5], UR[5], ob[5], ULb[5], URb[5];
   const T A = (T)M.area[3 * lev + a];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     UL[v] = (T)XL[v];
     UR[v] = bc == 0 ? (T)XR[v] : (bc == 1 ? (T)P.inlet[v] : UL[v]);
-    const double muL = cdt * lam[(size_t)v * M.ns + iL] / VL;
-    const double muR = bc == 0 ? cdt * lam[(size_t)v * M.ns + iR] / VR : 0.0;
+    const double muL = lam[(size_t)v * M.ns + iL] * cL;
+    const double muR = bc == 0 ? lam[(size_t)v * M.ns + iR] * cR : 0.0;
     ob[v] = -A * (T)(muL - muR);
     ULb[v] = URb[v] = (T)0;
   }
