@@ -171,6 +171,7 @@ _DECLS = [
     ("fvg_adj_objective", C.c_int, [_vp, _dp]),
     ("fvg_adj_backward", C.c_int, [_vp, _dp, _vp]),
     ("fvg_tlm", C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp, _dp]),
+    ("fvg_eos_host", C.c_int, [C.POINTER(FvgParams), _dp, _dp]),
     ("fvg_face_flux_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp]),
     ("fvg_face_vjp_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     ("fvg_face_flux_ns_host", C.c_int, [C.POINTER(FvgParams), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),

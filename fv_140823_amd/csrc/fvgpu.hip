@@ -2216,6 +2216,23 @@ int fvg_time_steps(fvg_ctx *c, int nsteps, int method, double dt, double *ms_tot
   return FVG_OK;
 }
 
+int fvg_eos_host(const fvg_params *p, const double *U, double *out) {
+  if (!p || !U || !out || fvg_neq(p->model) < 0) return FVG_ERR_ARG;
+  DevParams P;
+  memset(&P, 0, sizeof P);
+  to_dev_params(p, &P);
+  double r, u[3], e, pr = 0.0, c = 0.0, y;
+  bool ok;
+  switch (p->model) {
+    case 1: ok = eos64<1>(P, U, false, r, u, e, pr, c, y); break;
+    case 2: ok = eos64<2>(P, U, false, r, u, e, pr, c, y); break;
+    default: ok = eos64<3>(P, U, false, r, u, e, pr, c, y); break;
+  }
+  out[0] = pr;
+  out[1] = c;
+  return ok ? FVG_OK : FVG_ERR_NONFINITE;
+}
+
 int fvg_cfl_dt(fvg_ctx *c, double cfl, double *dt) {
   if (!c || !dt) return FVG_ERR_ARG;
   if (!c->has_mesh) return set_err(c, FVG_ERR_STATE, "no mesh");

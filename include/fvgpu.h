@@ -351,6 +351,10 @@ int fvg_face_vjp_ns_host(const fvg_params *p, int axis, int sign, const double *
                          double *ULb, double *URb, double *gb);
 int fvg_face_jvp_host(const fvg_params *p, int axis, int sign, const double *UL, const double *UR,
                       const double *ULd, const double *URd, double *outd);
+/* Host evaluation of the device equation of state (eos64: ideal / stiffened gas, the
+ * barotropic liquid-vapour law with its series power near rho_sat, the two-phase root) of
+ * one conserved vector: out[0] = p, out[1] = c.  FVG_ERR_NONFINITE on a failed root. */
+int fvg_eos_host(const fvg_params *p, const double *U, double *out /* [2] */);
 int fvg_limiter_host(int limiter, double eps, double d0, double d1, double dp1cc, double dm1cc,
                      double *g, double *g0, double *g1);
 /* Both face states' partials of one axis (the adjoint's reconstruction transpose):
