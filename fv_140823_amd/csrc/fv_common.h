@@ -168,6 +168,23 @@ __device__ __forceinline__ T limited(int lim, double eps, bool bnd, T d0, T d1, 
   return grad;
 }
 
+/* limited<T> on a regular face: both one-sided slopes over the same finite distance,
+ * given as its reciprocal ic.  In FP32 the minmod selection is one median: med3(a, b, 0)
+ * is the slope of smaller magnitude when a = -d1 ic and b = d0 ic have one sign and 0
+ * when their signs differ or one is 0, the element the select form of limited<float>
+ * picks (the two differ only where a b underflows, |d1| below ~1e-37).  FP64 and van
+ * Albada are limited<T> itself. */
+template <typename T>
+__device__ __forceinline__ T limited_regular(int lim, double eps, T d0, T d1, T ic) {
+  if constexpr (sizeof(T) == 4) {
+    if (lim == FVG_MINMOD) {
+      const float m = __builtin_amdgcn_fmed3f(-d1 * ic, d0 * ic, 0.0f);
+      return fabsf(d0) > 1e-10f ? m : 0.0f;
+    }
+  }
+  return limited<T>(lim, eps, false, d0, d1, ic, ic);
+}
+
 /* XCD-aware block order: consecutive logical blocks (contiguous Morton ranges
  * of cells or faces) land on the same XCD and so share its L2. */
 __device__ __forceinline__ int logical_block() {
@@ -285,10 +302,8 @@ __device__ __forceinline__ void recon_regular(const DevMesh &M, const DevParams 
   load_S<NEQ>(S, M.ns, M.fRR[j], B1);
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0[v] - A1[v]), (T)(A0[v] - A1[v]),
-                            (T)ic, (T)ic);
-    const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1[v] - B0[v]), (T)(B1[v] - B0[v]),
-                            (T)ic, (T)ic);
+    const T gL = limited_regular<T>(P.limiter, P.va_eps, (T)(B0[v] - A1[v]), (T)(A0[v] - A1[v]), (T)ic);
+    const T gR = limited_regular<T>(P.limiter, P.va_eps, (T)(A1[v] - B0[v]), (T)(B1[v] - B0[v]), (T)ic);
     UL[v] = A1[v] + (double)((T)pcf * gL);
     UR[v] = B0[v] + (double)((T)pcf * gR);
   }

@@ -415,8 +415,8 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
     load_Sf<5>(Sf, M.fRR[j], B1);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      U[v] = A1[v] + fp * limited<float>(P.limiter, P.va_eps, false, B0[v] - A1[v], A0[v] - A1[v], ic, ic);
-      V[v] = B0[v] + fp * limited<float>(P.limiter, P.va_eps, false, A1[v] - B0[v], B1[v] - B0[v], ic, ic);
+      U[v] = A1[v] + fp * limited_regular<float>(P.limiter, P.va_eps, B0[v] - A1[v], A0[v] - A1[v], ic);
+      V[v] = B0[v] + fp * limited_regular<float>(P.limiter, P.va_eps, A1[v] - B0[v], B1[v] - B0[v], ic);
     }
     fp32 = make_state_mixed_ideal(P, U, L);
     fp32 = make_state_mixed_ideal(P, V, Rs) && fp32;

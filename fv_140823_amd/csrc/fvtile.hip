@@ -128,8 +128,8 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
     for (int v = 0; v < 5; ++v) {
       const float A0 = reg[v * TREG + q0], A1 = reg[v * TREG + q1];
       const float B0 = reg[v * TREG + q2], B1 = reg[v * TREG + q3];
-      U[v] = A1 + fp * limited<float>(P.limiter, P.va_eps, false, B0 - A1, A0 - A1, fic, fic);
-      V[v] = B0 + fp * limited<float>(P.limiter, P.va_eps, false, A1 - B0, B1 - B0, fic, fic);
+      U[v] = A1 + fp * limited_regular<float>(P.limiter, P.va_eps, B0 - A1, A0 - A1, fic);
+      V[v] = B0 + fp * limited_regular<float>(P.limiter, P.va_eps, A1 - B0, B1 - B0, fic);
     }
     done = make_state_mixed_ideal(P, U, Ls);
     done = make_state_mixed_ideal(P, V, Rs) && done;
@@ -155,8 +155,8 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
         A0 = reg[v * TREG + q0]; A1 = reg[v * TREG + q1];
         B0 = reg[v * TREG + q2]; B1 = reg[v * TREG + q3];
       }
-      const T gL = limited<T>(P.limiter, P.va_eps, false, (T)(B0 - A1), (T)(A0 - A1), (T)ic, (T)ic);
-      const T gR = limited<T>(P.limiter, P.va_eps, false, (T)(A1 - B0), (T)(B1 - B0), (T)ic, (T)ic);
+      const T gL = limited_regular<T>(P.limiter, P.va_eps, (T)(B0 - A1), (T)(A0 - A1), (T)ic);
+      const T gR = limited_regular<T>(P.limiter, P.va_eps, (T)(A1 - B0), (T)(B1 - B0), (T)ic);
       UL[v] = A1 + (double)((T)pcf * gL);
       UR[v] = B0 + (double)((T)pcf * gR);
     }
