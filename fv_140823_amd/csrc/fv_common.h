@@ -185,6 +185,21 @@ __device__ __forceinline__ T limited_regular(int lim, double eps, T d0, T d1, T 
   return limited<T>(lim, eps, false, d0, d1, ic, ic);
 }
 
+/* An FP32 MUSCL face value on a regular face: own + fp g with g = limited_regular(other -
+ * own, far - own).  For minmod the distance's reciprocal is taken out of the median
+ * (med3(-d1 ic, d0 ic, 0) = ic med3(-d1, d0, 0), ic > 0) and folded into fp: fpic = fp ic,
+ * one product per face instead of two per variable.  The mixed ideal-gas kernels, the
+ * LDS-tiled one and the face-list one, both form their regular face states with it. */
+__device__ __forceinline__ float muscl_regular_f(int lim, double eps, float own, float other,
+                                                 float far, float fp, float ic, float fpic) {
+  const float d0 = other - own, d1 = far - own;
+  if (lim == FVG_MINMOD) {
+    const float m = __builtin_amdgcn_fmed3f(-d1, d0, 0.0f);
+    return fmaf(fpic, fabsf(d0) > 1e-10f ? m : 0.0f, own);
+  }
+  return own + fp * limited<float>(lim, eps, false, d0, d1, ic, ic);
+}
+
 /* XCD-aware block order: consecutive logical blocks (contiguous Morton ranges
  * of cells or faces) land on the same XCD and so share its L2. */
 __device__ __forceinline__ int logical_block() {

@@ -407,7 +407,7 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
     /* regular face: recon_face_f with the four-cell stencil and the same-level
      * distances known (same arithmetic) */
     const int k = (lev * 3 + a) * 3;
-    const float fp = (float)M.dist[2 * k], ic = (float)M.idist[k];
+    const float fp = (float)M.dist[2 * k], ic = (float)M.idist[k], fpic = fp * ic;
     float A0[5], A1[5], B0[5], B1[5], V[5];
     load_Sf<5>(Sf, M.fLL[j], A0);
     load_Sf<5>(Sf, iL, A1);
@@ -415,8 +415,8 @@ __device__ __forceinline__ bool face_fast_body(const DevMesh &M, const DevParams
     load_Sf<5>(Sf, M.fRR[j], B1);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      U[v] = A1[v] + fp * limited_regular<float>(P.limiter, P.va_eps, B0[v] - A1[v], A0[v] - A1[v], ic);
-      V[v] = B0[v] + fp * limited_regular<float>(P.limiter, P.va_eps, A1[v] - B0[v], B1[v] - B0[v], ic);
+      U[v] = muscl_regular_f(P.limiter, P.va_eps, A1[v], B0[v], A0[v], fp, ic, fpic);
+      V[v] = muscl_regular_f(P.limiter, P.va_eps, B0[v], A1[v], B1[v], fp, ic, fpic);
     }
     fp32 = make_state_mixed_ideal(P, U, L);
     fp32 = make_state_mixed_ideal(P, V, Rs) && fp32;

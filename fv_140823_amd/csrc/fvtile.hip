@@ -122,14 +122,14 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
   bool done = false;
   if constexpr (FAST) {
     /* the FP32 face states of the mixed ideal-gas kernel (face_fast_body, regular) */
-    const float fp = (float)pcf, fic = (float)ic;
+    const float fp = (float)pcf, fic = (float)ic, fpic = fp * fic;
     float U[5], V[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const float A0 = reg[v * TREG + q0], A1 = reg[v * TREG + q1];
       const float B0 = reg[v * TREG + q2], B1 = reg[v * TREG + q3];
-      U[v] = A1 + fp * limited_regular<float>(P.limiter, P.va_eps, B0 - A1, A0 - A1, fic);
-      V[v] = B0 + fp * limited_regular<float>(P.limiter, P.va_eps, A1 - B0, B1 - B0, fic);
+      U[v] = muscl_regular_f(P.limiter, P.va_eps, A1, B0, A0, fp, fic, fpic);
+      V[v] = muscl_regular_f(P.limiter, P.va_eps, B0, A1, B1, fp, fic, fpic);
     }
     done = make_state_mixed_ideal(P, U, Ls);
     done = make_state_mixed_ideal(P, V, Rs) && done;
