@@ -221,7 +221,18 @@ __device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ l
   sf_bar_at<R>(lUL, lUR, cf, f, M.gslot[(size_t)f * M.ns + i], out);
 }
 
-/* recon^T, part 1: own bar and the bars sent to the +/- neighbour of each axis */
+/* Whether a same-level neighbour nb of a cell lies in the cell's own block of FV_BLOCK
+ * consecutive cells: the bars between such a pair go through LDS in k_adj_recon, all
+ * others through the neighbour-bar arrays Pp / Pm. */
+__device__ __forceinline__ bool same_block(int i, int nb) {
+  return nb >= 0 && nb / FV_BLOCK == i / FV_BLOCK;
+}
+
+/* recon^T, part 1: own bar and the bars sent to the +/- neighbour of each axis.  A bar
+ * for a same-level neighbour in the same block of FV_BLOCK cells (most of them: a block
+ * is a Morton brick) is handed over in LDS and added here; the others (across a block
+ * boundary, across a level change) are written to Pp / Pm for k_adj_recv_stage, which
+ * adds them.  `self` receives the own bar plus the bars received in the block. */
 template <typename R>
 __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
                                                         const double *__restrict__ S,
@@ -230,12 +241,15 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
                                                         double *__restrict__ self,
                                                         double *__restrict__ Pp,
                                                         double *__restrict__ Pm) {
-  const int i = logical_block() * FV_BLOCK + threadIdx.x;
-  if (i >= M.N) return;
-  const uint32_t cf = M.cflag[i];
+  const int t = threadIdx.x;
+  const int i = logical_block() * FV_BLOCK + t;
+  const bool live = i < M.N;
   const size_t pl = (size_t)5 * M.ns;
   double acc[5] = {0, 0, 0, 0, 0};
   if (P.ischeme == 1) {
+    /* first order: no limiter, nothing to send */
+    if (!live) return;
+    const uint32_t cf = M.cflag[i];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       double b[5];
@@ -244,26 +258,26 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
       for (int v = 0; v < 5; ++v) acc[v] += b[v];
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      self[(size_t)v * M.ns + i] = acc[v];
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        Pp[ax * pl + (size_t)v * M.ns + i] = 0.0;
-        Pm[ax * pl + (size_t)v * M.ns + i] = 0.0;
-      }
-    }
+    for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
     return;
   }
-  double U[5];
-  load_S<5>(S, M.ns, i, U);
-  /* the neighbours and slots of all six faces are read before the axis loop, so that
-   * each axis waits for one round of loads (its states and bars), not two */
-  const int n0 = M.nbr[i], n1 = M.nbr[(size_t)M.ns + i], n2 = M.nbr[(size_t)2 * M.ns + i];
-  const int n3 = M.nbr[(size_t)3 * M.ns + i], n4 = M.nbr[(size_t)4 * M.ns + i];
-  const int n5 = M.nbr[(size_t)5 * M.ns + i];
-  const int s0 = M.gslot[i], s1 = M.gslot[(size_t)M.ns + i], s2 = M.gslot[(size_t)2 * M.ns + i];
-  const int s3 = M.gslot[(size_t)3 * M.ns + i], s4 = M.gslot[(size_t)4 * M.ns + i];
-  const int s5 = M.gslot[(size_t)5 * M.ns + i];
+  /* the bars sent to the + and - neighbour of the current axis, by thread */
+  __shared__ double sP[5][FV_BLOCK], sM[5][FV_BLOCK];
+  const uint32_t cf = live ? M.cflag[i] : 0u;
+  double U[5] = {0, 0, 0, 0, 0};
+  int n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0, s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1,
+      s5 = -1;
+  if (live) {
+    load_S<5>(S, M.ns, i, U);
+    /* the neighbours and slots of all six faces are read before the axis loop, so that
+     * each axis waits for one round of loads (its states and bars), not two */
+    n0 = M.nbr[i]; n1 = M.nbr[(size_t)M.ns + i]; n2 = M.nbr[(size_t)2 * M.ns + i];
+    n3 = M.nbr[(size_t)3 * M.ns + i]; n4 = M.nbr[(size_t)4 * M.ns + i];
+    n5 = M.nbr[(size_t)5 * M.ns + i];
+    s0 = M.gslot[i]; s1 = M.gslot[(size_t)M.ns + i]; s2 = M.gslot[(size_t)2 * M.ns + i];
+    s3 = M.gslot[(size_t)3 * M.ns + i]; s4 = M.gslot[(size_t)4 * M.ns + i];
+    s5 = M.gslot[(size_t)5 * M.ns + i];
+  }
   /* one axis at a time (not unrolled): the three axes' 20 live values each would
    * otherwise be interleaved and cost the kernel most of its occupancy */
 #pragma unroll 1
@@ -271,37 +285,59 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
     const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
     const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
     const int np = ax == 0 ? n1 : (ax == 1 ? n2 : n5), nm = ax == 0 ? n3 : (ax == 1 ? n0 : n4);
-    const int gp = ax == 0 ? s1 : (ax == 1 ? s2 : s5), gm = ax == 0 ? s3 : (ax == 1 ? s0 : s4);
-    double Up[5], Um[5], lp[5], lm[5];
-    nb_state_at<5>(M, P, S, i, cf, fp, np, U, Up);
-    nb_state_at<5>(M, P, S, i, cf, fm, nm, U, Um);
-    sf_bar_at<R>(lUL, lUR, cf, fp, gp, lp);
-    sf_bar_at<R>(lUL, lUR, cf, fm, gm, lm);
-    double pcf, pcc, mcf, mcc, pic, mic;
-    face_dist(M, cf, fp, pcf, pcc, pic);
-    face_dist(M, cf, fm, mcf, mcc, mic);
     const int cp = (cf >> (2 * fp)) & 3, cm = (cf >> (2 * fm)) & 3;
-    const bool bp = cp == CLS_BOUNDARY, bm = cm == CLS_BOUNDARY;
-    const bool ghost_p = bp && (-1 - np) == 2;
-    const bool ghost_m = bm && (-1 - nm) == 2;
+    const bool inP = live && cp == CLS_SAME && same_block(i, np);
+    const bool inM = live && cm == CLS_SAME && same_block(i, nm);
+    if (live) {
+      const int gp = ax == 0 ? s1 : (ax == 1 ? s2 : s5), gm = ax == 0 ? s3 : (ax == 1 ? s0 : s4);
+      double Up[5], Um[5], lp[5], lm[5];
+      nb_state_at<5>(M, P, S, i, cf, fp, np, U, Up);
+      nb_state_at<5>(M, P, S, i, cf, fm, nm, U, Um);
+      sf_bar_at<R>(lUL, lUR, cf, fp, gp, lp);
+      sf_bar_at<R>(lUL, lUR, cf, fm, gm, lm);
+      double pcf, pcc, mcf, mcc, pic, mic;
+      face_dist(M, cf, fp, pcf, pcc, pic);
+      face_dist(M, cf, fm, mcf, mcc, mic);
+      const bool bp = cp == CLS_BOUNDARY, bm = cm == CLS_BOUNDARY;
+      const bool ghost_p = bp && (-1 - np) == 2;
+      const bool ghost_m = bm && (-1 - nm) == 2;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const double dP = Up[v] - U[v], dM = Um[v] - U[v];
-      double g0, g1, h0, h1;
-      limited_d_axis(P.limiter, P.va_eps, bp, bm, dP, dM, pic, mic, g0, g1, h0, h1);
-      /* SF[fp] = U + pcf g(dP, dM), SF[fm] = U + mcf g(dM, dP) */
-      double dPb = lp[v] * pcf * g0 + lm[v] * mcf * h1;
-      double dMb = lp[v] * pcf * g1 + lm[v] * mcf * h0;
-      /* a transmissive ghost equals the cell itself: its delta is identically 0 */
-      if (ghost_p) dPb = 0.0;
-      if (ghost_m) dMb = 0.0;
-      acc[v] += lp[v] + lm[v] - dPb - dMb;
-      Pp[ax * pl + (size_t)v * M.ns + i] = bp ? 0.0 : dPb;
-      Pm[ax * pl + (size_t)v * M.ns + i] = bm ? 0.0 : dMb;
+      for (int v = 0; v < 5; ++v) {
+        const double dP = Up[v] - U[v], dM = Um[v] - U[v];
+        double g0, g1, h0, h1;
+        limited_d_axis(P.limiter, P.va_eps, bp, bm, dP, dM, pic, mic, g0, g1, h0, h1);
+        /* SF[fp] = U + pcf g(dP, dM), SF[fm] = U + mcf g(dM, dP) */
+        double dPb = lp[v] * pcf * g0 + lm[v] * mcf * h1;
+        double dMb = lp[v] * pcf * g1 + lm[v] * mcf * h0;
+        /* a transmissive ghost equals the cell itself: its delta is identically 0 */
+        if (ghost_p) dPb = 0.0;
+        if (ghost_m) dMb = 0.0;
+        acc[v] += lp[v] + lm[v] - dPb - dMb;
+        sP[v][t] = dPb;
+        sM[v][t] = dMb;
+        if (!inP) Pp[ax * pl + (size_t)v * M.ns + i] = bp ? 0.0 : dPb;
+        if (!inM) Pm[ax * pl + (size_t)v * M.ns + i] = bm ? 0.0 : dMb;
+      }
     }
-  }
+    __syncthreads();
+    /* what the same-level neighbours of this block sent: my - neighbour's bar for its +
+     * side, my + neighbour's for its - side */
+    if (inM) {
+      const int q = nm - (i - t);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
+      for (int v = 0; v < 5; ++v) acc[v] += sP[v][q];
+    }
+    if (inP) {
+      const int q = np - (i - t);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += sM[v][q];
+    }
+    __syncthreads();
+  }
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
+  }
 }
 
 /* the probe objective's gradient at state S, added to lam: dt Vol dp/dS */
@@ -329,7 +365,7 @@ __device__ __forceinline__ void obj_grad_cell(const DevMesh &M, const DevParams 
  * reverse sweep adds next; it is added here (the same additions k_obj_grad makes, in
  * the same order), saving a launch per step. */
 template <bool NS>
-__global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParams P,
+__global__ __launch_bounds__(FV_BLOCK, 4) void k_adj_recv_stage(DevMesh M, DevParams P,
                                                              const double *__restrict__ self,
                                                              const double *__restrict__ Pp,
                                                              const double *__restrict__ Pm,
@@ -347,9 +383,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParam
   double acc[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) acc[v] = self[(size_t)v * M.ns + i];
-  /* the six neighbours and the bars a single neighbour sent are loaded before any
-   * branch (the cell's own index stands in for the other classes), so that all of
-   * them are in flight together; the sums below keep their order */
+  /* the six neighbours and the bars a single neighbour sent through Pp / Pm are loaded
+   * before any branch, so that all of them are in flight together (a same-level
+   * neighbour in the cell's block handed its bar over in k_adj_recon's LDS, already in
+   * `self`); the sums below keep their order */
   int nbs[6];
   double rin[6][5];
 #pragma unroll
@@ -358,15 +395,16 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParam
     const int f = side == 0 ? (ax == 0 ? 3 : (ax == 1 ? 0 : 4)) : (ax == 0 ? 1 : (ax == 1 ? 2 : 5));
     nbs[q] = M.nbr[(size_t)f * M.ns + i];
   }
+  const bool muscl = P.ischeme != 1;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const int ax = q >> 1, side = q & 1;
     const int f = side == 0 ? (ax == 0 ? 3 : (ax == 1 ? 0 : 4)) : (ax == 0 ? 1 : (ax == 1 ? 2 : 5));
     const int cls = (cf >> (2 * f)) & 3;
     const double *Pin = (side == 0 ? Pp : Pm) + ax * pl;
-    const int src = (cls == CLS_SAME || cls == CLS_COARSER) ? nbs[q] : i;
+    const bool one = muscl && (cls == CLS_COARSER || (cls == CLS_SAME && !same_block(i, nbs[q])));
 #pragma unroll
-    for (int v = 0; v < 5; ++v) rin[q][v] = Pin[(size_t)v * M.ns + src];
+    for (int v = 0; v < 5; ++v) rin[q][v] = one ? Pin[(size_t)v * M.ns + nbs[q]] : 0.0;
   }
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
@@ -380,9 +418,13 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recv_stage(DevMesh M, DevParam
       const int cls = (cf >> (2 * f)) & 3;
       const int nb = nbs[2 * ax + side];
       const double *r = rin[2 * ax + side];
-      if (cls == CLS_SAME) {
+      if (!muscl) {
+        /* first order: no neighbour bars */
+      } else if (cls == CLS_SAME) {
+        if (!same_block(i, nb)) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[v] += r[v];
+          for (int v = 0; v < 5; ++v) acc[v] += r[v];
+        }
       } else if (cls == CLS_COARSER) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[v] += r[v] / ((double)4);
