@@ -221,20 +221,24 @@ __device__ __forceinline__ void sf_bar(const DevMesh &M, const R *__restrict__ l
   sf_bar_at<R>(lUL, lUR, cf, f, M.gslot[(size_t)f * M.ns + i], out);
 }
 
-/* Whether a same-level neighbour nb of a cell lies in the cell's own block of FV_BLOCK
+/* cells of a k_adj_recon workgroup: 512 consecutive cells, an 8 x 8 x 8 brick of a
+ * uniform region in Morton order */
+constexpr int ADJ_XB = 512;
+
+/* Whether a same-level neighbour nb of a cell lies in the cell's own block of ADJ_XB
  * consecutive cells: the bars between such a pair go through LDS in k_adj_recon, all
  * others through the neighbour-bar arrays Pp / Pm. */
 __device__ __forceinline__ bool same_block(int i, int nb) {
-  return nb >= 0 && nb / FV_BLOCK == i / FV_BLOCK;
+  return nb >= 0 && nb / ADJ_XB == i / ADJ_XB;
 }
 
 /* recon^T, part 1: own bar and the bars sent to the +/- neighbour of each axis.  A bar
- * for a same-level neighbour in the same block of FV_BLOCK cells (most of them: a block
+ * for a same-level neighbour in the same block of ADJ_XB cells (most of them: a block
  * is a Morton brick) is handed over in LDS and added here; the others (across a block
  * boundary, across a level change) are written to Pp / Pm for k_adj_recv_stage, which
  * adds them.  `self` receives the own bar plus the bars received in the block. */
 template <typename R>
-__global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
+__global__ __launch_bounds__(ADJ_XB) void k_adj_recon(DevMesh M, DevParams P,
                                                         const double *__restrict__ S,
                                                         const R *__restrict__ lUL,
                                                         const R *__restrict__ lUR,
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
                                                         double *__restrict__ Pp,
                                                         double *__restrict__ Pm) {
   const int t = threadIdx.x;
-  const int i = logical_block() * FV_BLOCK + t;
+  const int i = logical_block() * ADJ_XB + t;
   const bool live = i < M.N;
   const size_t pl = (size_t)5 * M.ns;
   double acc[5] = {0, 0, 0, 0, 0};
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_adj_recon(DevMesh M, DevParams P,
     return;
   }
   /* the bars sent to the + and - neighbour of the current axis, by thread */
-  __shared__ double sP[5][FV_BLOCK], sM[5][FV_BLOCK];
+  __shared__ double sP[5][ADJ_XB], sM[5][ADJ_XB];
   const uint32_t cf = live ? M.cflag[i] : 0u;
   double U[5] = {0, 0, 0, 0, 0};
   int n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0, s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1,
@@ -755,6 +759,7 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
   /* grids padded to a multiple of the 8 XCDs, so that logical_block() gives each XCD a
    * contiguous range (the kernels' bounds checks drop the padding) */
   const int nc = (int)round_up(nblk(c->N), 8);
+  const int nx = (int)round_up((c->N + ADJ_XB - 1) / ADJ_XB, 8);
   const bool ns = c->prm.ns != 0;
   if (ns) fv_launch_grad_raw(c, S); /* the momentum gradients of the stage's input, into c->SGm */
   dim3 gf((int)round_up(b0 + b1 + b2, 8)), b(FV_BLOCK);
@@ -769,7 +774,7 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     if (ns)
       hipLaunchKernelGGL(k_adj_sg<double>, dim3(nc), b, 0, c->stream, M, (const double *)lG, a->SGb);
-    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nc), b, 0, c->stream, M, c->dp, S,
+    hipLaunchKernelGGL((k_adj_recon<double>), dim3(nx), dim3(ADJ_XB), 0, c->stream, M, c->dp, S,
                        (const double *)lUL, (const double *)lUR, a->self, a->Pp, a->Pm);
   } else {
     const float *SGm = (const float *)c->SGm;
@@ -782,7 +787,7 @@ static void stage_vjp(fvg_ctx *c, const double *S, double cdt, double ca, double
                          n0, n1, n2, r0, r1, r2, S, a->lam, cdt, SGm, lUL, lUR, lG);
     if (ns)
       hipLaunchKernelGGL(k_adj_sg<float>, dim3(nc), b, 0, c->stream, M, (const float *)lG, a->SGb);
-    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nc), b, 0, c->stream, M, c->dp, S,
+    hipLaunchKernelGGL((k_adj_recon<float>), dim3(nx), dim3(ADJ_XB), 0, c->stream, M, c->dp, S,
                        (const float *)lUL, (const float *)lUR, a->self, a->Pp, a->Pm);
   }
   if (ns)
