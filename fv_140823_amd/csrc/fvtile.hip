@@ -95,24 +95,61 @@ __device__ __forceinline__ int reg_cell(const TileLinks &tb, int p, int u, int w
   return b + tmorton(x, y, z);
 }
 
-/* One face of axis a between the cells at positions p - 1 and p (p in [0, 8]) of the
- * row u, w.  The owner is the lower-indexed of the two cells, as in the face list:
- * the flux is the face-list kernel's for that face (recon_regular, make_state,
- * hllc_face, -Area), returned as the -1 side cell sees it through its +a face. */
+/* The cells and region slots of one face of axis a, between the cells at positions
+ * p - 1 and p of a row: cm, cp the two cells, c0, c3 those at p - 2 and p + 1 (read only
+ * by the mixed form's FP64 rebuild), r0..r3 the slots of positions p - 2 .. p + 1. */
+struct FaceIdx {
+  int cm, cp, c0, c3, r0, r1, r2, r3;
+};
+
+/* those of the face at position p (in [0, 8]) of the row u, w */
+template <int a>
+__device__ __forceinline__ FaceIdx face_idx(const TileLinks &tb, int p, int u, int w) {
+  return FaceIdx{reg_cell<a>(tb, p - 1, u, w), reg_cell<a>(tb, p, u, w),
+                 reg_cell<a>(tb, p - 2, u, w), reg_cell<a>(tb, p + 1, u, w),
+                 reg_slot<a>(p - 2, u, w), reg_slot<a>(p - 1, u, w), reg_slot<a>(p, u, w),
+                 reg_slot<a>(p + 1, u, w)};
+}
+
+/* the same for the -a face of the tile's cell t at position c along a (uw = u x 8 + w),
+ * stepping along a in the local Morton index: the bits of axis a sit at a, a + 3, a + 6
+ * (mask ma); with the other bits o = t & ~ma, the cell at position q of the row is
+ * o | spread(q), and spread(q -+ 1) = (spread(q) -+ 1 in those bits) & ma, a step from 0
+ * down wrapping to 7 (the neighbour tile's last layer) */
+template <int a>
+__device__ __forceinline__ FaceIdx own_face_idx(const TileLinks &tb, int t, int c, int uw) {
+  constexpr int ma = 0x49 << a, one = 1 << a;
+  const int o = t & ~ma, s0 = t & ma;
+  const int sm1 = (s0 - one) & ma, sm2 = (sm1 - one) & ma, sp1 = ((s0 | ~ma) + one) & ma;
+  const int bm = a == 0 ? tb.mx : (a == 1 ? tb.my : tb.mz);
+  const int bp = a == 0 ? tb.px : (a == 1 ? tb.py : tb.pz);
+  FaceIdx f;
+  f.cm = (c > 0 ? tb.base : bm) + (o | sm1);
+  f.cp = tb.base + t;
+  f.c0 = (c > 1 ? tb.base : bm) + (o | sm2);
+  f.c3 = (c < 7 ? tb.base : bp) + (o | sp1);
+  f.r0 = c > 1 ? (o | sm2) : 512 + (1 - c) * 64 + uw;
+  f.r1 = c > 0 ? (o | sm1) : 512 + uw;
+  f.r2 = t;
+  f.r3 = c < 7 ? (o | sp1) : 640 + uw;
+  return f;
+}
+
+/* One face of axis a (cells and slots fi).  The owner is the lower-indexed of the two
+ * cells, as in the face list: the flux is the face-list kernel's for that face
+ * (recon_regular, make_state, hllc_face, -Area), returned as the -1 side cell sees it
+ * through its +a face. */
 template <int MODEL, bool NS, int a, typename T, typename R, int OPT, bool FAST, typename ST>
 __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, const TileLinks &tb,
                                           int L, const ST *reg, const double *__restrict__ S,
-                                          const R *__restrict__ SGm, int p, int u, int w, R *out,
+                                          const R *__restrict__ SGm, const FaceIdx &fi, R *out,
                                           int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
-  const int cm = reg_cell<a>(tb, p - 1, u, w);
-  const int cp = reg_cell<a>(tb, p, u, w);
+  const int cm = fi.cm, cp = fi.cp;
   const bool own_minus = cm < cp;
   /* stencil in the owner's orientation: far side of the owner, owner, other, far side */
-  const int r0 = reg_slot<a>(p - 2, u, w), r1 = reg_slot<a>(p - 1, u, w),
-            r2 = reg_slot<a>(p, u, w), r3 = reg_slot<a>(p + 1, u, w);
-  const int q0 = own_minus ? r0 : r3, q1 = own_minus ? r1 : r2, q2 = own_minus ? r2 : r1,
-            q3 = own_minus ? r3 : r0;
+  const int q0 = own_minus ? fi.r0 : fi.r3, q1 = own_minus ? fi.r1 : fi.r2,
+            q2 = own_minus ? fi.r2 : fi.r1, q3 = own_minus ? fi.r3 : fi.r0;
   const int iL = own_minus ? cm : cp, iR = own_minus ? cp : cm;
   const T s = own_minus ? (T)1 : (T)-1;
   const int k = (L * 3 + a) * 3;
@@ -140,7 +177,7 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
     double UL[NEQ], UR[NEQ];
     const double *g0, *g1, *g2, *g3;
     if constexpr (FAST) {
-      const int c0 = reg_cell<a>(tb, p - 2, u, w), c3 = reg_cell<a>(tb, p + 1, u, w);
+      const int c0 = fi.c0, c3 = fi.c3;
       g0 = S + (size_t)(own_minus ? c0 : c3) * NEQ;
       g1 = S + (size_t)iL * NEQ;
       g2 = S + (size_t)iR * NEQ;
@@ -213,11 +250,13 @@ __device__ __forceinline__ void tile_pass(const DevMesh &M, const DevParams &P, 
   const int c = a == 0 ? cx : (a == 1 ? cy : cz);
   const int u = a == 0 ? cy : cx, w = a == 2 ? cy : cz;
   R out[NEQ];
-  tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm, c, u, w, out, err);
+  tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm,
+                                               own_face_idx<a>(tb, t, c, u * 8 + w), out, err);
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) fb[v * TFB + t] = out[v];
   if (t < 64) {
-    tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm, 8, t >> 3, t & 7, out, err);
+    tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm,
+                                                 face_idx<a>(tb, 8, t >> 3, t & 7), out, err);
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) fb[v * TFB + 512 + t] = out[v];
   }
