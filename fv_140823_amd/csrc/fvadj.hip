@@ -11,8 +11,10 @@
  *                     reconstruction), mu = c dt lam / Vol on both sides,
  *                     mu_L - mu_R -> hand-written VJP of HLLC + EOS
  *   k_adj_recon       per cell: face-state bars by gather slot (f = 0..5, k = 0..3),
- *                     limiter partials -> own bar and the bars sent to the +/- neighbours
- *   k_adj_recv_stage  per cell: own bar + what its neighbours sent (no atomics),
+ *                     limiter partials -> own bar and the bars sent to the +/- neighbours;
+ *                     over 512-cell blocks, a bar for a same-level neighbour of the block
+ *                     handed over in LDS, the others written out
+ *   k_adj_recv_stage  per cell: own bar + what its other neighbours sent (no atomics),
  *                     then the stage's reverse update of lam and lamSN
  * Stage states are recomputed from checkpoints every `ckpt` steps, written
  * straight into a segment store that the reverse sweep reads in place.
