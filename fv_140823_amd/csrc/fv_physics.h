@@ -46,6 +46,23 @@ template <> FV_HD __forceinline__ float qsqrt<float>(float a) {
 #endif
 }
 
+/* A reciprocal for the reverse pass's derivative factors only (never for a value the
+ * forward computes, whose bits the adjoint must reproduce): FP64 from the hardware
+ * reciprocal and one Newton step (e = 1 - x r, r + r e: the error squared, within an ulp
+ * or two), FP32 the hardware reciprocal itself; the host divides. */
+template <typename T> FV_HD __forceinline__ T qrcp(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+  } else {
+    return __builtin_amdgcn_rcpf(x);
+  }
+#else
+  return (T)1 / x;
+#endif
+}
+
 struct DevParams {
   int model, ns, ischeme, limiter, visc_work, neq;
   double gamma, pinf, mu, ymin;
@@ -464,11 +481,11 @@ template <typename T>
 FV_HD __forceinline__ void state_ideal_vjp(const DevParams &P, const FState<T> &s, T rb, T ub[3],
                                            T eb, T pb, T cb, T *Ub) {
   const T g = (T)P.gamma, pinf = (T)P.pinf;
-  /* the forward's ir = 1/r (make_state_ideal); the bars multiply by it */
-  const T ir = qdiv((T)1.0, s.r);
+  /* the forward's 1/r (make_state_ideal); the bars multiply by it */
+  const T ir = qrcp(s.r);
   /* c = sqrt(q), q = g (p + pinf) ir */
   const T q = g * (s.p + pinf) * ir;
-  const T qb = qdiv(cb, (T)2 * s.c);
+  const T qb = cb * qrcp((T)2 * s.c);
   pb += qb * g * ir;
   rb -= qb * q * ir;
   /* p = r (g - 1) ei - g pinf, ei = e ir - |u|^2 / 2 */
@@ -534,7 +551,7 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
      * differences from the forward's quotients do not matter */
     const T W = Kr * (SK - uK);
     const T B = SK - SS;
-    const T iKr = qdiv((T)1, Kr), iW = qdiv((T)1, W), iB = qdiv((T)1, B);
+    const T iKr = qrcp(Kr), iW = qrcp(W), iB = qrcp(B);
     const T cs = W * iB;
     const T Y = SS + Kp * iW;
     const T X = Ke * iKr + (SS - uK) * Y;
@@ -607,7 +624,7 @@ FV_HD __forceinline__ void hllc_vjp_ideal(const DevParams &P, const T *UL, const
   if (region == 2) SLb += SKb;
   else if (region == 3) SRb += SKb;
   /* S* = num / den */
-  const T iden = qdiv((T)1, den);
+  const T iden = qrcp(den);
   const T numb = SSb * iden;
   const T denb = -SSb * SS * iden;
   Lrb += numb * (uL * uL);
@@ -662,7 +679,7 @@ FV_HD __forceinline__ double limited_d(int lim, double eps, bool bnd, double d0,
   const double b = isinf(ip) ? 0.0 : d0 * ip;
   const double N = a * (b * b + eps) + b * (a * a + eps);
   const double D = a * a + b * b + 2.0 * eps;
-  const double iD = 1.0 / D;
+  const double iD = qrcp(D);
   const double ga = ((b * b + eps) + 2.0 * a * b) * iD - N * 2.0 * a * (iD * iD);
   const double gb = ((a * a + eps) + 2.0 * a * b) * iD - N * 2.0 * b * (iD * iD);
   if (!isinf(ip)) g0 = gb * ip;
