@@ -810,14 +810,19 @@ static double *seg_state(fv_adj_state *a, int sg, int m, int k) {
   return a->seg + (size_t)(3 * m + k) * a->plane;
 }
 
-/* the SSP-RK3 steps n0 .. n1-1 of segment sg from its checkpoint into the store */
-static void seg_forward(fvg_ctx *c, int sg, int n0, int n1, bool objective) {
+/* the SSP-RK3 steps n0 .. n1-1 of segment sg from its checkpoint into the store.
+ * The reverse sweep of a segment reads the stage inputs S^n, S1, S2 of its steps; the
+ * state after its last step is the next checkpoint, so a recomputation (`end` false)
+ * skips that step's third stage, except in the last segment, whose final state carries
+ * the objective's last term. */
+static void seg_forward(fvg_ctx *c, int sg, int n0, int n1, bool objective, bool end) {
   fv_adj_state *a = c->adj;
   for (int n = n0; n < n1; ++n) {
     const int m = n - n0;
     double *X0 = seg_state(a, sg, m, 0);
     fv_stage_ptrs(c, X0, X0, seg_state(a, sg, m, 1), 1, a->dt);
     fv_stage_ptrs(c, seg_state(a, sg, m, 1), X0, seg_state(a, sg, m, 2), 2, a->dt);
+    if (n == n1 - 1 && !end) break;
     fv_stage_ptrs(c, seg_state(a, sg, m, 2), X0, seg_state(a, sg, m + 1, 0), 3, a->dt);
     if (objective) obj_step(c, seg_state(a, sg, m + 1, 0), nullptr, n);
   }
@@ -899,7 +904,7 @@ int fvg_adj_forward(fvg_ctx *c, int nsteps, int ckpt, double dt, double *J) {
   }
   HIPCHK(c, hipMemcpyAsync(a->ckpts + (size_t)(nck - 1) * a->plane, c->S, pl,
                            hipMemcpyDeviceToDevice, c->stream));
-  seg_forward(c, nck - 1, nlast, nsteps, true);
+  seg_forward(c, nck - 1, nlast, nsteps, true, true);
   a->seg_valid = nck - 1;
   HIPCHK(c, hipMemcpyAsync(c->S, seg_state(a, nck - 1, nsteps - nlast, 0), pl,
                            hipMemcpyDeviceToDevice, c->stream));
@@ -939,7 +944,7 @@ int fvg_adj_backward(fvg_ctx *c, double *grad_host, void *grad_dev) {
   for (int sg = a->nck - 1; sg >= 0; --sg) {
     const int n0 = sg * a->ckpt, n1 = std::min(n0 + a->ckpt, a->nsteps);
     if (a->seg_valid != sg) {
-      seg_forward(c, sg, n0, n1, false);
+      seg_forward(c, sg, n0, n1, false, sg == a->nck - 1);
       a->seg_valid = sg;
     }
     for (int n = n1 - 1; n >= n0; --n) {

@@ -341,10 +341,12 @@ __device__ __forceinline__ void face_fused_body(const DevMesh &M, const DevParam
 }
 
 template <int MODEL, bool NS, typename T, typename R, int OPT = 0>
-/* occupancy targets: 6 waves per SIMD for the Euler flux (78-80 VGPRs, no
- * spill); the viscous variants need 96-98 VGPRs and ask for fewer waves instead
- * of spilling.  OPT: option bits (primitive MUSCL, PVRS, heat conduction). */
-__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : (OPT ? 4 : 6)) void k_face_fused(
+/* occupancy targets: 6 waves per SIMD for the ideal-gas Euler flux (78-80 VGPRs, no
+ * spill; at 7 waves it spills 22 VGPRs and runs 3% slower), 7 for the barotropic one
+ * (72 VGPRs, 2 spilled, 1% faster than at 6); the viscous variants need 96-98 VGPRs
+ * and ask for fewer waves instead of spilling.  OPT: option bits (primitive MUSCL,
+ * PVRS, heat conduction). */
+__global__ __launch_bounds__(FV_BLOCK, NS ? (sizeof(T) == 4 ? 5 : 4) : (OPT ? 4 : (MODEL == 2 ? 7 : 6))) void k_face_fused(
     DevMesh M, DevParams P, int n0, int n1, int n2, int r0, int r1, int r2,
     const double *__restrict__ S, const R *__restrict__ SGm, R *__restrict__ G,
     int *__restrict__ err, const uint8_t *__restrict__ skip) {
