@@ -388,7 +388,10 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
  * bitwise the same.  Halo slot
  * of face f: 512 + f x 64 + u x 8 + w, (u, w) the other two coordinates in axis order. */
 template <int MODEL, typename R, bool HEAT, typename SS = double>
-__global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
+/* FP32 state (the mixed form): the rows are staged in FP32 (each value converts exactly
+ * to the double the arithmetic uses), 25 KB of LDS and at most 64 VGPRs, so that four
+ * workgroups share a CU instead of three */
+__global__ __launch_bounds__(TT, sizeof(SS) == 4 ? 4 : 1) void k_tile_grad(DevMesh M, DevParams P,
                                                   const int32_t *__restrict__ tiles,
                                                   const SS *__restrict__ S,
                                                   R *__restrict__ SGm) {
@@ -396,14 +399,14 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
   constexpr int NV = HEAT ? NEQ : 3; /* staged rows: all (theta needs them) or the momenta */
   constexpr int V0 = HEAT ? 0 : 1;   /* first staged variable */
   constexpr int HR = TT + 384;
-  __shared__ double reg[NV * HR];
+  __shared__ SS reg[NV * HR];
   __shared__ double th[HEAT ? HR : 1];
   const int t = threadIdx.x;
   const int32_t *row = tiles + (size_t)logical_block() * 8;
   const TileLinks tb{row[0], row[1], row[2], row[3], row[4], row[5], row[6]};
   const int L = row[7];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) reg[v * HR + t] = (double)S[(size_t)(tb.base + t) * NEQ + V0 + v];
+  for (int v = 0; v < NV; ++v) reg[v * HR + t] = S[(size_t)(tb.base + t) * NEQ + V0 + v];
   if (t < 384) {
     const int f = t >> 6, u = (t >> 3) & 7, w = t & 7;
     int cell;
@@ -416,17 +419,17 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
       default: cell = tb.pz + tmorton(u, w, 0); break;
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v) reg[v * HR + TT + t] = (double)S[(size_t)cell * NEQ + V0 + v];
+    for (int v = 0; v < NV; ++v) reg[v * HR + TT + t] = S[(size_t)cell * NEQ + V0 + v];
   }
   __syncthreads();
   if constexpr (HEAT) {
     double U[NEQ];
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) U[v] = reg[v * HR + t];
+    for (int v = 0; v < NEQ; ++v) U[v] = (double)reg[v * HR + t];
     th[t] = theta_ideal(P, U);
     if (t < 384) {
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) U[v] = reg[v * HR + TT + t];
+      for (int v = 0; v < NEQ; ++v) U[v] = (double)reg[v * HR + TT + t];
       th[TT + t] = theta_ideal(P, U);
     }
     __syncthreads();
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
   const double V = M.vol[L];
   double U[4];
 #pragma unroll
-  for (int v = 1; v < 4; ++v) U[v] = reg[(v - V0) * HR + t];
+  for (int v = 1; v < 4; ++v) U[v] = (double)reg[(v - V0) * HR + t];
   const double town = HEAT ? th[t] : 0.0;
   const int i = tb.base + t;
   double gm[9]; /* gm[3 i + d] = d(rho u_i)/dx_d */
@@ -466,7 +469,7 @@ __global__ __launch_bounds__(TT) void k_tile_grad(DevMesh M, DevParams P,
       const double wgt = nd * A / V;
       const int r = nslot(f);
 #pragma unroll
-      for (int v = 1; v < 4; ++v) acc[v] += 0.5 * (reg[(v - V0) * HR + r] + U[v]) * wgt;
+      for (int v = 1; v < 4; ++v) acc[v] += 0.5 * ((double)reg[(v - V0) * HR + r] + U[v]) * wgt;
       if constexpr (HEAT) acct += 0.5 * (th[r] + town) * wgt;
     }
     if constexpr (HEAT) SGm[((size_t)VROW_HEAT + d) * M.ns + i] = (R)acct;
