@@ -200,6 +200,22 @@ __device__ __forceinline__ float muscl_regular_f(int lim, double eps, float own,
   return own + fp * limited<float>(lim, eps, false, d0, d1, ic, ic);
 }
 
+/* The update of SSP-RK stage irk (RK.c:24-47): S_k = a SN + b S + c dt R / Vol.  Every
+ * kernel that applies it (the fused gather, the tiled stage, k_rk) calls this one
+ * sequence of roundings with no fused multiply-adds, so that they give the same bits
+ * whatever code surrounds them (the host restatement rounds the same way). */
+__device__ __forceinline__ double rk_update(int irk, double s, double sn, double r, double dt,
+                                            double Vol) {
+#pragma clang fp contract(off)
+  switch (irk) {
+    case 1: return sn + dt * (1.0 / Vol) * r;
+    case 2: return (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol;
+    case 3: return (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol;
+    case 4: return (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol;
+    default: return s;
+  }
+}
+
 /* XCD-aware block order: consecutive logical blocks (contiguous Morton ranges
  * of cells or faces) land on the same XCD and so share its L2. */
 __device__ __forceinline__ int logical_block() {

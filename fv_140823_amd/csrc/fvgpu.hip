@@ -783,16 +783,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_gather_rk(DevMesh M, const R *__re
         const double s = shS[o];
         double sn;
         if constexpr (FIRST) { sn = s; shN[o] = s; } else { sn = shN[o]; }
-        const double r = rhs[v];
-        double out;
-        switch (irk) {
-          case 1: out = sn + dt * (1.0 / Vol) * r; break;
-          case 2: out = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
-          case 3: out = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
-          case 4: out = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
-          default: out = s; break;
-        }
-        shS[o] = out;
+        shS[o] = rk_update(irk, s, sn, rhs[v], dt, Vol);
       }
     } else if constexpr (WRITE_RHS) {
 #pragma unroll
@@ -836,14 +827,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_rk(DevMesh M, double *__restrict__
   for (int v = 0; v < NEQ; ++v) {
     const size_t o = (size_t)i * NEQ + v;
     const double s = S[o];
-    switch (irk) {
-      case 0: SN[o] = s; break;
-      case 1: S[o] = SN[o] + dt * (1.0 / Vol) * RHS[o]; break;
-      case 2: S[o] = (3.0 / 4.0) * SN[o] + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * RHS[o] / Vol; break;
-      case 3: S[o] = (1.0 / 3.0) * SN[o] + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * RHS[o] / Vol; break;
-      case 4: S[o] = (1.0 / 2.0) * SN[o] + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * RHS[o] / Vol; break;
-      default: break;
-    }
+    if (irk == 0) SN[o] = s;
+    else if (irk >= 1 && irk <= 4) S[o] = rk_update(irk, s, SN[o], RHS[o], dt, Vol);
   }
 }
 

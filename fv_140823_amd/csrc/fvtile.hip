@@ -293,10 +293,9 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   using ST = typename std::conditional<FAST, float, double>::type;
   /* one LDS buffer: region [NEQ][TREG] (ST), face fluxes [NEQ][TFB] (R), sums [NEQ][TT]
-   * (double); region and fluxes are also the staging span of the stage update */
+   * (double) */
   constexpr int B_REG = (int)sizeof(ST) * NEQ * TREG, B_FB = (int)sizeof(R) * NEQ * TFB;
   static_assert(B_REG % 16 == 0 && B_FB % 16 == 0, "LDS carve alignment");
-  static_assert(B_REG + B_FB >= 8 * NEQ * TT, "staging span");
   __shared__ __align__(16) unsigned char smem[B_REG + B_FB + 8 * NEQ * TT];
   ST *reg = (ST *)smem;
   R *fb = (R *)(smem + B_REG);
@@ -333,51 +332,23 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
     bad |= (rhs[v] != rhs[v]);
   }
   if (bad) atomicMin(err, tb.base + t);
-  /* 3. the stage update of the tile's 512 consecutive cells, S / SN / Sout moved as one
-   * contiguous span through LDS (region and flux buffers are free now) */
-  double own[NEQ];
-  double *sh = (double *)(void *)smem;
-  if constexpr (FAST) {
-    __syncthreads();
-    for (int k = t; k < TT * NEQ; k += TT) sh[k] = S[(size_t)tb.base * NEQ + k];
-    __syncthreads();
+  /* 3. the stage update of the tile's 512 consecutive cells: each thread its own cell's
+   * rows of S, SN and Sout (a wave's five loads of a row cover one contiguous span) */
+  const size_t crow = (size_t)(tb.base + t) * NEQ;
+  double own[NEQ], sn[NEQ];
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) own[v] = sh[t * NEQ + v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) own[v] = reg[v * TREG + t];
+  for (int v = 0; v < NEQ; ++v) {
+    if constexpr (FAST) own[v] = S[crow + v];
+    else own[v] = reg[v * TREG + t];
+    sn[v] = SN[crow + v];
   }
-  __syncthreads();
-  for (int k = t; k < TT * NEQ; k += TT) sh[k] = SN[(size_t)tb.base * NEQ + k];
-  __syncthreads();
   const double Vol = M.vol[L];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    const double s = own[v], sn = sh[t * NEQ + v], r = rhs[v];
-    double o;
-    switch (irk) {
-      case 1: o = sn + dt * (1.0 / Vol) * r; break;
-      case 2: o = (3.0 / 4.0) * sn + (1.0 / 4.0) * s + (1.0 / 4.0) * dt * r / Vol; break;
-      case 3: o = (1.0 / 3.0) * sn + (2.0 / 3.0) * s + (2.0 / 3.0) * dt * r / Vol; break;
-      case 4: o = (1.0 / 2.0) * sn + (1.0 / 2.0) * s + (1.0 / 2.0) * dt * r / Vol; break;
-      default: o = s; break;
-    }
-    own[v] = o;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) sh[t * NEQ + v] = own[v];
-  __syncthreads();
-  for (int k = t; k < TT * NEQ; k += TT) {
-    Sout[(size_t)tb.base * NEQ + k] = sh[k];
-    if (SfOut) SfOut[(size_t)tb.base * NEQ + k] = (float)sh[k];
-  }
-  if (write_rhs) {
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) sh[t * NEQ + v] = rhs[v];
-    __syncthreads();
-    for (int k = t; k < TT * NEQ; k += TT) RHS[(size_t)tb.base * NEQ + k] = sh[k];
+    const double o = rk_update(irk, own[v], sn[v], rhs[v], dt, Vol);
+    Sout[crow + v] = o;
+    if (SfOut) SfOut[crow + v] = (float)o;
+    if (write_rhs) RHS[crow + v] = rhs[v];
   }
 }
 
