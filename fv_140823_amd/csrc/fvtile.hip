@@ -222,33 +222,55 @@ __device__ __forceinline__ void tile_face(const DevMesh &M, const DevParams &P, 
   if (!ok) atomicMin(err, iL);
 }
 
-/* the two slabs of axis a (256 cells) into the region's slab slots */
+/* the two slabs of axis a (256 cells, one per thread t < 256): fetched into registers,
+ * then stored into the region's slab slots */
 template <int a, bool FAST, int NEQ, typename ST>
-__device__ __forceinline__ void load_slabs(const TileLinks &tb, const double *__restrict__ S,
-                                           const float *__restrict__ Sf, ST *reg, int t) {
+__device__ __forceinline__ void fetch_slabs(const TileLinks &tb, const double *__restrict__ S,
+                                            const float *__restrict__ Sf, ST *val, int t) {
   if (t < 256) {
     const int side = t >> 7, d = (t >> 6) & 1, u = (t >> 3) & 7, w = t & 7;
     const int cell = reg_cell<a>(tb, side ? 8 + d : -1 - d, u, w);
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
-      if constexpr (FAST) reg[v * TREG + 512 + t] = Sf[(size_t)cell * NEQ + v];
-      else reg[v * TREG + 512 + t] = S[(size_t)cell * NEQ + v];
+      if constexpr (FAST) val[v] = Sf[(size_t)cell * NEQ + v];
+      else val[v] = S[(size_t)cell * NEQ + v];
     }
   }
+}
+
+template <int NEQ, typename ST>
+__device__ __forceinline__ void store_slabs(const ST *val, ST *reg, int t) {
+  if (t < 256) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) reg[v * TREG + 512 + t] = val[v];
+  }
+}
+
+template <int a, bool FAST, int NEQ, typename ST>
+__device__ __forceinline__ void load_slabs(const TileLinks &tb, const double *__restrict__ S,
+                                           const float *__restrict__ Sf, ST *reg, int t) {
+  ST val[NEQ];
+  fetch_slabs<a, FAST, NEQ, ST>(tb, S, Sf, val, t);
+  store_slabs<NEQ, ST>(val, reg, t);
 }
 
 /* One axis: the slabs of a are in the region.  The -a face of every cell (fb slot = its
  * local index) and the +a faces on the tile's +a side (slots 512 + u x 8 + w, by the
  * first wave) into fb; then each cell's -a and +a terms into its sum acc (the gather's
  * terms: 0 + the signed flux, added to the sum). */
-template <int MODEL, bool NS, int a, typename T, typename R, int OPT, bool FAST, typename ST>
+template <int MODEL, bool NS, int a, int NA, typename T, typename R, int OPT, bool FAST, typename ST>
 __device__ __forceinline__ void tile_pass(const DevMesh &M, const DevParams &P, const TileLinks &tb,
-                                          int L, const ST *reg, R *fb, double *acc,
-                                          const double *__restrict__ S, const R *__restrict__ SGm,
+                                          int L, ST *reg, R *fb, double *acc,
+                                          const double *__restrict__ S, const float *__restrict__ Sf,
+                                          const R *__restrict__ SGm,
                                           int t, int cx, int cy, int cz, int *__restrict__ err) {
   constexpr int NEQ = ModelTraits<MODEL>::NEQ;
   const int c = a == 0 ? cx : (a == 1 ? cy : cz);
   const int u = a == 0 ? cy : cx, w = a == 2 ? cy : cz;
+  /* the next axis's slabs (NA >= 0) are fetched before this axis's faces and stored once
+   * they are done, so that their loads are in flight during the face arithmetic */
+  ST nx[NEQ];
+  if constexpr (NA >= 0) fetch_slabs<NA, FAST, NEQ, ST>(tb, S, Sf, nx, t);
   R out[NEQ];
   tile_face<MODEL, NS, a, T, R, OPT, FAST, ST>(M, P, tb, L, reg, S, SGm,
                                                own_face_idx<a>(tb, t, c, u * 8 + w), out, err);
@@ -261,6 +283,7 @@ __device__ __forceinline__ void tile_pass(const DevMesh &M, const DevParams &P, 
     for (int v = 0; v < NEQ; ++v) fb[v * TFB + 512 + t] = out[v];
   }
   __syncthreads();
+  if constexpr (NA >= 0) store_slabs<NEQ, ST>(nx, reg, t);
   const int sp = c < 7 ? tmorton(cx + (a == 0), cy + (a == 1), cz + (a == 2)) : 512 + u * 8 + w;
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -317,13 +340,11 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   demorton(t, cx, cy, cz);
   /* 2. faces axis by axis, each cell's sum in the gather's order (-x, +x, -y, +y, -z,
    * +z); the next axis's slabs replace this one's once its faces are done */
-  tile_pass<MODEL, NS, 0, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
-  load_slabs<1, FAST, NEQ, ST>(tb, S, Sf, reg, t);
+  tile_pass<MODEL, NS, 0, 1, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, Sf, SGm, t, cx, cy, cz, err);
   __syncthreads();
-  tile_pass<MODEL, NS, 1, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
-  load_slabs<2, FAST, NEQ, ST>(tb, S, Sf, reg, t);
+  tile_pass<MODEL, NS, 1, 2, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, Sf, SGm, t, cx, cy, cz, err);
   __syncthreads();
-  tile_pass<MODEL, NS, 2, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, SGm, t, cx, cy, cz, err);
+  tile_pass<MODEL, NS, 2, -1, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, Sf, SGm, t, cx, cy, cz, err);
   double rhs[NEQ];
   bool bad = false;
 #pragma unroll
