@@ -521,7 +521,7 @@ def main():
     TILING = a.tiling
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the path has no "
-              "multi-GPU halo exchange; the sharding is described in DESIGN.md", file=sys.stderr)
+              "multi-GPU halo exchange", file=sys.stderr)
         sys.exit(2)
     import torch
 
