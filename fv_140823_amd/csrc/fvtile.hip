@@ -344,6 +344,15 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   __syncthreads();
   tile_pass<MODEL, NS, 1, 2, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, Sf, SGm, t, cx, cy, cz, err);
   __syncthreads();
+  /* FP64 (the mixed kernel has no registers to spare for it): the update's SN row is
+   * fetched before the last pass, in the registers the slab prefetch of the other passes
+   * uses, so that its loads are in flight during it */
+  const size_t crow = (size_t)(tb.base + t) * NEQ;
+  double sn[NEQ];
+  if constexpr (!FAST) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) sn[v] = SN[crow + v];
+  }
   tile_pass<MODEL, NS, 2, -1, T, R, OPT, FAST, ST>(M, P, tb, L, reg, fb, acc, S, Sf, SGm, t, cx, cy, cz, err);
   double rhs[NEQ];
   bool bad = false;
@@ -355,13 +364,15 @@ __global__ __launch_bounds__(TT, FAST ? 6 : 4) void k_tile_stage(DevMesh M, DevP
   if (bad) atomicMin(err, tb.base + t);
   /* 3. the stage update of the tile's 512 consecutive cells: each thread its own cell's
    * rows of S, SN and Sout (a wave's five loads of a row cover one contiguous span) */
-  const size_t crow = (size_t)(tb.base + t) * NEQ;
-  double own[NEQ], sn[NEQ];
+  double own[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
-    if constexpr (FAST) own[v] = S[crow + v];
-    else own[v] = reg[v * TREG + t];
-    sn[v] = SN[crow + v];
+    if constexpr (FAST) {
+      own[v] = S[crow + v];
+      sn[v] = SN[crow + v];
+    } else {
+      own[v] = reg[v * TREG + t];
+    }
   }
   const double Vol = M.vol[L];
 #pragma unroll
