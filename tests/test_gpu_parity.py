@@ -177,6 +177,9 @@ def test_rk_stage_formula(irk):
     for which in (fv.FIELD_S, fv.FIELD_SN):
         e = rel_linf(g.get_field(which), F.get_field(which))
         assert np.all(e <= 1e-15), (which, e)
+        # the device update rounds each operation as the restatement does (no fused
+        # multiply-add on either side), so the two agree bit for bit
+        assert np.array_equal(g.get_field(which), F.get_field(which)), (which, e)
 
 
 def test_steps_parity_amr():
