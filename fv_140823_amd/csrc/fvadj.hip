@@ -267,8 +267,10 @@ __global__ __launch_bounds__(ADJ_XB) void k_adj_recon(DevMesh M, DevParams P,
     for (int v = 0; v < 5; ++v) self[(size_t)v * M.ns + i] = acc[v];
     return;
   }
-  /* the bars sent to the + and - neighbour of the current axis, by thread */
-  __shared__ double sP[5][ADJ_XB], sM[5][ADJ_XB];
+  /* the bars sent to the + and - neighbour of the current axis, by thread; two sets,
+   * taken by axis parity, so that an axis's writes never meet the previous axis's reads
+   * and one barrier per axis suffices (80 KB: two blocks per CU) */
+  __shared__ double sPb[2][5][ADJ_XB], sMb[2][5][ADJ_XB];
   const uint32_t cf = live ? M.cflag[i] : 0u;
   double U[5] = {0, 0, 0, 0, 0};
   int n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0, s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1,
@@ -288,6 +290,8 @@ __global__ __launch_bounds__(ADJ_XB) void k_adj_recon(DevMesh M, DevParams P,
    * otherwise be interleaved and cost the kernel most of its occupancy */
 #pragma unroll 1
   for (int ax = 0; ax < 3; ++ax) {
+    double(*sP)[ADJ_XB] = sPb[ax & 1];
+    double(*sM)[ADJ_XB] = sMb[ax & 1];
     const int fp = ax == 0 ? 1 : (ax == 1 ? 2 : 5);
     const int fm = ax == 0 ? 3 : (ax == 1 ? 0 : 4);
     const int np = ax == 0 ? n1 : (ax == 1 ? n2 : n5), nm = ax == 0 ? n3 : (ax == 1 ? n0 : n4);
@@ -338,7 +342,6 @@ __global__ __launch_bounds__(ADJ_XB) void k_adj_recon(DevMesh M, DevParams P,
 #pragma unroll
       for (int v = 0; v < 5; ++v) acc[v] += sM[v][q];
     }
-    __syncthreads();
   }
   if (live) {
 #pragma unroll
